@@ -1,0 +1,230 @@
+/*
+ * kgmt.h — C-ABI of the MI355X-native KGMT expansion path.
+ *
+ * Plain C types only (no HIP, no torch): pointers, sizes and status codes.
+ * Every function returns KGMT_OK (0) or a negative KGMT_E_* code; the text
+ * of the last error on the calling thread is returned by kgmt_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the KGMT repository):
+ *   kgmt_config            include/config/config.h:209-307 (compile-time macros
+ *                          MODEL, MAX_TREE_SIZE, MAX_ITER, STEP_SIZE, GOAL_THRESH,
+ *                          W_MIN/W_MAX, V_MIN/V_MAX, W_R1_LENGTH, W_R2_LENGTH,
+ *                          EPSILON, quad parameters) become one runtime struct.
+ *   kgmt_create            KGMT::KGMT            src/planners/KGMT.cu:4-39
+ *                          Planner::Planner      src/planners/Planner.cu:4-35
+ *                          Graph::Graph          src/graphs/Graph.cu:5-42
+ *   kgmt_plan              KGMT::plan            src/planners/KGMT.cu:41-105
+ *                          (virtual Planner::plan, include/planners/Planner.cuh:26)
+ *   kgmt_get_tree          public members d_treeSamples_, d_treeSamplesParentIdxs_,
+ *                          d_treeSampleCosts_    include/planners/Planner.cuh:37-38
+ *   kgmt_get_path          h_controlPathToGoal_  include/planners/Planner.cuh:34,
+ *                          filled at src/planners/KGMT.cu:92
+ *   kgmt_propagate_collide propagateAndCheck*    src/statePropagator/statePropagator.cu:4-370
+ *                          + isMotionValid       src/collisionCheck/collisionCheck.cu:3-10,49-62
+ *                          + getRegion/getSubRegion src/graphs/Graph.cu:99-205
+ *   kgmt_draw_controls     the curand_uniform draws at statePropagator.cu:78-81,
+ *                          149-152, 248-253 and KGMT.cu:254,294,372
+ *   kgmt_region_update     counter/flag section of propagateFrontier_kernel1/2
+ *                          src/planners/KGMT.cu:249-256, 289-296
+ *   kgmt_score             Graph::updateVertices + score kernels src/graphs/Graph.cu:207-358
+ *   kgmt_compact           thrust::exclusive_scan + findInd src/planners/KGMT.cu:170-173,
+ *                          383-385; src/planners/Planner.cu:52-58
+ *   kgmt_repeat            thrust::exclusive_scan + repeatInd src/planners/KGMT.cu:176-181;
+ *                          src/planners/Planner.cu:68-80
+ *   kgmt_batch_*           no single counterpart: many KGMT::plan calls served by
+ *                          one launch sequence per iteration.
+ */
+#ifndef KGMT_H
+#define KGMT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KGMT_ABI_VERSION 1
+
+/* ---- status codes -------------------------------------------------------- */
+enum {
+    KGMT_OK            = 0,
+    KGMT_E_ARG         = -1, /* bad argument (null pointer, size out of range)  */
+    KGMT_E_HIP         = -2, /* a HIP runtime call failed                       */
+    KGMT_E_NOMEM       = -3, /* device or host allocation failed                */
+    KGMT_E_CAPACITY    = -4, /* input larger than the context was created for   */
+    KGMT_E_UNSUPPORTED = -5, /* model / option not built                        */
+    KGMT_E_NODEVICE    = -6  /* no HIP device visible                           */
+};
+
+/* ---- dynamics models (numbering follows config.h MODEL) ------------------ */
+enum {
+    KGMT_MODEL_UNICYCLE          = 0, /* statePropagator.cu:13-71 (not built)  */
+    KGMT_MODEL_DOUBLE_INTEGRATOR = 1, /* statePropagator.cu:76-142             */
+    KGMT_MODEL_DUBINS_AIRPLANE   = 2, /* statePropagator.cu:147-241 (6D, v state) */
+    KGMT_MODEL_QUADCOPTER        = 3, /* statePropagator.cu:246-370            */
+    KGMT_MODEL_DUBINS_5D         = 4  /* 5D Dubins airplane, speed is a control */
+};
+
+/* double-integrator position update */
+enum {
+    KGMT_DI_REFERENCE = 0, /* statePropagator.cu:97-99 literally: x += (5v + 2ah)h/6 */
+    KGMT_DI_EXACT     = 1  /* textbook RK4 of x'' = a: x += v h + a h^2 / 2          */
+};
+
+/* presets for kgmt_config_preset */
+enum {
+    KGMT_PRESET_REFERENCE_QUAD = 0, /* config.h:209-307 as shipped (quad, W 0..100) */
+    KGMT_PRESET_DI6D           = 1, /* 6D double integrator, unit cube, 2^17 nodes  */
+    KGMT_PRESET_DUBINS5D       = 2, /* 5D Dubins airplane, unit cube, 2^18 nodes    */
+    KGMT_PRESET_QUAD12D        = 3  /* 12D quadcopter, unit cube, 2^19 nodes        */
+};
+
+/* Sample layout (one tree node / one row of the path), as in the reference
+ * (statePropagator.cu:139-140, 229-238, 314-320):
+ *   [ state (S) | control (C) | duration ]   D = S + C + 1 floats
+ *   double integrator : S=6  (x y z vx vy vz)        C=3 (ax ay az)
+ *   dubins airplane   : S=6  (x y z yaw pitch v)     C=3 (yawRate pitchRate a)
+ *   quadcopter        : S=12 (x y z phi theta psi u v w p q r) C=4 (Zc Lc Mc Nc)
+ *   dubins 5D         : S=5  (x y z yaw pitch)       C=3 (yawRate pitchRate speed)
+ */
+#define KGMT_MAX_STATE   12
+#define KGMT_MAX_CONTROL 4
+#define KGMT_MAX_SAMPLE  17
+
+typedef struct kgmt_config {
+    int32_t  model;              /* KGMT_MODEL_*                                   */
+    int32_t  integrator;         /* KGMT_DI_* (double integrator only)             */
+    uint32_t capacity;           /* MAX_TREE_SIZE: tree nodes per query            */
+    uint32_t max_iter;           /* MAX_ITER                                       */
+    float    dt;                 /* STEP_SIZE: integration step                    */
+    int32_t  steps_min;          /* edge duration = steps * dt, steps uniform in   */
+    int32_t  steps_max;          /*   [steps_min, steps_max]                       */
+    float    goal_thresh;        /* GOAL_THRESH                                    */
+    float    ws_min, ws_max;     /* W_MIN, W_MAX: workspace cube bounds            */
+    float    v_min, v_max;       /* V_MIN, V_MAX: velocity bounds (DI, dubins, quad body vel) */
+    float    ctrl_min[4];        /* per-control uniform draw range                 */
+    float    ctrl_max[4];
+    float    pitch_min, pitch_max; /* dubins pitch bounds                          */
+    int32_t  r1_len;             /* W_R1_LENGTH: regions per workspace axis        */
+    int32_t  r2_len;             /* W_R2_LENGTH: subregions per region axis        */
+    float    epsilon;            /* EPSILON                                        */
+    uint32_t children_per_entry; /* h_activeBlockSize_ (32)                        */
+    uint32_t root_repeat;        /* repeat count of the root (5)                   */
+    uint32_t repeat_sparse;      /* repeat of a new node in a sparse region (15)   */
+    uint32_t repeat_dense;       /* repeat of a new node otherwise (1)             */
+    int32_t  sparse_valid_below; /* region is sparse while valid count < this (10) */
+    float    quad_mass, quad_ix, quad_iy, quad_iz;
+    float    quad_nu, quad_mu, gravity;
+    uint32_t reserved0;
+    uint64_t seed;               /* replaces the wall-clock seed of KGMT.cu:80-81  */
+} kgmt_config;
+
+typedef struct kgmt_result {
+    int32_t  success;     /* 1 if a node within goal_thresh of the goal was added   */
+    int32_t  goal_index;  /* tree index of that node (h_pathToGoal_), -1 if none     */
+    uint32_t iterations;  /* h_itr_                                                  */
+    uint32_t tree_size;   /* h_treeSize_                                             */
+    uint32_t path_rows;   /* rows of the goal->root path (depth + 1), 0 if none      */
+    float    path_cost;   /* sum of edge durations along the path                    */
+    float    path_length; /* Euclidean length of the path's position polyline        */
+    float    time_ms;     /* wall time of the plan call (host clock)                  */
+    uint64_t expanded;    /* children propagated over all iterations                 */
+} kgmt_result;
+
+typedef struct kgmt_ctx kgmt_ctx;
+typedef struct kgmt_batch kgmt_batch;
+
+/* ---- library ------------------------------------------------------------- */
+int         kgmt_abi_version(void);
+const char* kgmt_last_error(void);
+/* Dimensions of a model: S, C, D = S + C + 1. */
+int kgmt_model_dims(int32_t model, int32_t* state_dim, int32_t* control_dim, int32_t* sample_dim);
+/* Fill *out with one of the KGMT_PRESET_* configurations. */
+int kgmt_config_preset(int32_t preset, kgmt_config* out);
+/* Validate a configuration without touching the device. */
+int kgmt_config_check(const kgmt_config* cfg);
+
+/* ---- single-query planner (KGMT::plan) ---------------------------------- */
+int kgmt_create(const kgmt_config* cfg, kgmt_ctx** out);
+int kgmt_destroy(kgmt_ctx* ctx);
+/* Use an existing hipStream_t for every launch of this context (NULL: own stream). */
+int kgmt_set_stream(kgmt_ctx* ctx, void* hip_stream);
+/* start, goal: host arrays of D floats (only state[0..S) of start and
+ * state[0..3) of goal are read).  obstacles: 6*n_obs floats, one box per row
+ * (xmin ymin zmin xmax ymax zmax); a device pointer when obstacles_on_device
+ * is non-zero (as in the reference), else host memory. Blocking. */
+int kgmt_plan(kgmt_ctx* ctx, const float* start, const float* goal, const float* obstacles,
+              uint32_t n_obs, int32_t obstacles_on_device, kgmt_result* out);
+/* Host copies of the tree of the last plan: samples as rows of D floats
+ * (row-major, like d_treeSamples_), parents (-1 for the root), costs
+ * (distance of the node position to the goal). Any pointer may be NULL.
+ * cap: rows available; *n receives the tree size. */
+int kgmt_get_tree(kgmt_ctx* ctx, float* samples, int32_t* parents, float* costs,
+                  uint32_t cap, uint32_t* n);
+/* Rows goal->root of the last successful plan (D floats each). */
+int kgmt_get_path(kgmt_ctx* ctx, float* rows, uint32_t cap_rows, uint32_t* n_rows);
+/* Per-region counters of the last plan: total[R1], valid[R1], score[R1]. */
+int kgmt_get_regions(kgmt_ctx* ctx, int32_t* total, int32_t* valid, float* score, uint32_t cap,
+                     uint32_t* n_regions);
+
+/* ---- batched planner ----------------------------------------------------- */
+/* n_q queries, each with its own configuration (capacity, model, ...) and its
+ * own obstacle set. obstacle_offsets[q] (in boxes) indexes `obstacles`;
+ * obstacle_counts[q] boxes belong to query q. starts/goals: n_q rows of
+ * KGMT_MAX_SAMPLE floats. All inputs are host memory. */
+int kgmt_batch_create(const kgmt_config* cfgs, uint32_t n_q, kgmt_batch** out);
+int kgmt_batch_destroy(kgmt_batch* b);
+int kgmt_batch_set_stream(kgmt_batch* b, void* hip_stream);
+/* use_graph != 0: one iteration is captured once as a hipGraph and replayed. */
+int kgmt_batch_plan(kgmt_batch* b, const float* starts, const float* goals, const float* obstacles,
+                    const uint32_t* obstacle_offsets, const uint32_t* obstacle_counts,
+                    int32_t use_graph, kgmt_result* out /* n_q */, float* finish_ms /* n_q or NULL */);
+int kgmt_batch_get_path(kgmt_batch* b, uint32_t q, float* rows, uint32_t cap_rows, uint32_t* n_rows);
+int kgmt_batch_get_tree(kgmt_batch* b, uint32_t q, float* samples, int32_t* parents, float* costs,
+                        uint32_t cap, uint32_t* n);
+/* Launch shape of the persistent kernels (0 = autotuned/default). */
+int kgmt_batch_set_launch(kgmt_batch* b, uint32_t propagate_block, uint32_t propagate_grid,
+                          uint32_t chunk_grid);
+
+/* ---- stage-level entry points (device pointers, asynchronous on `stream`) -
+ * These run one stage of an iteration on caller-owned device buffers so that
+ * each stage can be checked on identical inputs. SoA layout: element (d, i)
+ * of an [X][n] array lives at d * n + i. */
+
+/* Philox draws of iteration `itr` for children u = u0 .. u0+n-1 of query
+ * `query`: controls [C][n], steps [n], frontier uniform [n]. */
+int kgmt_draw_controls(const kgmt_config* cfg, uint32_t query, uint32_t itr, uint32_t u0, uint32_t n,
+                       float* d_controls, int32_t* d_steps, float* d_ufront, void* stream);
+/* Propagate + collide: child i starts from parent_states[:, parent_idx[i]]
+ * ([S][n_parents]), applies controls[:, i] for steps[i] steps. Writes child
+ * samples [D][n] (state, control, duration), valid[n], r1[n], r2[n]. */
+int kgmt_propagate_collide(const kgmt_config* cfg, const int32_t* d_parent_idx,
+                           const float* d_parent_states, uint32_t n_parents,
+                           const float* d_controls, const int32_t* d_steps, uint32_t n_children,
+                           const float* d_obstacles, uint32_t n_obs, float* d_child_samples,
+                           uint8_t* d_valid, int32_t* d_r1, int32_t* d_r2, void* stream);
+/* Region bookkeeping of one iteration: total[r1]++ for every child,
+ * valid[r1]++ and active_next[r2]=1 for valid ones, and
+ * fnext[i] = valid && (ufront[i] < score[r1] || !active_prev[r2]).
+ * active_* are bitsets of R2 bits (uint32 words); active_next must hold
+ * active_prev on entry. total/valid accumulate. */
+int kgmt_region_update(const kgmt_config* cfg, const int32_t* d_r1, const int32_t* d_r2,
+                       const uint8_t* d_valid, const float* d_ufront, uint32_t n,
+                       const float* d_score, const uint32_t* d_active_prev, uint32_t* d_active_next,
+                       int32_t* d_total, int32_t* d_valid_count, uint8_t* d_fnext, void* stream);
+/* Region scores from total/valid counters and the subregion bitset. */
+int kgmt_score(const kgmt_config* cfg, const int32_t* d_total, const int32_t* d_valid_count,
+               const uint32_t* d_active, float* d_score, void* stream);
+/* Stream compaction: out[k] = k-th index i (ascending) with flags[i] != 0;
+ * *d_count = number of such indices. */
+int kgmt_compact(const uint8_t* d_flags, uint32_t n, uint32_t* d_out, uint32_t* d_count, void* stream);
+/* Segmented repeat: node ids[i] written counts[i] times, in order, into out;
+ * *d_total = sum of counts. */
+int kgmt_repeat(const uint32_t* d_ids, const uint32_t* d_counts, uint32_t n, uint32_t* d_out,
+                uint32_t out_cap, uint32_t* d_total, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KGMT_H */

@@ -1,0 +1,605 @@
+// kgmt_device.hpp — device math of the KGMT expansion path for gfx950.
+//
+// Everything here is evaluated per lane in registers. The file is compiled with
+// -ffp-contract=off: each float operator is one IEEE operation, in source
+// order, so results match a CPU build of the same expressions bit for bit
+// (division and sqrt are correctly rounded in HIP by default).
+//
+// Reference (paths relative to the KGMT repository):
+//   dynamics      src/statePropagator/statePropagator.cu:76-142, 147-241, 246-370
+//   collision     src/collisionCheck/collisionCheck.cu:3-10, 49-62
+//   regions       src/graphs/Graph.cu:53-97 (min corners), 99-153, 155-205
+//   goal distance include/helper/helper.cuh:138-152
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/kgmt.h"
+
+namespace kgmt
+{
+
+constexpr uint32_t kChunk   = 1024;  // elements per bookkeeping chunk (one workgroup job)
+constexpr uint32_t kTile    = 256;   // children per propagate tile
+constexpr uint32_t kBlock   = 256;   // threads of chunk / tile workgroups
+constexpr uint32_t kMaxObs  = 2048;  // obstacles a query may carry (LDS: 32 B each)
+constexpr int32_t kNoGoal   = 0x7fffffff;
+
+// -------------------------------------------------------------------------
+// Per-query descriptor (read-only during a plan) and mutable state.
+// -------------------------------------------------------------------------
+struct QueryDesc
+{
+    kgmt_config cfg;
+    int32_t S, C, D;                 // state, control, sample dims
+    int32_t R1, P, R2, R2words;      // regions, subregions per region, subregions, bitset words
+    uint32_t cap, nChunks;           // capacity, ceil(cap / kChunk)
+    uint64_t sampleOff;              // float offset of this query's [D][cap] sample blocks
+    uint64_t nodeOff;                // element offset of per-node arrays (cap entries)
+    uint32_t chunkOff;               // offset of per-chunk arrays (nChunks + 1 entries)
+    uint32_t regOff;                 // offset of per-region arrays (R1 entries)
+    uint32_t subOff;                 // offset of subregion bitset words (R2words entries)
+    uint32_t obsOff, nObs;           // first box and box count in the obstacle pool
+    uint32_t pathOff;                // float offset of the path rows ((max_iter+2)*D)
+    float r1size, vol;               // W_R1_SIZE, W_R1_VOL
+    float goal[3];
+    float start[KGMT_MAX_SAMPLE];
+    float qminv, qax, qbx, qay, qby, qaz, qbz;  // quad: 1/m, (Iy-Iz)/Ix, 1/Ix, ...
+    uint32_t pad[3];
+};
+
+struct QueryState
+{
+    uint32_t itr;        // current iteration (1-based)
+    uint32_t treeSize;   // nodes in the tree
+    uint32_t nChildren;  // children to propagate this iteration
+    uint32_t per;        // children per expansion entry
+    uint32_t R;          // size of the expansion multiset
+    uint32_t nReact;     // re-activated old nodes this iteration
+    uint32_t nAcc;       // children flagged for the next frontier
+    uint32_t next;       // children appended (nAcc clamped to capacity)
+    float fAccept;       // re-activation bonus
+    uint32_t done;       // 1 once solved or out of iterations
+    uint32_t success;
+    int32_t goalIdx;     // smallest tree index within goal_thresh (kNoGoal if none)
+    uint32_t pathRows;
+    float pathCost, pathLen;
+    uint32_t jobsDone;   // last-block counter of the append kernel
+    uint64_t expanded;
+    uint32_t finishItr;
+    uint32_t pad[3];
+};
+
+// Pointers of the device pools (shared by all queries of a batch).
+struct Pools
+{
+    float* treeS;        // tree samples        [D][cap] per query
+    float* childS;       // unexplored samples  [D][cap] per query
+    int32_t* parent;     // tree parents
+    float* cost;         // distance of node position to goal
+    uint8_t* frontier;   // frontier flag per tree node
+    int32_t* r1node;     // region of each tree node
+    int32_t* parentU;    // parent of each unexplored child
+    int32_t* r1child;    // region of each unexplored child
+    uint8_t* fnext;      // child flagged for the next frontier
+    uint8_t* react;      // old node re-activated this iteration
+    uint32_t* repU;      // repeat count of each appended child, 0 otherwise
+    uint32_t* repIdx;    // expansion list: node id of multiset entry r
+    uint32_t* partA;     // per child chunk: flagged children, then exclusive offsets
+    uint32_t* partB;     // per old chunk: re-activated nodes, then exclusive offsets
+    uint32_t* partC;     // per child chunk: sum of repU, then exclusive offsets
+    int32_t* total;      // per region: children counted
+    int32_t* validc;     // per region: valid children
+    float* score;        // per region score
+    uint32_t* actPrev;   // subregion bitset as of iteration start
+    uint32_t* actNext;   // subregion bitset being built
+    float4* obs;         // obstacle pool, 2 float4 per box (lo, hi)
+    float* path;         // goal->root rows
+};
+
+// Work lists over queries (prefix sums, nq + 1 entries each).
+struct WorkLists
+{
+    uint32_t* tileStart;   // propagate tiles
+    uint32_t* jobStart;    // append / expand chunk jobs
+    uint32_t* counter;     // last-block counter of the plan kernel
+    uint32_t* allDone;     // 1 once every query is done
+};
+
+// -------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11), published multipliers and Weyl keys.
+// -------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                              uint32_t k1, uint32_t out[4])
+{
+#pragma unroll
+    for(int round = 0; round < 10; ++round)
+        {
+            if(round > 0)
+                {
+                    k0 += 0x9E3779B9u;
+                    k1 += 0xBB67AE85u;
+                }
+            const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+            const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+            const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+            c0 = n0, c1 = lo1, c2 = n2, c3 = lo0;
+        }
+    out[0] = c0, out[1] = c1, out[2] = c2, out[3] = c3;
+}
+
+// uniform in (0, 1]: top 24 bits plus one, times 2^-24
+__device__ __forceinline__ float u01(uint32_t x)
+{
+    return (float)((x >> 8) + 1u) * 5.9604644775390625e-08f;
+}
+
+__device__ __forceinline__ void draw_block(const kgmt_config& c, uint32_t q, uint32_t itr, uint32_t idx,
+                                           uint32_t stream, uint32_t out[4])
+{
+    philox4x32_10(idx, itr, stream, q, (uint32_t)c.seed, (uint32_t)(c.seed >> 32), out);
+}
+
+// Controls (sample order) from words d[0..C), steps from d[C], frontier
+// uniform from d[7]; blocks 0 and 1 of counter (u, itr, *, q).
+template <int C>
+__device__ __forceinline__ void draw_child(const kgmt_config& c, uint32_t q, uint32_t itr, uint32_t u, float* ctrl,
+                                           int32_t& steps, float& ufront)
+{
+    uint32_t d[8];
+    draw_block(c, q, itr, u, 0, d);
+    draw_block(c, q, itr, u, 1, d + 4);
+#pragma unroll
+    for(int i = 0; i < C; ++i)
+        {
+            const float range = c.ctrl_max[i] - c.ctrl_min[i];
+            ctrl[i]           = c.ctrl_min[i] + u01(d[i]) * range;
+        }
+    const int32_t n = c.steps_max - c.steps_min + 1;
+    int32_t k       = (int32_t)(u01(d[C]) * (float)n);
+    if(k > n - 1) k = n - 1;
+    steps  = c.steps_min + k;
+    ufront = u01(d[7]);
+}
+
+__device__ __forceinline__ float draw_react(const kgmt_config& c, uint32_t q, uint32_t itr, uint32_t node)
+{
+    uint32_t d[4];
+    draw_block(c, q, itr, node, 2, d);
+    return u01(d[0]);
+}
+
+// -------------------------------------------------------------------------
+// sin/cos: Cody–Waite reduction by pi/2 (split constants with 8 and 11
+// significant bits, so k*P1, k*P2 are exact for |k| < 2^13) and Taylor
+// polynomials (sin to x^9, cos to x^12) on [-pi/4, pi/4].
+// -------------------------------------------------------------------------
+__device__ __forceinline__ void sincos_k(float x, float& s, float& c)
+{
+    const float fk = __builtin_rintf(x * 0x1.45f306p-1f);
+    float r        = x - fk * 0x1.92p+0f;
+    r              = r - fk * 0x1.fb4p-12f;
+    r              = r - fk * 0x1.4442d2p-24f;
+    const int q    = (int)fk;
+    const float z  = r * r;
+    float ps       = 0x1.71de3ap-19f;
+    ps             = ps * z + -0x1.a01a02p-13f;
+    ps             = ps * z + 0x1.111112p-7f;
+    ps             = ps * z + -0x1.555556p-3f;
+    const float sr = r + (r * z) * ps;
+    float pc       = 0x1.1eed8ep-29f;
+    pc             = pc * z + -0x1.27e4fcp-22f;
+    pc             = pc * z + 0x1.a01a02p-16f;
+    pc             = pc * z + -0x1.6c16c2p-10f;
+    pc             = pc * z + 0x1.555556p-5f;
+    const float cr = (1.0f - 0.5f * z) + (z * z) * pc;
+    const int m    = q & 3;
+    const float a  = (m & 1) ? cr : sr;
+    const float b  = (m & 1) ? sr : cr;
+    s              = (m & 2) ? -a : a;
+    c              = ((m + 1) & 2) ? -b : b;
+}
+
+// -------------------------------------------------------------------------
+// Collision: segment AABB vs boxes staged in LDS as (lo, hi) float4 pairs.
+// A box is hit unless some axis separates (bbMax <= lo or hi <= bbMin),
+// exactly the reference's test (NaN coordinates count as a hit, like it).
+// -------------------------------------------------------------------------
+__device__ __forceinline__ bool segment_free(float ax, float ay, float az, float bx, float by, float bz,
+                                             const float4* __restrict__ sObs, int nObs)
+{
+    const float mnx = ax > bx ? bx : ax, mxx = ax > bx ? ax : bx;
+    const float mny = ay > by ? by : ay, mxy = ay > by ? ay : by;
+    const float mnz = az > bz ? bz : az, mxz = az > bz ? az : bz;
+    bool hit = false;
+    for(int o = 0; o < nObs; ++o)
+        {
+            const float4 lo = sObs[2 * o];
+            const float4 hi = sObs[2 * o + 1];
+            const bool sep  = (mxx <= lo.x) | (hi.x <= mnx) | (mxy <= lo.y) | (hi.y <= mny) | (mxz <= lo.z) |
+                             (hi.z <= mnz);
+            hit |= !sep;
+        }
+    return !hit;
+}
+
+__device__ __forceinline__ bool ws_out(const kgmt_config& c, float x, float y, float z)
+{
+    return x < c.ws_min || x > c.ws_max || y < c.ws_min || y > c.ws_max || z < c.ws_min || z > c.ws_max;
+}
+
+// -------------------------------------------------------------------------
+// Dynamics. s[] holds the state on entry and the final (break-point) state on
+// exit; returns validity of the whole edge.
+// -------------------------------------------------------------------------
+template <int MODEL>
+struct ModelDims;
+template <>
+struct ModelDims<KGMT_MODEL_DOUBLE_INTEGRATOR>
+{
+    static constexpr int S = 6, C = 3;
+};
+template <>
+struct ModelDims<KGMT_MODEL_DUBINS_AIRPLANE>
+{
+    static constexpr int S = 6, C = 3;
+};
+template <>
+struct ModelDims<KGMT_MODEL_QUADCOPTER>
+{
+    static constexpr int S = 12, C = 4;
+};
+template <>
+struct ModelDims<KGMT_MODEL_DUBINS_5D>
+{
+    static constexpr int S = 5, C = 3;
+};
+
+// statePropagator.cu:76-142
+__device__ __forceinline__ bool prop_di(const QueryDesc& qd, float* s, const float* u, int steps,
+                                        const float4* sObs, int nObs)
+{
+    const kgmt_config& c = qd.cfg;
+    const float h        = c.dt;
+    const float ax = u[0], ay = u[1], az = u[2];
+    float x = s[0], y = s[1], z = s[2], vx = s[3], vy = s[4], vz = s[5];
+    const bool exact = c.integrator == KGMT_DI_EXACT;
+    bool valid       = true;
+    for(int i = 0; i < steps; ++i)
+        {
+            const float px = x, py = y, pz = z;
+            if(exact)
+                {
+                    x  = x + (vx * h + ((ax * h) * h) / 2.0f);
+                    y  = y + (vy * h + ((ay * h) * h) / 2.0f);
+                    z  = z + (vz * h + ((az * h) * h) / 2.0f);
+                    vx = vx + ax * h;
+                    vy = vy + ay * h;
+                    vz = vz + az * h;
+                }
+            else
+                {
+                    x  = x + (vx + (vx + 2.0f * (vx + ax * h / 2.0f) + (vx + ax * h))) * h / 6.0f;
+                    y  = y + (vy + (vy + 2.0f * (vy + ay * h / 2.0f) + (vy + ay * h))) * h / 6.0f;
+                    z  = z + (vz + (vz + 2.0f * (vz + az * h / 2.0f) + (vz + az * h))) * h / 6.0f;
+                    vx = vx + (ax + 2.0f * ax + 2.0f * ax + ax) * h / 6.0f;
+                    vy = vy + (ay + 2.0f * ay + 2.0f * ay + ay) * h / 6.0f;
+                    vz = vz + (az + 2.0f * az + 2.0f * az + az) * h / 6.0f;
+                }
+            if(vx < c.v_min || vx > c.v_max || vy < c.v_min || vy > c.v_max || vz < c.v_min || vz > c.v_max)
+                {
+                    valid = false;
+                    break;
+                }
+            if(ws_out(c, x, y, z))
+                {
+                    valid = false;
+                    break;
+                }
+            if(!segment_free(px, py, pz, x, y, z, sObs, nObs))
+                {
+                    valid = false;
+                    break;
+                }
+        }
+    s[0] = x, s[1] = y, s[2] = z, s[3] = vx, s[4] = vy, s[5] = vz;
+    return valid;
+}
+
+// statePropagator.cu:147-241; FIVE_D: speed is control u[2], no acceleration.
+template <bool FIVE_D>
+__device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const float* u, int steps,
+                                            const float4* sObs, int nObs)
+{
+    const kgmt_config& c = qd.cfg;
+    const float h        = c.dt;
+    const float yr = u[0], pr = u[1];
+    const float a = FIVE_D ? 0.0f : u[2];
+    float x = s[0], y = s[1], z = s[2], yaw = s[3], pitch = s[4];
+    float v    = FIVE_D ? u[2] : s[5];
+    bool valid = true;
+    for(int i = 0; i < steps; ++i)
+        {
+            const float px = x, py = y, pz = z;
+            const float hh = 0.5f * h;
+            const float vm = v + hh * a, pm = pitch + hh * pr, ym = yaw + hh * yr;
+            const float ve = v + h * a, pe = pitch + h * pr, ye = yaw + h * yr;
+            float sp0, cp0, sy0, cy0, spm, cpm, sym, cym, spe, cpe, sye, cye;
+            sincos_k(pitch, sp0, cp0);
+            sincos_k(yaw, sy0, cy0);
+            sincos_k(pm, spm, cpm);
+            sincos_k(ym, sym, cym);
+            sincos_k(pe, spe, cpe);
+            sincos_k(ye, sye, cye);
+            const float mx = vm * cpm * cym;
+            const float my = vm * cpm * sym;
+            const float mz = vm * spm;
+            x              = x + (h / 6.0f) * (v * cp0 * cy0 + 2.0f * (mx + mx) + ve * cpe * cye);
+            y              = y + (h / 6.0f) * (v * cp0 * sy0 + 2.0f * (my + my) + ve * cpe * sye);
+            z              = z + (h / 6.0f) * (v * sp0 + 2.0f * (mz + mz) + ve * spe);
+            yaw            = yaw + h * yr;
+            pitch          = pitch + h * pr;
+            v              = v + (h / 6.0f) * (a + 2.0f * (a + a) + a);
+            if(!FIVE_D && (v < c.v_min || v > c.v_max))
+                {
+                    valid = false;
+                    break;
+                }
+            if(pitch < c.pitch_min || pitch > c.pitch_max)
+                {
+                    valid = false;
+                    break;
+                }
+            if(ws_out(c, x, y, z))
+                {
+                    valid = false;
+                    break;
+                }
+            if(!segment_free(px, py, pz, x, y, z, sObs, nObs))
+                {
+                    valid = false;
+                    break;
+                }
+        }
+    s[0] = x, s[1] = y, s[2] = z, s[3] = yaw, s[4] = pitch;
+    if(!FIVE_D) s[5] = v;
+    return valid;
+}
+
+// ode, statePropagator.cu:325-370
+__device__ __forceinline__ void quad_ode(const QueryDesc& qd, const float* st, const float* uc, float* xd)
+{
+    const kgmt_config& c = qd.cfg;
+    const float phi = st[3], theta = st[4], psi = st[5];
+    const float u = st[6], v = st[7], w = st[8], p = st[9], q = st[10], r = st[11];
+    float sph, cph, sth, cth, sps, cps;
+    sincos_k(phi, sph, cph);
+    sincos_k(theta, sth, cth);
+    sincos_k(psi, sps, cps);
+    const float tth = sth / cth;
+    const float g   = c.gravity;
+    xd[0]           = cth * cps * u + (sph * sth * cps - cph * sps) * v + (cph * sth * cps + sph * sps) * w;
+    xd[1]           = cth * sps * u + (sph * sth * sps + cph * cps) * v + (cph * sth * sps - sph * cps) * w;
+    xd[2]           = -sth * u + sph * cth * v + cph * cth * w;
+    xd[3]           = p + (q * sph + r * cph) * tth;
+    xd[4]           = q * cph - r * sph;
+    xd[5]           = (q * sph + r * cph) / cth;
+    const float XYZ = -c.quad_nu * __builtin_sqrtf(u * u + v * v + w * w);
+    xd[6]           = (r * v - q * w) - g * sth + qd.qminv * XYZ * u;
+    xd[7]           = (p * w - r * u) + g * cth * sph + qd.qminv * XYZ * v;
+    xd[8]           = (q * u - p * v) + g * cth * cph + qd.qminv * XYZ * w + qd.qminv * uc[0];
+    const float LMN = -c.quad_mu * __builtin_sqrtf(p * p + q * q + r * r);
+    xd[9]           = qd.qax * q * r + qd.qbx * LMN * p + qd.qbx * uc[1];
+    xd[10]          = qd.qay * p * r + qd.qby * LMN * q + qd.qby * uc[2];
+    xd[11]          = qd.qaz * p * q + qd.qbz * LMN * r + qd.qbz * uc[3];
+}
+
+// statePropagator.cu:246-323
+__device__ __forceinline__ bool prop_quad(const QueryDesc& qd, float* s, const float* uc, int steps,
+                                          const float4* sObs, int nObs)
+{
+    const kgmt_config& c = qd.cfg;
+    const float h = c.dt, hh = 0.5f * h, h6 = h / 6.0f;
+    bool valid = true;
+    for(int i = 0; i < steps; ++i)
+        {
+            const float px = s[0], py = s[1], pz = s[2];
+            float k1[12], k2[12], k3[12], k4[12], st[12];
+            quad_ode(qd, s, uc, k1);
+#pragma unroll
+            for(int j = 0; j < 12; ++j) st[j] = s[j] + hh * k1[j];
+            quad_ode(qd, st, uc, k2);
+#pragma unroll
+            for(int j = 0; j < 12; ++j) st[j] = s[j] + hh * k2[j];
+            quad_ode(qd, st, uc, k3);
+#pragma unroll
+            for(int j = 0; j < 12; ++j) st[j] = s[j] + h * k3[j];
+            quad_ode(qd, st, uc, k4);
+#pragma unroll
+            for(int j = 0; j < 12; ++j) s[j] = s[j] + h6 * (k1[j] + 2.0f * k2[j] + 2.0f * k3[j] + k4[j]);
+            if(s[6] < c.v_min || s[6] > c.v_max || s[7] < c.v_min || s[7] > c.v_max || s[8] < c.v_min ||
+               s[8] > c.v_max)
+                {
+                    valid = false;
+                    break;
+                }
+            if(ws_out(c, s[0], s[1], s[2]))
+                {
+                    valid = false;
+                    break;
+                }
+            if(!segment_free(px, py, pz, s[0], s[1], s[2], sObs, nObs))
+                {
+                    valid = false;
+                    break;
+                }
+        }
+    return valid;
+}
+
+template <int MODEL>
+__device__ __forceinline__ bool propagate(const QueryDesc& qd, float* s, const float* u, int steps,
+                                          const float4* sObs, int nObs)
+{
+    if constexpr(MODEL == KGMT_MODEL_DOUBLE_INTEGRATOR)
+        return prop_di(qd, s, u, steps, sObs, nObs);
+    else if constexpr(MODEL == KGMT_MODEL_DUBINS_AIRPLANE)
+        return prop_dubins<false>(qd, s, u, steps, sObs, nObs);
+    else if constexpr(MODEL == KGMT_MODEL_DUBINS_5D)
+        return prop_dubins<true>(qd, s, u, steps, sObs, nObs);
+    else
+        return prop_quad(qd, s, u, steps, sObs, nObs);
+}
+
+// -------------------------------------------------------------------------
+// Regions (workspace-only grid, z fastest). Graph.cu:99-153, 155-205; the
+// region minimum corner is ws_min + index * W_R1_SIZE (Graph.cu:67-70).
+// -------------------------------------------------------------------------
+__device__ __forceinline__ int grid_index(float f, int L)
+{
+    if(!(f >= 0.0f)) return 0;
+    if(f >= (float)L) return L - 1;
+    const int i = (int)f;
+    return i >= L ? L - 1 : i;
+}
+
+__device__ __forceinline__ int region_of(const QueryDesc& qd, float x, float y, float z, int& ix, int& iy, int& iz)
+{
+    const kgmt_config& c = qd.cfg;
+    const int L          = c.r1_len;
+    const float span     = c.ws_max - c.ws_min;
+    const float fL       = (float)L;
+    iz                   = grid_index((fL * (z - c.ws_min)) / span, L);
+    iy                   = grid_index((fL * (y - c.ws_min)) / span, L);
+    ix                   = grid_index((fL * (x - c.ws_min)) / span, L);
+    return (ix * L + iy) * L + iz;
+}
+
+__device__ __forceinline__ int subregion_of(const QueryDesc& qd, float x, float y, float z, int r1, int ix, int iy,
+                                            int iz)
+{
+    const kgmt_config& c = qd.cfg;
+    const int L2         = c.r2_len;
+    const float fL2      = (float)L2;
+    const float size     = qd.r1size;
+    const float mx = c.ws_min + (float)ix * size, my = c.ws_min + (float)iy * size, mz = c.ws_min + (float)iz * size;
+    const int lz = grid_index((fL2 * (z - mz)) / size, L2);
+    const int ly = grid_index((fL2 * (y - my)) / size, L2);
+    const int lx = grid_index((fL2 * (x - mx)) / size, L2);
+    return r1 * qd.P + (lx * L2 + ly) * L2 + lz;
+}
+
+// helper.cuh:138-152 (3D goal distance)
+__device__ __forceinline__ float goal_dist(const QueryDesc& qd, float x, float y, float z)
+{
+    const float dx = x - qd.goal[0], dy = y - qd.goal[1], dz = z - qd.goal[2];
+    return __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+// -------------------------------------------------------------------------
+// Wave / block helpers (wave64).
+// -------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id()
+{
+    return __lane_id();
+}
+
+// inclusive scan of v over the 64 lanes of a wave
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for(int d = 1; d < 64; d <<= 1)
+        {
+            const uint32_t t = __shfl_up(v, d, 64);
+            if(lane >= (uint32_t)d) v += t;
+        }
+    return v;
+}
+
+// Exclusive scan of one value per thread over a block of NT threads
+// (NT/64 waves). Returns the exclusive prefix; *total receives the block sum.
+// sbuf: at least NT/64 + 1 words of LDS. Contains barriers: call uniformly.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sbuf, uint32_t& total)
+{
+    constexpr int NW     = NT / 64;
+    const uint32_t lane  = lane_id();
+    const uint32_t wave  = threadIdx.x / 64;
+    const uint32_t incl  = wave_incl_scan(v);
+    if(lane == 63) sbuf[wave] = incl;
+    __syncthreads();
+    if(threadIdx.x == 0)
+        {
+            uint32_t run = 0;
+            for(int w = 0; w < NW; ++w)
+                {
+                    const uint32_t t = sbuf[w];
+                    sbuf[w]          = run;
+                    run += t;
+                }
+            sbuf[NW] = run;
+        }
+    __syncthreads();
+    const uint32_t ex = sbuf[wave] + incl - v;
+    total             = sbuf[NW];
+    __syncthreads();
+    return ex;
+}
+
+// Exclusive scan (in place) of n words with one block of NT threads,
+// returns the sum. Barriers inside: call uniformly.
+template <int NT>
+__device__ __forceinline__ uint32_t block_scan_array(uint32_t* a, uint32_t n, uint32_t* sbuf)
+{
+    uint32_t carry = 0;
+    for(uint32_t base = 0; base < n; base += NT)
+        {
+            const uint32_t i = base + threadIdx.x;
+            const uint32_t v = i < n ? a[i] : 0u;
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan<NT>(v, sbuf, tot);
+            if(i < n) a[i] = carry + ex;
+            carry += tot;
+        }
+    return carry;
+}
+
+__device__ __forceinline__ bool test_bit(const uint32_t* words, uint32_t bit)
+{
+    return (words[bit >> 5] >> (bit & 31u)) & 1u;
+}
+
+// number of set bits in [start, start + len) of a bitset
+__device__ __forceinline__ uint32_t count_bits(const uint32_t* words, uint32_t start, uint32_t len)
+{
+    uint32_t cnt = 0, b = start;
+    const uint32_t end = start + len;
+    while(b < end)
+        {
+            const uint32_t w   = b >> 5, off = b & 31u;
+            const uint32_t n   = (32u - off) < (end - b) ? (32u - off) : (end - b);
+            const uint32_t msk = n == 32u ? 0xffffffffu : (((1u << n) - 1u) << off);
+            cnt += __popc(words[w] & msk);
+            b += n;
+        }
+    return cnt;
+}
+
+// index of the query owning global job t (prefix array of nq + 1 entries)
+__device__ __forceinline__ uint32_t locate(const uint32_t* prefix, uint32_t nq, uint32_t t)
+{
+    uint32_t lo = 0, hi = nq;  // prefix[lo] <= t < prefix[hi]
+    while(hi - lo > 1)
+        {
+            const uint32_t mid = (lo + hi) >> 1;
+            if(prefix[mid] <= t)
+                lo = mid;
+            else
+                hi = mid;
+        }
+    return lo;
+}
+
+}  // namespace kgmt

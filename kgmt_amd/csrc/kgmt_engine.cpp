@@ -1,0 +1,837 @@
+// kgmt_engine.cpp — host side of the KGMT expansion path: device pools, the
+// per-iteration launch sequence (plain or hipGraph), and the C-ABI of
+// include/kgmt.h.
+//
+// Reference host code replaced (paths relative to the KGMT repository):
+//   KGMT::KGMT / Planner::Planner / Graph::Graph  src/planners/KGMT.cu:4-39,
+//       src/planners/Planner.cu:4-35, src/graphs/Graph.cu:5-42  -> Engine::Engine
+//   KGMT::plan                                    src/planners/KGMT.cu:41-105 -> Engine::run
+// The reference reads five sizes back to the host every iteration
+// (KGMT.cu:171-172, 180-181, 384, 399); here all sizes stay on the device and
+// the host polls one completion word a few iterations behind the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kgmt.h"
+#include "kgmt_launch.hpp"
+
+using namespace kgmt;
+
+namespace
+{
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+    do                                                                                           \
+        {                                                                                        \
+            hipError_t e_ = (expr);                                                              \
+            if(e_ != hipSuccess) return fail(KGMT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+        }                                                                                        \
+    while(0)
+
+int model_dims(int model, int& S, int& C)
+{
+    switch(model)
+        {
+            case KGMT_MODEL_DOUBLE_INTEGRATOR:
+            case KGMT_MODEL_DUBINS_AIRPLANE: S = 6, C = 3; return 0;
+            case KGMT_MODEL_QUADCOPTER: S = 12, C = 4; return 0;
+            case KGMT_MODEL_DUBINS_5D: S = 5, C = 3; return 0;
+            default: return -1;
+        }
+}
+
+int check_config(const kgmt_config* c)
+{
+    if(!c) return fail(KGMT_E_ARG, "config is null");
+    int S, C;
+    if(model_dims(c->model, S, C))
+        return fail(c->model == KGMT_MODEL_UNICYCLE ? KGMT_E_UNSUPPORTED : KGMT_E_ARG,
+                    "model " + std::to_string(c->model) + " is not built");
+    if(c->capacity < 2 || c->capacity > (1u << 26)) return fail(KGMT_E_ARG, "capacity must be in [2, 2^26]");
+    if(c->max_iter < 1 || c->max_iter > 1000000) return fail(KGMT_E_ARG, "max_iter must be in [1, 1e6]");
+    if(!(c->dt > 0.0f) || !std::isfinite(c->dt)) return fail(KGMT_E_ARG, "dt must be positive");
+    if(c->steps_min < 1 || c->steps_max < c->steps_min || c->steps_max > 100000)
+        return fail(KGMT_E_ARG, "steps range must satisfy 1 <= steps_min <= steps_max <= 1e5");
+    if(!(c->ws_max > c->ws_min)) return fail(KGMT_E_ARG, "ws_max must exceed ws_min");
+    if(c->r1_len < 1 || c->r2_len < 1) return fail(KGMT_E_ARG, "grid lengths must be >= 1");
+    const int64_t R1 = (int64_t)c->r1_len * c->r1_len * c->r1_len;
+    const int64_t P  = (int64_t)c->r2_len * c->r2_len * c->r2_len;
+    if(R1 > 8192) return fail(KGMT_E_ARG, "at most 8192 regions (r1_len <= 20)");
+    if(R1 * P > (1ll << 28)) return fail(KGMT_E_ARG, "too many subregions");
+    if(c->children_per_entry < 1 || c->children_per_entry > 1024)
+        return fail(KGMT_E_ARG, "children_per_entry must be in [1, 1024]");
+    if(c->root_repeat < 1) return fail(KGMT_E_ARG, "root_repeat must be >= 1");
+    if(c->repeat_sparse > 63 || c->repeat_dense > 63) return fail(KGMT_E_ARG, "repeat counts must be <= 63");
+    if(c->model == KGMT_MODEL_QUADCOPTER &&
+       (!(c->quad_mass > 0.0f) || !(c->quad_ix > 0.0f) || !(c->quad_iy > 0.0f) || !(c->quad_iz > 0.0f)))
+        return fail(KGMT_E_ARG, "quad mass and inertia must be positive");
+    for(int i = 0; i < C; ++i)
+        if(!(c->ctrl_max[i] >= c->ctrl_min[i])) return fail(KGMT_E_ARG, "control range reversed");
+    return KGMT_OK;
+}
+
+// descriptor fields that depend only on the configuration
+void fill_desc(QueryDesc& qd, const kgmt_config& c)
+{
+    std::memset(&qd, 0, sizeof(qd));
+    qd.cfg = c;
+    int S = 0, C = 0;
+    model_dims(c.model, S, C);
+    qd.S = S, qd.C = C, qd.D = S + C + 1;
+    qd.R1      = c.r1_len * c.r1_len * c.r1_len;
+    qd.P       = c.r2_len * c.r2_len * c.r2_len;
+    qd.R2      = qd.R1 * qd.P;
+    qd.R2words = (qd.R2 + 31) / 32;
+    qd.cap     = c.capacity;
+    qd.nChunks = (c.capacity + kChunk - 1) / kChunk;
+    // W_R1_SIZE and W_R1_VOL as the float macros of config.h:247-251 evaluate
+    qd.r1size = (c.ws_max - c.ws_min) / (float)c.r1_len;
+    qd.vol    = qd.r1size * qd.r1size * qd.r1size;
+    qd.qminv  = 1.0f / c.quad_mass;
+    qd.qax = (c.quad_iy - c.quad_iz) / c.quad_ix, qd.qbx = 1.0f / c.quad_ix;
+    qd.qay = (c.quad_iz - c.quad_ix) / c.quad_iy, qd.qby = 1.0f / c.quad_iy;
+    qd.qaz = (c.quad_ix - c.quad_iy) / c.quad_iz, qd.qbz = 1.0f / c.quad_iz;
+}
+
+// obstacle rows (xmin ymin zmin xmax ymax zmax) -> (lo, hi) float4 pairs
+void pack_obstacles(const float* rows, uint32_t n, float4* out)
+{
+    for(uint32_t i = 0; i < n; ++i)
+        {
+            const float* r = rows + 6 * (size_t)i;
+            out[2 * i]     = make_float4(r[0], r[1], r[2], 0.0f);
+            out[2 * i + 1] = make_float4(r[3], r[4], r[5], 0.0f);
+        }
+}
+
+__global__ void k_pack_obstacles(const float* __restrict__ rows, uint32_t n, float4* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    const float* r = rows + 6 * (size_t)i;
+    out[2 * i]     = make_float4(r[0], r[1], r[2], 0.0f);
+    out[2 * i + 1] = make_float4(r[3], r[4], r[5], 0.0f);
+}
+
+template <typename T>
+int dev_alloc(T*& p, size_t count, std::vector<void*>& owned)
+{
+    void* q = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if(hipMalloc(&q, bytes) != hipSuccess)
+        return fail(KGMT_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    owned.push_back(q);
+    p = static_cast<T*>(q);
+    return KGMT_OK;
+}
+
+constexpr int kRing = 8;  // completion polls in flight
+constexpr int kLag  = 2;  // iterations the host stays behind the GPU
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Engine: device pools and the iteration loop for a table of queries
+// ---------------------------------------------------------------------------
+struct Engine
+{
+    uint32_t nq = 0;
+    std::vector<QueryDesc> hq;
+    std::vector<QueryState> hs;
+    std::vector<void*> owned;
+    QueryDesc* dq  = nullptr;
+    QueryState* ds = nullptr;
+    Pools P{};
+    WorkLists W{};
+    uint32_t obsCap = 0;     // boxes the obstacle pool holds
+    float4* obsPool = nullptr;
+    uint32_t maxObs = 0;     // largest per-query box count of the current plan
+    int model       = -1;    // common model or -1
+    uint32_t maxIter = 0;
+    hipStream_t stream = nullptr;
+    bool ownStream     = false;
+    uint32_t* pinned   = nullptr;
+    hipEvent_t ev[kRing] = {};
+    Launch L;
+    hipGraph_t graph         = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    uint32_t graphObs        = 0xffffffffu;
+    Launch graphLaunch;
+    std::vector<double> iterMs;
+    uint32_t itersRun = 0;
+    int device        = 0;
+
+    ~Engine()
+    {
+        if(graphExec) (void)hipGraphExecDestroy(graphExec);
+        if(graph) (void)hipGraphDestroy(graph);
+        for(auto& e : ev)
+            if(e) (void)hipEventDestroy(e);
+        if(pinned) (void)hipHostFree(pinned);
+        for(void* p : owned) (void)hipFree(p);
+        if(ownStream && stream) (void)hipStreamDestroy(stream);
+    }
+
+    int init(const kgmt_config* cfgs, uint32_t n)
+    {
+        if(n == 0) return fail(KGMT_E_ARG, "no queries");
+        int ndev = 0;
+        if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(KGMT_E_NODEVICE, "no HIP device visible");
+        HIP_TRY(hipGetDevice(&device));
+        nq = n;
+        hq.resize(n);
+        hs.resize(n);
+        uint64_t samples = 0, nodes = 0, paths = 0;
+        uint32_t chunks = 0, regs = 0, subs = 0;
+        model = cfgs[0].model;
+        for(uint32_t q = 0; q < n; ++q)
+            {
+                int rc = check_config(&cfgs[q]);
+                if(rc) return rc;
+                QueryDesc& qd = hq[q];
+                fill_desc(qd, cfgs[q]);
+                qd.sampleOff = samples;
+                qd.nodeOff   = nodes;
+                qd.chunkOff  = chunks;
+                qd.regOff    = regs;
+                qd.subOff    = subs;
+                qd.pathOff   = (uint32_t)paths;
+                samples += (uint64_t)qd.D * qd.cap;
+                nodes += qd.cap;
+                chunks += qd.nChunks + 1;
+                regs += (uint32_t)qd.R1;
+                subs += (uint32_t)qd.R2words;
+                paths += (uint64_t)(cfgs[q].max_iter + 2) * qd.D;
+                if(paths > 0xffffffffull) return fail(KGMT_E_CAPACITY, "path pool too large");
+                if(cfgs[q].model != model) model = -1;
+                maxIter = std::max(maxIter, cfgs[q].max_iter);
+            }
+        int rc = 0;
+        if((rc = dev_alloc(dq, n, owned)) || (rc = dev_alloc(ds, n, owned)) ||
+           (rc = dev_alloc(P.treeS, samples, owned)) || (rc = dev_alloc(P.childS, samples, owned)) ||
+           (rc = dev_alloc(P.parent, nodes, owned)) || (rc = dev_alloc(P.cost, nodes, owned)) ||
+           (rc = dev_alloc(P.frontier, nodes, owned)) || (rc = dev_alloc(P.r1node, nodes, owned)) ||
+           (rc = dev_alloc(P.parentU, nodes, owned)) || (rc = dev_alloc(P.r1child, nodes, owned)) ||
+           (rc = dev_alloc(P.fnext, nodes, owned)) || (rc = dev_alloc(P.react, nodes, owned)) ||
+           (rc = dev_alloc(P.repU, nodes, owned)) || (rc = dev_alloc(P.repIdx, nodes, owned)) ||
+           (rc = dev_alloc(P.partA, chunks, owned)) || (rc = dev_alloc(P.partB, chunks, owned)) ||
+           (rc = dev_alloc(P.partC, chunks, owned)) || (rc = dev_alloc(P.total, regs, owned)) ||
+           (rc = dev_alloc(P.validc, regs, owned)) || (rc = dev_alloc(P.score, regs, owned)) ||
+           (rc = dev_alloc(P.actPrev, subs, owned)) || (rc = dev_alloc(P.actNext, subs, owned)) ||
+           (rc = dev_alloc(P.path, paths, owned)) || (rc = dev_alloc(W.tileStart, n + 1, owned)) ||
+           (rc = dev_alloc(W.jobStart, n + 1, owned)) || (rc = dev_alloc(W.counter, 1, owned)) ||
+           (rc = dev_alloc(W.allDone, 1, owned)))
+            return rc;
+        HIP_TRY(hipMemset(W.counter, 0, sizeof(uint32_t)));
+        HIP_TRY(hipMemset(W.allDone, 0, sizeof(uint32_t)));
+        HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        ownStream = true;
+        HIP_TRY(hipHostMalloc((void**)&pinned, kRing * sizeof(uint32_t), hipHostMallocDefault));
+        for(auto& e : ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        // launch shape: persistent workgroups sized to the device
+        hipDeviceProp_t prop;
+        HIP_TRY(hipGetDeviceProperties(&prop, device));
+        const uint32_t cus = (uint32_t)std::max(prop.multiProcessorCount, 1);
+        L.propGrid         = cus * 4;
+        L.chunkGrid        = cus * 4;
+        return KGMT_OK;
+    }
+
+    int set_stream(void* s)
+    {
+        if(ownStream && stream) HIP_TRY(hipStreamDestroy(stream));
+        ownStream = false;
+        stream    = (hipStream_t)s;
+        if(!stream)
+            {
+                HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+                ownStream = true;
+            }
+        drop_graph();
+        return KGMT_OK;
+    }
+
+    void drop_graph()
+    {
+        if(graphExec) (void)hipGraphExecDestroy(graphExec);
+        if(graph) (void)hipGraphDestroy(graph);
+        graphExec = nullptr, graph = nullptr, graphObs = 0xffffffffu;
+    }
+
+    int ensure_obstacles(uint32_t boxes)
+    {
+        if(boxes <= obsCap && obsPool) return KGMT_OK;
+        const uint32_t cap = std::max<uint32_t>(boxes, 64);
+        float4* p          = nullptr;
+        if(hipMalloc((void**)&p, (size_t)cap * 2 * sizeof(float4)) != hipSuccess)
+            return fail(KGMT_E_NOMEM, "obstacle pool allocation failed");
+        if(obsPool)
+            {
+                auto it = std::find(owned.begin(), owned.end(), (void*)obsPool);
+                if(it != owned.end()) owned.erase(it);
+                HIP_TRY(hipStreamSynchronize(stream));
+                (void)hipFree(obsPool);
+            }
+        owned.push_back(p);
+        obsPool = p, obsCap = cap, P.obs = p;
+        drop_graph();
+        return KGMT_OK;
+    }
+
+    int enqueue_iteration()
+    {
+        HIP_TRY(launch_propagate(model, dq, ds, P, W, nq, maxObs, L, stream));
+        HIP_TRY(launch_score(dq, ds, P, nq, stream));
+        HIP_TRY(launch_append(dq, ds, P, W, nq, L, stream));
+        HIP_TRY(launch_expand(dq, ds, P, W, nq, L, stream));
+        HIP_TRY(launch_plan(dq, ds, P, W, nq, stream));
+        return KGMT_OK;
+    }
+
+    int build_graph()
+    {
+        drop_graph();
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_iteration();
+        hipError_t e = hipMemcpyAsync(pinned, W.allDone, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(stream, &g);
+        if(rc) return rc;
+        if(e != hipSuccess) return fail(KGMT_E_HIP, std::string("capture memcpy: ") + hipGetErrorString(e));
+        if(e2 != hipSuccess) return fail(KGMT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+        graph = g;
+        HIP_TRY(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0));
+        graphObs    = maxObs;
+        graphLaunch = L;
+        return KGMT_OK;
+    }
+
+    // Upload descriptors, run init + iterations until every query is done.
+    int run(bool useGraph)
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        HIP_TRY(hipMemcpyAsync(dq, hq.data(), sizeof(QueryDesc) * nq, hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemsetAsync(W.counter, 0, sizeof(uint32_t), stream));
+        HIP_TRY(hipMemsetAsync(W.allDone, 0, sizeof(uint32_t), stream));
+        HIP_TRY(launch_init(dq, ds, P, W, nq, stream));
+        if(useGraph && (!graphExec || graphObs != maxObs || graphLaunch.propGrid != L.propGrid ||
+                        graphLaunch.chunkGrid != L.chunkGrid))
+            {
+                int rc = build_graph();
+                if(rc) return rc;
+            }
+        iterMs.assign(maxIter + 1, 0.0);
+        for(int k = 0; k < kRing; ++k) pinned[k] = 0;
+        uint32_t it = 0, observed = 0;
+        bool finished = false;
+        while(it < maxIter && !finished)
+            {
+                const int slot = (int)(it % kRing);
+                if(useGraph)
+                    HIP_TRY(hipGraphLaunch(graphExec, stream));
+                else
+                    {
+                        int rc = enqueue_iteration();
+                        if(rc) return rc;
+                        HIP_TRY(hipMemcpyAsync(&pinned[slot], W.allDone, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                               stream));
+                    }
+                HIP_TRY(hipEventRecord(ev[slot], stream));
+                ++it;
+                // observe iterations that are kLag behind
+                while(observed + kLag < it || (it == maxIter && observed < it))
+                    {
+                        const int os = (int)(observed % kRing);
+                        hipError_t q;
+                        while((q = hipEventQuery(ev[os])) == hipErrorNotReady) {}
+                        if(q != hipSuccess) return fail(KGMT_E_HIP, std::string("iteration: ") + hipGetErrorString(q));
+                        iterMs[observed] =
+                          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                        const uint32_t v = useGraph ? pinned[0] : pinned[os];
+                        ++observed;
+                        if(v)
+                            {
+                                finished = true;
+                                break;
+                            }
+                    }
+            }
+        HIP_TRY(hipStreamSynchronize(stream));
+        itersRun = it;
+        HIP_TRY(hipMemcpy(hs.data(), ds, sizeof(QueryState) * nq, hipMemcpyDeviceToHost));
+        return KGMT_OK;
+    }
+
+    void result(uint32_t q, kgmt_result* out, double wallMs) const
+    {
+        const QueryState& s = hs[q];
+        out->success        = (int32_t)s.success;
+        out->goal_index     = s.success ? s.goalIdx : -1;
+        out->iterations     = s.done ? s.finishItr : s.itr;
+        out->tree_size      = s.treeSize;
+        out->path_rows      = s.pathRows;
+        out->path_cost      = s.pathCost;
+        out->path_length    = s.pathLen;
+        out->time_ms        = (float)wallMs;
+        out->expanded       = s.expanded;
+    }
+
+    int get_tree(uint32_t q, float* samples, int32_t* parents, float* costs, uint32_t cap, uint32_t* n)
+    {
+        const QueryDesc& qd = hq[q];
+        const uint32_t ts   = hs[q].treeSize;
+        if(n) *n = ts;
+        if(!samples && !parents && !costs) return KGMT_OK;
+        if(cap < ts) return fail(KGMT_E_CAPACITY, "tree output buffer too small");
+        if(samples)
+            {
+                std::vector<float> soa((size_t)qd.D * ts);
+                for(int d = 0; d < qd.D; ++d)
+                    HIP_TRY(hipMemcpy(soa.data() + (size_t)d * ts, P.treeS + qd.sampleOff + (size_t)d * qd.cap,
+                                      sizeof(float) * ts, hipMemcpyDeviceToHost));
+                for(uint32_t i = 0; i < ts; ++i)
+                    for(int d = 0; d < qd.D; ++d) samples[(size_t)i * qd.D + d] = soa[(size_t)d * ts + i];
+            }
+        if(parents)
+            HIP_TRY(hipMemcpy(parents, P.parent + qd.nodeOff, sizeof(int32_t) * ts, hipMemcpyDeviceToHost));
+        if(costs) HIP_TRY(hipMemcpy(costs, P.cost + qd.nodeOff, sizeof(float) * ts, hipMemcpyDeviceToHost));
+        return KGMT_OK;
+    }
+
+    int get_path(uint32_t q, float* rows, uint32_t capRows, uint32_t* nRows)
+    {
+        const QueryDesc& qd = hq[q];
+        const uint32_t r    = hs[q].success ? hs[q].pathRows : 0;
+        if(nRows) *nRows = r;
+        if(!rows || r == 0) return KGMT_OK;
+        if(capRows < r) return fail(KGMT_E_CAPACITY, "path output buffer too small");
+        HIP_TRY(hipMemcpy(rows, P.path + qd.pathOff, sizeof(float) * r * qd.D, hipMemcpyDeviceToHost));
+        return KGMT_OK;
+    }
+};
+
+struct kgmt_ctx
+{
+    Engine eng;
+};
+
+struct kgmt_batch
+{
+    Engine eng;
+};
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int kgmt_abi_version(void)
+{
+    return KGMT_ABI_VERSION;
+}
+
+const char* kgmt_last_error(void)
+{
+    return g_err.c_str();
+}
+
+int kgmt_model_dims(int32_t model, int32_t* s, int32_t* c, int32_t* d)
+{
+    int S, C;
+    if(model_dims(model, S, C)) return fail(KGMT_E_UNSUPPORTED, "model not built");
+    if(s) *s = S;
+    if(c) *c = C;
+    if(d) *d = S + C + 1;
+    return KGMT_OK;
+}
+
+int kgmt_config_preset(int32_t preset, kgmt_config* o)
+{
+    if(!o) return fail(KGMT_E_ARG, "out is null");
+    std::memset(o, 0, sizeof(*o));
+    // shared planner constants (KGMT.cu:30, 67, 336-339; config.h:262)
+    o->epsilon            = 1e-2f;
+    o->children_per_entry = 32;
+    o->root_repeat        = 5;
+    o->repeat_sparse      = 15;
+    o->repeat_dense       = 1;
+    o->sparse_valid_below = 10;
+    o->max_iter           = 300;
+    o->quad_mass = 1.0f, o->quad_ix = 1.0f, o->quad_iy = 1.0f, o->quad_iz = 2.0f;
+    o->quad_nu = 10e-3f, o->quad_mu = 2e-6f, o->gravity = -9.81f;
+    o->pitch_min = -(float)M_PI / 3.0f, o->pitch_max = (float)M_PI / 3.0f;
+    o->seed      = 1;
+    const float pi = (float)M_PI;
+    switch(preset)
+        {
+            case KGMT_PRESET_REFERENCE_QUAD:  // config.h:209-307
+                o->model = KGMT_MODEL_QUADCOPTER;
+                o->capacity = 400000, o->dt = 0.1f, o->steps_min = 1, o->steps_max = 10;
+                o->goal_thresh = 5.0f, o->ws_min = 0.0f, o->ws_max = 100.0f;
+                o->v_min = -30.0f, o->v_max = 30.0f;
+                o->ctrl_min[0] = 0.0f, o->ctrl_max[0] = 30.0f;
+                for(int i = 1; i < 4; ++i) o->ctrl_min[i] = -pi, o->ctrl_max[i] = pi;
+                o->r1_len = 8, o->r2_len = 2;
+                return KGMT_OK;
+            case KGMT_PRESET_DI6D:
+                o->model = KGMT_MODEL_DOUBLE_INTEGRATOR, o->integrator = KGMT_DI_REFERENCE;
+                o->capacity = 1u << 17, o->dt = 0.05f, o->steps_min = 2, o->steps_max = 10;
+                o->goal_thresh = 0.05f, o->ws_min = 0.0f, o->ws_max = 1.0f;
+                o->v_min = -1.0f, o->v_max = 1.0f;
+                for(int i = 0; i < 3; ++i) o->ctrl_min[i] = -1.0f, o->ctrl_max[i] = 1.0f;
+                o->r1_len = 8, o->r2_len = 4;
+                return KGMT_OK;
+            case KGMT_PRESET_DUBINS5D:
+                o->model    = KGMT_MODEL_DUBINS_5D;
+                o->capacity = 1u << 18, o->dt = 0.02f, o->steps_min = 5, o->steps_max = 25;
+                o->goal_thresh = 0.05f, o->ws_min = 0.0f, o->ws_max = 1.0f;
+                o->v_min = 0.0f, o->v_max = 1.0f;
+                o->ctrl_min[0] = -1.0f, o->ctrl_max[0] = 1.0f;
+                o->ctrl_min[1] = -0.5f, o->ctrl_max[1] = 0.5f;
+                o->ctrl_min[2] = 0.2f, o->ctrl_max[2] = 0.4f;
+                o->r1_len = 16, o->r2_len = 2;
+                return KGMT_OK;
+            case KGMT_PRESET_QUAD12D:
+                o->model    = KGMT_MODEL_QUADCOPTER;
+                o->capacity = 1u << 19, o->dt = 0.01f, o->steps_min = 10, o->steps_max = 50;
+                o->goal_thresh = 0.05f, o->ws_min = 0.0f, o->ws_max = 1.0f;
+                o->v_min = -5.0f, o->v_max = 5.0f;
+                o->ctrl_min[0] = 0.0f, o->ctrl_max[0] = 19.62f;
+                for(int i = 1; i < 4; ++i) o->ctrl_min[i] = -0.05f, o->ctrl_max[i] = 0.05f;
+                o->quad_ix = 0.01f, o->quad_iy = 0.01f, o->quad_iz = 0.02f;
+                o->r1_len = 10, o->r2_len = 4;
+                return KGMT_OK;
+            default: return fail(KGMT_E_ARG, "unknown preset");
+        }
+}
+
+int kgmt_config_check(const kgmt_config* cfg)
+{
+    return check_config(cfg);
+}
+
+int kgmt_create(const kgmt_config* cfg, kgmt_ctx** out)
+{
+    if(!cfg || !out) return fail(KGMT_E_ARG, "null argument");
+    *out   = nullptr;
+    auto* c = new(std::nothrow) kgmt_ctx;
+    if(!c) return fail(KGMT_E_NOMEM, "context allocation failed");
+    int rc = c->eng.init(cfg, 1);
+    if(rc)
+        {
+            delete c;
+            return rc;
+        }
+    *out = c;
+    return KGMT_OK;
+}
+
+int kgmt_destroy(kgmt_ctx* ctx)
+{
+    delete ctx;
+    return KGMT_OK;
+}
+
+int kgmt_set_stream(kgmt_ctx* ctx, void* s)
+{
+    if(!ctx) return fail(KGMT_E_ARG, "null context");
+    return ctx->eng.set_stream(s);
+}
+
+static int upload_obstacles(Engine& e, const float* obstacles, const uint32_t* offsets, const uint32_t* counts,
+                            uint32_t total, int onDevice)
+{
+    int rc = e.ensure_obstacles(total);
+    if(rc) return rc;
+    e.maxObs = 0;
+    for(uint32_t q = 0; q < e.nq; ++q)
+        {
+            const uint32_t c = counts ? counts[q] : total;
+            if(c > kMaxObs) return fail(KGMT_E_CAPACITY, "more than 2048 obstacles in one query");
+            e.hq[q].obsOff = offsets ? offsets[q] : 0;
+            e.hq[q].nObs   = c;
+            if((uint64_t)e.hq[q].obsOff + c > total) return fail(KGMT_E_ARG, "obstacle range out of bounds");
+            e.maxObs = std::max(e.maxObs, c);
+        }
+    if(total == 0) return KGMT_OK;
+    if(!obstacles) return fail(KGMT_E_ARG, "obstacles pointer is null");
+    if(onDevice)
+        {
+            hipLaunchKernelGGL(k_pack_obstacles, dim3((total + 255) / 256), dim3(256), 0, e.stream, obstacles, total,
+                               e.obsPool);
+            HIP_TRY(hipGetLastError());
+        }
+    else
+        {
+            std::vector<float4> packed((size_t)total * 2);
+            pack_obstacles(obstacles, total, packed.data());
+            HIP_TRY(hipMemcpyAsync(e.obsPool, packed.data(), packed.size() * sizeof(float4), hipMemcpyHostToDevice,
+                                   e.stream));
+            HIP_TRY(hipStreamSynchronize(e.stream));
+        }
+    return KGMT_OK;
+}
+
+int kgmt_plan(kgmt_ctx* ctx, const float* start, const float* goal, const float* obstacles, uint32_t n_obs,
+              int32_t on_device, kgmt_result* out)
+{
+    if(!ctx || !start || !goal || !out) return fail(KGMT_E_ARG, "null argument");
+    const auto t0 = std::chrono::steady_clock::now();
+    Engine& e     = ctx->eng;
+    QueryDesc& qd = e.hq[0];
+    std::memset(qd.start, 0, sizeof(qd.start));
+    std::memcpy(qd.start, start, sizeof(float) * qd.D);
+    qd.goal[0] = goal[0], qd.goal[1] = goal[1], qd.goal[2] = goal[2];
+    int rc = upload_obstacles(e, obstacles, nullptr, nullptr, n_obs, on_device);
+    if(rc) return rc;
+    rc = e.run(false);
+    if(rc) return rc;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    e.result(0, out, ms);
+    return KGMT_OK;
+}
+
+int kgmt_get_tree(kgmt_ctx* ctx, float* samples, int32_t* parents, float* costs, uint32_t cap, uint32_t* n)
+{
+    if(!ctx) return fail(KGMT_E_ARG, "null context");
+    return ctx->eng.get_tree(0, samples, parents, costs, cap, n);
+}
+
+int kgmt_get_path(kgmt_ctx* ctx, float* rows, uint32_t cap_rows, uint32_t* n_rows)
+{
+    if(!ctx) return fail(KGMT_E_ARG, "null context");
+    return ctx->eng.get_path(0, rows, cap_rows, n_rows);
+}
+
+int kgmt_get_regions(kgmt_ctx* ctx, int32_t* total, int32_t* valid, float* score, uint32_t cap, uint32_t* n)
+{
+    if(!ctx) return fail(KGMT_E_ARG, "null context");
+    const Engine& e     = ctx->eng;
+    const QueryDesc& qd = e.hq[0];
+    if(n) *n = (uint32_t)qd.R1;
+    if(cap < (uint32_t)qd.R1 && (total || valid || score)) return fail(KGMT_E_CAPACITY, "region buffer too small");
+    if(total) HIP_TRY(hipMemcpy(total, e.P.total + qd.regOff, sizeof(int32_t) * qd.R1, hipMemcpyDeviceToHost));
+    if(valid) HIP_TRY(hipMemcpy(valid, e.P.validc + qd.regOff, sizeof(int32_t) * qd.R1, hipMemcpyDeviceToHost));
+    if(score) HIP_TRY(hipMemcpy(score, e.P.score + qd.regOff, sizeof(float) * qd.R1, hipMemcpyDeviceToHost));
+    return KGMT_OK;
+}
+
+// ---- batch ----------------------------------------------------------------
+int kgmt_batch_create(const kgmt_config* cfgs, uint32_t n_q, kgmt_batch** out)
+{
+    if(!cfgs || !out || n_q == 0) return fail(KGMT_E_ARG, "bad argument");
+    *out   = nullptr;
+    auto* b = new(std::nothrow) kgmt_batch;
+    if(!b) return fail(KGMT_E_NOMEM, "batch allocation failed");
+    int rc = b->eng.init(cfgs, n_q);
+    if(rc)
+        {
+            delete b;
+            return rc;
+        }
+    *out = b;
+    return KGMT_OK;
+}
+
+int kgmt_batch_destroy(kgmt_batch* b)
+{
+    delete b;
+    return KGMT_OK;
+}
+
+int kgmt_batch_set_stream(kgmt_batch* b, void* s)
+{
+    if(!b) return fail(KGMT_E_ARG, "null batch");
+    return b->eng.set_stream(s);
+}
+
+int kgmt_batch_set_launch(kgmt_batch* b, uint32_t propagate_block, uint32_t propagate_grid, uint32_t chunk_grid)
+{
+    if(!b) return fail(KGMT_E_ARG, "null batch");
+    if(propagate_block && propagate_block != kTile) return fail(KGMT_E_UNSUPPORTED, "propagate block must be 256");
+    if(propagate_grid) b->eng.L.propGrid = propagate_grid;
+    if(chunk_grid) b->eng.L.chunkGrid = chunk_grid;
+    return KGMT_OK;
+}
+
+int kgmt_batch_plan(kgmt_batch* b, const float* starts, const float* goals, const float* obstacles,
+                    const uint32_t* obstacle_offsets, const uint32_t* obstacle_counts, int32_t use_graph,
+                    kgmt_result* out, float* finish_ms)
+{
+    if(!b || !starts || !goals || !out || !obstacle_offsets || !obstacle_counts)
+        return fail(KGMT_E_ARG, "null argument");
+    const auto t0 = std::chrono::steady_clock::now();
+    Engine& e     = b->eng;
+    uint32_t total = 0;
+    for(uint32_t q = 0; q < e.nq; ++q)
+        {
+            QueryDesc& qd = e.hq[q];
+            std::memset(qd.start, 0, sizeof(qd.start));
+            std::memcpy(qd.start, starts + (size_t)q * KGMT_MAX_SAMPLE, sizeof(float) * qd.D);
+            const float* g = goals + (size_t)q * KGMT_MAX_SAMPLE;
+            qd.goal[0] = g[0], qd.goal[1] = g[1], qd.goal[2] = g[2];
+            total = std::max(total, obstacle_offsets[q] + obstacle_counts[q]);
+        }
+    int rc = upload_obstacles(e, obstacles, obstacle_offsets, obstacle_counts, total, 0);
+    if(rc) return rc;
+    rc = e.run(use_graph != 0);
+    if(rc) return rc;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for(uint32_t q = 0; q < e.nq; ++q)
+        {
+            e.result(q, &out[q], ms);
+            if(finish_ms)
+                {
+                    const uint32_t fi = e.hs[q].done ? e.hs[q].finishItr : e.itersRun;
+                    finish_ms[q]      = fi >= 1 && fi <= e.iterMs.size() && e.iterMs[fi - 1] > 0.0
+                                          ? (float)e.iterMs[fi - 1]
+                                          : (float)ms;
+                }
+        }
+    return KGMT_OK;
+}
+
+int kgmt_batch_get_path(kgmt_batch* b, uint32_t q, float* rows, uint32_t cap_rows, uint32_t* n_rows)
+{
+    if(!b || q >= b->eng.nq) return fail(KGMT_E_ARG, "bad argument");
+    return b->eng.get_path(q, rows, cap_rows, n_rows);
+}
+
+int kgmt_batch_get_tree(kgmt_batch* b, uint32_t q, float* samples, int32_t* parents, float* costs, uint32_t cap,
+                        uint32_t* n)
+{
+    if(!b || q >= b->eng.nq) return fail(KGMT_E_ARG, "bad argument");
+    return b->eng.get_tree(q, samples, parents, costs, cap, n);
+}
+
+// ---- stage-level ----------------------------------------------------------
+namespace
+{
+struct StageDesc
+{
+    QueryDesc* d = nullptr;
+    hipStream_t s;
+    explicit StageDesc(hipStream_t st) : s(st) {}
+    int make(const kgmt_config* cfg)
+    {
+        int rc = check_config(cfg);
+        if(rc) return rc;
+        QueryDesc h;
+        fill_desc(h, *cfg);
+        HIP_TRY(hipMallocAsync((void**)&d, sizeof(QueryDesc), s));
+        HIP_TRY(hipMemcpyAsync(d, &h, sizeof(QueryDesc), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        return KGMT_OK;
+    }
+    ~StageDesc()
+    {
+        if(d) (void)hipFreeAsync(d, s);
+    }
+};
+}  // namespace
+
+int kgmt_draw_controls(const kgmt_config* cfg, uint32_t query, uint32_t itr, uint32_t u0, uint32_t n,
+                       float* d_controls, int32_t* d_steps, float* d_ufront, void* stream)
+{
+    if(n && (!d_controls || !d_steps || !d_ufront)) return fail(KGMT_E_ARG, "null buffer");
+    StageDesc sd((hipStream_t)stream);
+    int rc = sd.make(cfg);
+    if(rc) return rc;
+    int S, C;
+    model_dims(cfg->model, S, C);
+    HIP_TRY(launch_stage_draw(sd.d, C, query, itr, u0, n, d_controls, d_steps, d_ufront, sd.s));
+    return KGMT_OK;
+}
+
+int kgmt_propagate_collide(const kgmt_config* cfg, const int32_t* d_parent_idx, const float* d_parent_states,
+                           uint32_t n_parents, const float* d_controls, const int32_t* d_steps, uint32_t n_children,
+                           const float* d_obstacles, uint32_t n_obs, float* d_child_samples, uint8_t* d_valid,
+                           int32_t* d_r1, int32_t* d_r2, void* stream)
+{
+    if(n_children && (!d_parent_idx || !d_parent_states || !d_controls || !d_steps || !d_child_samples || !d_valid ||
+                      !d_r1 || !d_r2))
+        return fail(KGMT_E_ARG, "null buffer");
+    if(n_obs > kMaxObs) return fail(KGMT_E_CAPACITY, "more than 2048 obstacles");
+    if(n_obs && !d_obstacles) return fail(KGMT_E_ARG, "obstacles pointer is null");
+    StageDesc sd((hipStream_t)stream);
+    int rc = sd.make(cfg);
+    if(rc) return rc;
+    float4* packed = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&packed, sizeof(float4) * 2 * std::max<uint32_t>(n_obs, 1), sd.s));
+    if(n_obs) hipLaunchKernelGGL(k_pack_obstacles, dim3((n_obs + 255) / 256), dim3(256), 0, sd.s, d_obstacles, n_obs, packed);
+    hipError_t e = launch_stage_propagate(cfg->model, sd.d, d_parent_idx, d_parent_states, n_parents, d_controls,
+                                          d_steps, n_children, packed, n_obs, d_child_samples, d_valid, d_r1, d_r2,
+                                          sd.s);
+    (void)hipFreeAsync(packed, sd.s);
+    if(e != hipSuccess) return fail(KGMT_E_HIP, std::string("propagate: ") + hipGetErrorString(e));
+    return KGMT_OK;
+}
+
+int kgmt_region_update(const kgmt_config* cfg, const int32_t* d_r1, const int32_t* d_r2, const uint8_t* d_valid,
+                       const float* d_ufront, uint32_t n, const float* d_score, const uint32_t* d_active_prev,
+                       uint32_t* d_active_next, int32_t* d_total, int32_t* d_valid_count, uint8_t* d_fnext,
+                       void* stream)
+{
+    if(n && (!d_r1 || !d_r2 || !d_valid || !d_ufront || !d_score || !d_active_prev || !d_active_next || !d_total ||
+             !d_valid_count || !d_fnext))
+        return fail(KGMT_E_ARG, "null buffer");
+    StageDesc sd((hipStream_t)stream);
+    int rc = sd.make(cfg);
+    if(rc) return rc;
+    HIP_TRY(launch_stage_region_update(sd.d, d_r1, d_r2, d_valid, d_ufront, n, d_score, d_active_prev, d_active_next,
+                                       d_total, d_valid_count, d_fnext, sd.s));
+    return KGMT_OK;
+}
+
+int kgmt_score(const kgmt_config* cfg, const int32_t* d_total, const int32_t* d_valid_count,
+               const uint32_t* d_active, float* d_score, void* stream)
+{
+    if(!d_total || !d_valid_count || !d_active || !d_score) return fail(KGMT_E_ARG, "null buffer");
+    StageDesc sd((hipStream_t)stream);
+    int rc = sd.make(cfg);
+    if(rc) return rc;
+    HIP_TRY(launch_stage_score(sd.d, d_total, d_valid_count, d_active, d_score, sd.s));
+    return KGMT_OK;
+}
+
+int kgmt_compact(const uint8_t* d_flags, uint32_t n, uint32_t* d_out, uint32_t* d_count, void* stream)
+{
+    if(!d_count || (n && (!d_flags || !d_out))) return fail(KGMT_E_ARG, "null buffer");
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t* tmp = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&tmp, sizeof(uint32_t) * ((n + kChunk - 1) / kChunk + 1), s));
+    hipError_t e = launch_stage_compact(d_flags, n, d_out, d_count, tmp, s);
+    (void)hipFreeAsync(tmp, s);
+    if(e != hipSuccess) return fail(KGMT_E_HIP, std::string("compact: ") + hipGetErrorString(e));
+    return KGMT_OK;
+}
+
+int kgmt_repeat(const uint32_t* d_ids, const uint32_t* d_counts, uint32_t n, uint32_t* d_out, uint32_t out_cap,
+                uint32_t* d_total, void* stream)
+{
+    if(!d_total || (n && (!d_ids || !d_counts)) || (out_cap && !d_out)) return fail(KGMT_E_ARG, "null buffer");
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t* tmp = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&tmp, sizeof(uint32_t) * ((n + kChunk - 1) / kChunk + 1), s));
+    hipError_t e = launch_stage_repeat(d_ids, d_counts, n, d_out, out_cap, d_total, tmp, s);
+    (void)hipFreeAsync(tmp, s);
+    if(e != hipSuccess) return fail(KGMT_E_HIP, std::string("repeat: ") + hipGetErrorString(e));
+    return KGMT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,916 @@
+// kgmt_kernels.hip — HIP kernels of the KGMT per-iteration expansion path (gfx950).
+//
+// One iteration of the planner is five launches over a table of queries
+// (a single-query plan is a table of one):
+//   k_propagate  persistent; one lane per child: Philox draws, RK4 rollout,
+//                AABB test against obstacles staged in LDS, child written SoA,
+//                region counters via wave-aggregated atomics, subregion bitset,
+//                next-frontier flag, per-chunk flag counts.
+//                (KGMT.cu:218-299, statePropagator.cu, collisionCheck.cu)
+//   k_score      one workgroup per query: region scores with a fixed pairwise
+//                summation order, bitset snapshot, scan of the flag counts.
+//                (Graph.cu:207-358, KGMT.cu:383-388)
+//   k_append     persistent over 1024-element chunk jobs: compaction of the
+//                flagged children and tree append, goal test; re-activation
+//                of old nodes. The last job of a query scans its partials.
+//                (KGMT.cu:305-378)
+//   k_expand     same jobs: writes the next expansion list (segmented repeat).
+//                (KGMT.cu:170-181, Planner.cu:52-80)
+//   k_plan       one workgroup per query: tree size, goal path, next launch
+//                regime (KGMT.cu:183-201); the last workgroup rebuilds the
+//                work lists of all queries.
+// All sizes stay on the device: the host only polls a completion word.
+#include "kgmt_launch.hpp"
+
+namespace kgmt
+{
+
+// ---------------------------------------------------------------------------
+// shared helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void stage_obstacles(const float4* __restrict__ pool, uint32_t off, uint32_t n,
+                                                float4* sObs)
+{
+    for(uint32_t i = threadIdx.x; i < 2 * n; i += blockDim.x) sObs[i] = pool[2 * (size_t)off + i];
+}
+
+// total[key]++ for lanes with `has`, valid[key]++ for lanes with `has && ok`:
+// one atomic per distinct key in the wave (leader loop over the ballot).
+__device__ __forceinline__ void wave_count(int32_t* total, int32_t* validc, int key, bool has, bool ok)
+{
+    uint64_t pending = __ballot(has);
+    while(pending)
+        {
+            const int leader    = __ffsll((unsigned long long)pending) - 1;
+            const int k         = __shfl(key, leader, 64);
+            const uint64_t same = __ballot(has && key == k);
+            const uint64_t good = __ballot(has && ok && key == k);
+            if((int)lane_id() == leader)
+                {
+                    atomicAdd(&total[k], (int32_t)__popcll(same));
+                    if(good) atomicAdd(&validc[k], (int32_t)__popcll(good));
+                }
+            pending &= ~same;
+        }
+}
+
+// Region scores of one query (block of NT threads). a: LDS of at least
+// pow2(R1) floats. Q6 semantics: closed formula, pairwise sum.
+template <int NT>
+__device__ void score_regions(const QueryDesc& qd, const int32_t* __restrict__ total,
+                              const int32_t* __restrict__ validc, const uint32_t* __restrict__ act,
+                              float* __restrict__ score, float* a)
+{
+    const int R1    = qd.R1;
+    const uint32_t P = (uint32_t)qd.P;
+    const float eps = qd.cfg.epsilon;
+    int P2          = 1;
+    while(P2 < R1) P2 <<= 1;
+    for(int r = threadIdx.x; r < P2; r += NT)
+        {
+            float s = 0.0f;
+            if(r < R1 && validc[r] > 0)
+                {
+                    const uint32_t cnt = count_bits(act, (uint32_t)r * P, P);
+                    const float cov    = (float)cnt / (float)P;
+                    const float vf     = (float)validc[r];
+                    const float fv     = (eps + vf) / ((eps + vf) + (float)(total[r] - validc[r])) * qd.vol;
+                    const float f2     = fv * fv;
+                    const float f4     = f2 * f2;
+                    const float t      = (float)total[r];
+                    s                  = f4 / ((1.0f + cov) * (1.0f + t * t));
+                }
+            a[r] = s;
+            if(r < R1) score[r] = s;
+        }
+    __syncthreads();
+    for(int st = P2 / 2; st >= 1; st >>= 1)
+        {
+            for(int i = threadIdx.x; i < st; i += NT) a[i] = a[i] + a[i + st];
+            __syncthreads();
+        }
+    const float sum = a[0];
+    for(int r = threadIdx.x; r < R1; r += NT) score[r] = validc[r] == 0 ? 1.0f : eps + score[r] / sum;
+    __syncthreads();
+}
+
+__device__ __forceinline__ float faccept(const QueryDesc& qd, uint32_t itr, uint32_t tree, uint32_t next)
+{
+    const float a  = 1.0f - (float)(tree + next) / (float)qd.cap;
+    const float a2 = a * a;
+    const float a5 = (a2 * a2) * a;
+    return ((float)itr * qd.cfg.epsilon) * a5;
+}
+
+// children per expansion entry and child count (KGMT.cu:183-201, Q9)
+__device__ __forceinline__ void launch_regime(const QueryDesc& qd, uint32_t R, uint32_t tree, uint32_t& per,
+                                              uint32_t& nChildren)
+{
+    const uint32_t B     = qd.cfg.children_per_entry;
+    const uint32_t avail = qd.cap - tree;
+    if((uint64_t)R * B <= avail)
+        per = B;
+    else
+        {
+            const uint32_t qv = avail / R;
+            per               = qv > B ? B : qv;
+            if(per == 0) per = 1;
+        }
+    const uint64_t nc = (uint64_t)R * per;
+    nChildren         = nc > qd.cap ? qd.cap : (uint32_t)nc;
+}
+
+// Work lists for the next launches (block of NT threads, last workgroup).
+template <int NT>
+__device__ void build_worklists(const QueryState* sts, const WorkLists& W, uint32_t nq, uint32_t* sbuf)
+{
+    uint32_t carryT = 0, carryJ = 0, carryA = 0;
+    for(uint32_t base = 0; base < nq; base += NT)
+        {
+            const uint32_t q = base + threadIdx.x;
+            uint32_t tiles = 0, jobs = 0, alive = 0;
+            if(q < nq)
+                {
+                    const uint32_t done = __hip_atomic_load(&sts[q].done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if(!done)
+                        {
+                            const uint32_t nc = __hip_atomic_load(&sts[q].nChildren, __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_AGENT);
+                            const uint32_t tr = __hip_atomic_load(&sts[q].treeSize, __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_AGENT);
+                            tiles             = (nc + kTile - 1) / kTile;
+                            jobs              = (nc + kChunk - 1) / kChunk + (tr + kChunk - 1) / kChunk;
+                            alive             = 1;
+                        }
+                }
+            uint32_t tT, tJ, tA;
+            const uint32_t eT = block_excl_scan<NT>(tiles, sbuf, tT);
+            const uint32_t eJ = block_excl_scan<NT>(jobs, sbuf, tJ);
+            block_excl_scan<NT>(alive, sbuf, tA);
+            if(q < nq)
+                {
+                    W.tileStart[q] = carryT + eT;
+                    W.jobStart[q]  = carryJ + eJ;
+                }
+            carryT += tT, carryJ += tJ, carryA += tA;
+        }
+    if(threadIdx.x == 0)
+        {
+            W.tileStart[nq] = carryT;
+            W.jobStart[nq]  = carryJ;
+            *W.allDone      = carryA == 0 ? 1u : 0u;
+            *W.counter      = 0;
+        }
+}
+
+// Returns true in every thread of the last workgroup to arrive (all writes of
+// the earlier ones are visible to it afterwards).
+__device__ __forceinline__ bool last_arrival(uint32_t* counter, uint32_t expected, uint32_t* sflag)
+{
+    __threadfence();
+    __syncthreads();
+    if(threadIdx.x == 0)
+        {
+            const uint32_t prev = atomicAdd(counter, 1u);
+            *sflag              = prev == expected - 1 ? 1u : 0u;
+        }
+    __syncthreads();
+    const bool last = *sflag != 0;
+    if(last) __threadfence();
+    return last;
+}
+
+// ---------------------------------------------------------------------------
+// init: one workgroup per query (KGMT.cu:50-81)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_init(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
+                                                 Pools P, WorkLists W, uint32_t nq)
+{
+    __shared__ uint32_t sbuf[kBlock / 64 + 1];
+    __shared__ uint32_t sflag;
+    const uint32_t q    = blockIdx.x;
+    const QueryDesc& qd = qds[q];
+    QueryState& st      = sts[q];
+    for(int r = threadIdx.x; r < qd.R1; r += kBlock)
+        {
+            P.total[qd.regOff + r]  = 0;
+            P.validc[qd.regOff + r] = 0;
+            P.score[qd.regOff + r]  = 0.0f;
+        }
+    for(int w = threadIdx.x; w < qd.R2words; w += kBlock)
+        {
+            P.actPrev[qd.subOff + w] = 0u;
+            P.actNext[qd.subOff + w] = 0u;
+        }
+    for(uint32_t c = threadIdx.x; c <= qd.nChunks; c += kBlock) P.partA[qd.chunkOff + c] = 0u;
+    const uint32_t rootRep = qd.cfg.root_repeat < qd.cap ? qd.cfg.root_repeat : qd.cap;
+    for(uint32_t k = threadIdx.x; k < rootRep; k += kBlock) P.repIdx[qd.nodeOff + k] = 0u;
+    for(int d = threadIdx.x; d < qd.D; d += kBlock) P.treeS[qd.sampleOff + (size_t)d * qd.cap] = qd.start[d];
+    if(threadIdx.x == 0)
+        {
+            P.parent[qd.nodeOff]   = -1;
+            P.cost[qd.nodeOff]     = 0.0f;
+            P.frontier[qd.nodeOff] = 1;
+            int ix, iy, iz;
+            P.r1node[qd.nodeOff] = region_of(qd, qd.start[0], qd.start[1], qd.start[2], ix, iy, iz);
+            st.itr               = 1;
+            st.treeSize          = 1;
+            st.R                 = qd.cfg.root_repeat;
+            uint32_t per, nc;
+            launch_regime(qd, st.R, 1, per, nc);
+            st.per       = per;
+            st.nChildren = nc;
+            st.nReact = 0, st.nAcc = 0, st.next = 0;
+            st.fAccept = 0.0f;
+            st.done = 0, st.success = 0;
+            st.goalIdx  = kNoGoal;
+            st.pathRows = 0;
+            st.pathCost = 0.0f, st.pathLen = 0.0f;
+            st.jobsDone  = 0;
+            st.expanded  = 0;
+            st.finishItr = 0;
+        }
+    if(last_arrival(W.counter, nq, &sflag)) build_worklists<kBlock>(sts, W, nq, sbuf);
+}
+
+// ---------------------------------------------------------------------------
+// propagate: persistent over 256-child tiles
+// ---------------------------------------------------------------------------
+template <int MODEL>
+__device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState& st, uint32_t q, uint32_t u,
+                                           const Pools& P, const float4* sObs)
+{
+    constexpr int S = ModelDims<MODEL>::S, C = ModelDims<MODEL>::C, D = S + C + 1;
+    const uint32_t nC   = st.nChildren;
+    const bool has      = u < nC;
+    const uint32_t cap  = qd.cap;
+    const size_t nodeO  = qd.nodeOff;
+    int r1              = 0;
+    bool ok = false, fn = false;
+    if(has)
+        {
+            const uint32_t per = st.per;
+            const uint32_t p   = P.repIdx[nodeO + u / per];
+            const float* T     = P.treeS + qd.sampleOff;
+            float s[S];
+#pragma unroll
+            for(int d = 0; d < S; ++d) s[d] = T[(size_t)d * cap + p];
+            float uc[C];
+            int32_t steps;
+            float uf;
+            draw_child<C>(qd.cfg, q, st.itr, u, uc, steps, uf);
+            ok       = propagate<MODEL>(qd, s, uc, steps, sObs, (int)qd.nObs);
+            float* U = P.childS + qd.sampleOff;
+#pragma unroll
+            for(int d = 0; d < S; ++d) U[(size_t)d * cap + u] = s[d];
+#pragma unroll
+            for(int i = 0; i < C; ++i) U[(size_t)(S + i) * cap + u] = uc[i];
+            U[(size_t)(D - 1) * cap + u] = qd.cfg.dt * (float)steps;
+            P.parentU[nodeO + u]          = (int32_t)p;
+            if(u % per == 0) P.frontier[nodeO + p] = 0;
+            int ix, iy, iz;
+            r1                   = region_of(qd, s[0], s[1], s[2], ix, iy, iz);
+            P.r1child[nodeO + u] = r1;
+            if(ok)
+                {
+                    const int r2    = subregion_of(qd, s[0], s[1], s[2], r1, ix, iy, iz);
+                    const bool seen = test_bit(P.actPrev + qd.subOff, (uint32_t)r2);
+                    fn              = (uf < P.score[qd.regOff + r1]) || !seen;
+                    if(!seen) atomicOr(&P.actNext[qd.subOff + ((uint32_t)r2 >> 5)], 1u << ((uint32_t)r2 & 31u));
+                }
+            P.fnext[nodeO + u] = fn ? 1 : 0;
+        }
+    wave_count(P.total + qd.regOff, P.validc + qd.regOff, r1, has, ok);
+    const uint64_t fm = __ballot(fn);
+    if(lane_id() == 0 && fm) atomicAdd(&P.partA[qd.chunkOff + u / kChunk], (uint32_t)__popcll(fm));
+}
+
+template <int MODEL>
+__global__ __launch_bounds__(kTile) void k_propagate(const QueryDesc* __restrict__ qds,
+                                                     const QueryState* __restrict__ sts, Pools P, WorkLists W,
+                                                     uint32_t nq)
+{
+    extern __shared__ float4 sObs[];
+    const uint32_t nTiles = W.tileStart[nq];
+    uint32_t staged       = 0xffffffffu;
+    for(uint32_t t = blockIdx.x; t < nTiles; t += gridDim.x)
+        {
+            const uint32_t q    = locate(W.tileStart, nq, t);
+            const QueryDesc& qd = qds[q];
+            if(q != staged)
+                {
+                    __syncthreads();
+                    stage_obstacles(P.obs, qd.obsOff, qd.nObs, sObs);
+                    __syncthreads();
+                    staged = q;
+                }
+            const uint32_t u = (t - W.tileStart[q]) * kTile + threadIdx.x;
+            if constexpr(MODEL >= 0)
+                child_tile<MODEL>(qd, sts[q], q, u, P, sObs);
+            else
+                {
+                    switch(qd.cfg.model)
+                        {
+                            case KGMT_MODEL_DOUBLE_INTEGRATOR:
+                                child_tile<KGMT_MODEL_DOUBLE_INTEGRATOR>(qd, sts[q], q, u, P, sObs);
+                                break;
+                            case KGMT_MODEL_DUBINS_AIRPLANE:
+                                child_tile<KGMT_MODEL_DUBINS_AIRPLANE>(qd, sts[q], q, u, P, sObs);
+                                break;
+                            case KGMT_MODEL_DUBINS_5D: child_tile<KGMT_MODEL_DUBINS_5D>(qd, sts[q], q, u, P, sObs); break;
+                            default: child_tile<KGMT_MODEL_QUADCOPTER>(qd, sts[q], q, u, P, sObs); break;
+                        }
+                }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// score: one workgroup (1024 threads) per query
+// ---------------------------------------------------------------------------
+constexpr int kScoreThreads = 1024;
+constexpr int kMaxRegions   = 8192;
+
+__global__ __launch_bounds__(kScoreThreads) void k_score(const QueryDesc* __restrict__ qds,
+                                                         QueryState* __restrict__ sts, Pools P)
+{
+    __shared__ float a[kMaxRegions];
+    __shared__ uint32_t sbuf[kScoreThreads / 64 + 1];
+    const uint32_t q = blockIdx.x;
+    QueryState& st   = sts[q];
+    if(st.done) return;
+    const QueryDesc& qd = qds[q];
+    score_regions<kScoreThreads>(qd, P.total + qd.regOff, P.validc + qd.regOff, P.actNext + qd.subOff,
+                                 P.score + qd.regOff, a);
+    for(int w = threadIdx.x; w < qd.R2words; w += kScoreThreads) P.actPrev[qd.subOff + w] = P.actNext[qd.subOff + w];
+    const uint32_t nCC  = (st.nChildren + kChunk - 1) / kChunk;
+    const uint32_t nAcc = block_scan_array<kScoreThreads>(P.partA + qd.chunkOff, nCC, sbuf);
+    if(threadIdx.x == 0)
+        {
+            const uint32_t avail = qd.cap - st.treeSize;
+            const uint32_t next  = nAcc < avail ? nAcc : avail;
+            st.nAcc              = nAcc;
+            st.next              = next;
+            st.fAccept           = faccept(qd, st.itr, st.treeSize, next);
+        }
+}
+
+// ---------------------------------------------------------------------------
+// append: child chunks (compaction + tree append + goal) and old-node chunks
+// (re-activation)
+// ---------------------------------------------------------------------------
+__device__ void append_children(const QueryDesc& qd, QueryState& st, uint32_t c, const Pools& P, uint32_t* sbuf)
+{
+    const uint32_t cap = qd.cap, nC = st.nChildren, tree = st.treeSize, next = st.next;
+    const size_t nodeO = qd.nodeOff;
+    const float* U     = P.childS + qd.sampleOff;
+    float* T           = P.treeS + qd.sampleOff;
+    const uint32_t offA = P.partA[qd.chunkOff + c];
+    uint32_t run = 0, repSum = 0;
+    for(uint32_t k = 0; k < kChunk / kBlock; ++k)
+        {
+            const uint32_t u = c * kChunk + k * kBlock + threadIdx.x;
+            const bool in    = u < nC;
+            const bool f     = in && P.fnext[nodeO + u] != 0;
+            uint32_t tot;
+            const uint32_t ex   = block_excl_scan<kBlock>(f ? 1u : 0u, sbuf, tot);
+            const uint32_t rank = offA + run + ex;
+            uint32_t rep        = 0;
+            if(f && rank < next)
+                {
+                    const uint32_t t = tree + rank;
+                    for(int d = 0; d < qd.D; ++d) T[(size_t)d * cap + t] = U[(size_t)d * cap + u];
+                    P.parent[nodeO + t]   = P.parentU[nodeO + u];
+                    const float dg        = goal_dist(qd, U[u], U[(size_t)cap + u], U[2 * (size_t)cap + u]);
+                    P.cost[nodeO + t]     = dg;
+                    P.frontier[nodeO + t] = 1;
+                    const int r1          = P.r1child[nodeO + u];
+                    P.r1node[nodeO + t]   = r1;
+                    rep = P.validc[qd.regOff + r1] < qd.cfg.sparse_valid_below ? qd.cfg.repeat_sparse
+                                                                                : qd.cfg.repeat_dense;
+                    if(dg < qd.cfg.goal_thresh) atomicMin(&st.goalIdx, (int32_t)t);
+                }
+            if(in) P.repU[nodeO + u] = rep;
+            repSum += rep;
+            run += tot;
+        }
+    uint32_t tot;
+    block_excl_scan<kBlock>(repSum, sbuf, tot);
+    if(threadIdx.x == 0) P.partC[qd.chunkOff + c] = tot;
+}
+
+__device__ void reactivate(const QueryDesc& qd, QueryState& st, uint32_t q, uint32_t o, const Pools& P,
+                           uint32_t* sbuf)
+{
+    const uint32_t tree = st.treeSize, itr = st.itr;
+    const float fA      = st.fAccept;
+    const size_t nodeO  = qd.nodeOff;
+    uint32_t cnt        = 0;
+    for(uint32_t k = 0; k < kChunk / kBlock; ++k)
+        {
+            const uint32_t i = o * kChunk + k * kBlock + threadIdx.x;
+            if(i < tree)
+                {
+                    bool fl = false;
+                    if(P.frontier[nodeO + i] == 0)
+                        {
+                            const float Ur = draw_react(qd.cfg, q, itr, i);
+                            if(Ur <= P.score[qd.regOff + P.r1node[nodeO + i]] + fA)
+                                {
+                                    fl                    = true;
+                                    P.frontier[nodeO + i] = 1;
+                                }
+                        }
+                    P.react[nodeO + i] = fl ? 1 : 0;
+                    cnt += fl ? 1u : 0u;
+                }
+        }
+    uint32_t tot;
+    block_excl_scan<kBlock>(cnt, sbuf, tot);
+    if(threadIdx.x == 0) P.partB[qd.chunkOff + o] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_append(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
+                                                   Pools P, WorkLists W, uint32_t nq)
+{
+    __shared__ uint32_t sbuf[kBlock / 64 + 1];
+    __shared__ uint32_t sflag;
+    const uint32_t nJobs = W.jobStart[nq];
+    for(uint32_t j = blockIdx.x; j < nJobs; j += gridDim.x)
+        {
+            const uint32_t q     = locate(W.jobStart, nq, j);
+            const QueryDesc& qd  = qds[q];
+            QueryState& st       = sts[q];
+            const uint32_t local = j - W.jobStart[q];
+            const uint32_t jobsQ = W.jobStart[q + 1] - W.jobStart[q];
+            const uint32_t nCC   = (st.nChildren + kChunk - 1) / kChunk;
+            if(local < nCC)
+                append_children(qd, st, local, P, sbuf);
+            else
+                reactivate(qd, st, q, local - nCC, P, sbuf);
+            if(last_arrival(&st.jobsDone, jobsQ, &sflag))
+                {
+                    const uint32_t nOC    = (st.treeSize + kChunk - 1) / kChunk;
+                    const uint32_t nReact = block_scan_array<kBlock>(P.partB + qd.chunkOff, nOC, sbuf);
+                    const uint32_t sumRep = block_scan_array<kBlock>(P.partC + qd.chunkOff, nCC, sbuf);
+                    if(threadIdx.x == 0)
+                        {
+                            st.nReact   = nReact;
+                            st.R        = nReact + sumRep;
+                            st.jobsDone = 0;
+                        }
+                }
+            __syncthreads();
+        }
+}
+
+// ---------------------------------------------------------------------------
+// expand: next expansion list (re-activated ascending, then new nodes each
+// repeated repU times), truncated at capacity entries
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__ qds,
+                                                   const QueryState* __restrict__ sts, Pools P, WorkLists W,
+                                                   uint32_t nq)
+{
+    __shared__ uint32_t sbuf[kBlock / 64 + 1];
+    const uint32_t nJobs = W.jobStart[nq];
+    for(uint32_t j = blockIdx.x; j < nJobs; j += gridDim.x)
+        {
+            const uint32_t q      = locate(W.jobStart, nq, j);
+            const QueryDesc& qd   = qds[q];
+            const QueryState& st  = sts[q];
+            const uint32_t local  = j - W.jobStart[q];
+            const uint32_t nC     = st.nChildren;
+            const uint32_t nCC    = (nC + kChunk - 1) / kChunk;
+            const uint32_t cap    = qd.cap, tree = st.treeSize;
+            const size_t nodeO    = qd.nodeOff;
+            uint32_t* out         = P.repIdx + nodeO;
+            if(local < nCC)
+                {
+                    const uint32_t c    = local;
+                    const uint32_t offA = P.partA[qd.chunkOff + c];
+                    const uint32_t offC = P.partC[qd.chunkOff + c] + st.nReact;
+                    uint32_t runN = 0, runS = 0;
+                    for(uint32_t k = 0; k < kChunk / kBlock; ++k)
+                        {
+                            const uint32_t u   = c * kChunk + k * kBlock + threadIdx.x;
+                            const uint32_t rep = u < nC ? P.repU[nodeO + u] : 0u;
+                            uint32_t tot;
+                            const uint32_t ex = block_excl_scan<kBlock>((rep ? (1u << 16) : 0u) | rep, sbuf, tot);
+                            if(rep)
+                                {
+                                    const uint32_t t   = tree + offA + runN + (ex >> 16);
+                                    const uint32_t pos = offC + runS + (ex & 0xffffu);
+                                    for(uint32_t m = 0; m < rep; ++m)
+                                        if(pos + m < cap) out[pos + m] = t;
+                                }
+                            runN += tot >> 16;
+                            runS += tot & 0xffffu;
+                        }
+                }
+            else
+                {
+                    const uint32_t o    = local - nCC;
+                    const uint32_t offB = P.partB[qd.chunkOff + o];
+                    uint32_t run        = 0;
+                    for(uint32_t k = 0; k < kChunk / kBlock; ++k)
+                        {
+                            const uint32_t i = o * kChunk + k * kBlock + threadIdx.x;
+                            const bool fl    = i < tree && P.react[nodeO + i] != 0;
+                            uint32_t tot;
+                            const uint32_t ex = block_excl_scan<kBlock>(fl ? 1u : 0u, sbuf, tot);
+                            const uint32_t pos = offB + run + ex;
+                            if(fl && pos < cap) out[pos] = i;
+                            run += tot;
+                        }
+                }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// plan: per-query bookkeeping of the iteration end; last workgroup rebuilds
+// the work lists (KGMT.cu:84-95, 183-201, 342-361, 399-402)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_plan(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
+                                                 Pools P, WorkLists W, uint32_t nq)
+{
+    __shared__ uint32_t sbuf[kBlock / 64 + 1];
+    __shared__ uint32_t sflag;
+    const uint32_t q    = blockIdx.x;
+    const QueryDesc& qd = qds[q];
+    QueryState& st      = sts[q];
+    if(!st.done)
+        {
+            for(uint32_t c = threadIdx.x; c <= qd.nChunks; c += kBlock) P.partA[qd.chunkOff + c] = 0u;
+            if(threadIdx.x == 0)
+                {
+                    const uint32_t tree = st.treeSize + st.next;
+                    st.treeSize         = tree;
+                    st.expanded += st.nChildren;
+                    if(st.goalIdx != kNoGoal)
+                        {
+                            // goal -> root rows, exact depth (Q13)
+                            const uint32_t cap = qd.cap;
+                            const int D        = qd.D;
+                            const float* T     = P.treeS + qd.sampleOff;
+                            float* out         = P.path + qd.pathOff;
+                            int32_t n          = st.goalIdx;
+                            uint32_t rows      = 0;
+                            float pc = 0.0f, pl = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
+                            const uint32_t maxRows = qd.cfg.max_iter + 2;
+                            while(n != -1 && rows < maxRows)
+                                {
+                                    for(int d = 0; d < D; ++d) out[rows * D + d] = T[(size_t)d * cap + n];
+                                    const float x = T[n], y = T[(size_t)cap + n], z = T[2 * (size_t)cap + n];
+                                    const int32_t par = P.parent[qd.nodeOff + n];
+                                    if(par != -1) pc = pc + T[(size_t)(D - 1) * cap + n];
+                                    if(rows > 0)
+                                        {
+                                            const float dx = px - x, dy = py - y, dz = pz - z;
+                                            pl             = pl + __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+                                        }
+                                    px = x, py = y, pz = z;
+                                    ++rows;
+                                    n = par;
+                                }
+                            st.pathRows  = rows;
+                            st.pathCost  = pc;
+                            st.pathLen   = pl;
+                            st.success   = 1;
+                            st.done      = 1;
+                            st.finishItr = st.itr;
+                        }
+                    else if(st.itr >= qd.cfg.max_iter)
+                        {
+                            st.done      = 1;
+                            st.finishItr = st.itr;
+                        }
+                    else
+                        {
+                            uint32_t per, nc;
+                            launch_regime(qd, st.R, tree, per, nc);
+                            st.per       = per;
+                            st.nChildren = nc;
+                            st.itr       = st.itr + 1;
+                        }
+                }
+        }
+    if(last_arrival(W.counter, nq, &sflag)) build_worklists<kBlock>(sts, W, nq, sbuf);
+}
+
+// ---------------------------------------------------------------------------
+// stage-level kernels (single query, caller buffers)
+// ---------------------------------------------------------------------------
+template <int C>
+__global__ __launch_bounds__(kBlock) void k_stage_draw(const QueryDesc* __restrict__ qd, uint32_t query,
+                                                       uint32_t itr, uint32_t u0, uint32_t n, float* controls,
+                                                       int32_t* steps, float* ufront)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if(i >= n) return;
+    float uc[C];
+    int32_t st;
+    float uf;
+    draw_child<C>(qd->cfg, query, itr, u0 + i, uc, st, uf);
+#pragma unroll
+    for(int k = 0; k < C; ++k) controls[(size_t)k * n + i] = uc[k];
+    steps[i]  = st;
+    ufront[i] = uf;
+}
+
+template <int MODEL>
+__global__ __launch_bounds__(kBlock) void k_stage_propagate(const QueryDesc* __restrict__ qdp,
+                                                            const int32_t* __restrict__ parentIdx,
+                                                            const float* __restrict__ parentStates,
+                                                            uint32_t nParents, const float* __restrict__ controls,
+                                                            const int32_t* __restrict__ steps, uint32_t n,
+                                                            const float4* __restrict__ obs, uint32_t nObs,
+                                                            float* child, uint8_t* valid, int32_t* r1o, int32_t* r2o)
+{
+    constexpr int S = ModelDims<MODEL>::S, C = ModelDims<MODEL>::C, D = S + C + 1;
+    extern __shared__ float4 sObs[];
+    const QueryDesc& qd = *qdp;
+    stage_obstacles(obs, 0, nObs, sObs);
+    __syncthreads();
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if(i >= n) return;
+    const int32_t p = parentIdx[i];
+    if(p < 0 || (uint32_t)p >= nParents)
+        {
+            // caller index out of range: no state is read; flagged by r1 = r2 = -1
+            for(int d = 0; d < D; ++d) child[(size_t)d * n + i] = 0.0f;
+            valid[i] = 0;
+            r1o[i]   = -1;
+            r2o[i]   = -1;
+            return;
+        }
+    float s[S], uc[C];
+#pragma unroll
+    for(int d = 0; d < S; ++d) s[d] = parentStates[(size_t)d * nParents + p];
+#pragma unroll
+    for(int k = 0; k < C; ++k) uc[k] = controls[(size_t)k * n + i];
+    const int32_t st = steps[i];
+    const bool ok    = propagate<MODEL>(qd, s, uc, st, sObs, (int)nObs);
+#pragma unroll
+    for(int d = 0; d < S; ++d) child[(size_t)d * n + i] = s[d];
+#pragma unroll
+    for(int k = 0; k < C; ++k) child[(size_t)(S + k) * n + i] = uc[k];
+    child[(size_t)(D - 1) * n + i] = qd.cfg.dt * (float)st;
+    int ix, iy, iz;
+    const int r1 = region_of(qd, s[0], s[1], s[2], ix, iy, iz);
+    valid[i]     = ok ? 1 : 0;
+    r1o[i]       = r1;
+    r2o[i]       = subregion_of(qd, s[0], s[1], s[2], r1, ix, iy, iz);
+}
+
+__global__ __launch_bounds__(kBlock) void k_stage_region_update(const QueryDesc* __restrict__ qdp,
+                                                                const int32_t* __restrict__ r1,
+                                                                const int32_t* __restrict__ r2,
+                                                                const uint8_t* __restrict__ valid,
+                                                                const float* __restrict__ ufront, uint32_t n,
+                                                                const float* __restrict__ score,
+                                                                const uint32_t* __restrict__ actPrev,
+                                                                uint32_t* actNext, int32_t* total, int32_t* validc,
+                                                                uint8_t* fnext)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const QueryDesc& qd = *qdp;
+    int key             = i < n ? r1[i] : 0;
+    const int sub       = i < n ? r2[i] : 0;
+    // children with a region id out of range are ignored
+    const bool has = i < n && key >= 0 && key < qd.R1 && sub >= 0 && sub < qd.R2;
+    if(!has) key = 0;
+    const bool ok = has && valid[i] != 0;
+    bool fn       = false;
+    if(ok)
+        {
+            const uint32_t b = (uint32_t)sub;
+            const bool seen  = test_bit(actPrev, b);
+            fn               = (ufront[i] < score[key]) || !seen;
+            if(!seen) atomicOr(&actNext[b >> 5], 1u << (b & 31u));
+        }
+    if(i < n) fnext[i] = fn ? 1 : 0;
+    wave_count(total, validc, key, has, ok);
+}
+
+__global__ __launch_bounds__(kScoreThreads) void k_stage_score(const QueryDesc* __restrict__ qd,
+                                                               const int32_t* __restrict__ total,
+                                                               const int32_t* __restrict__ validc,
+                                                               const uint32_t* __restrict__ act, float* score)
+{
+    __shared__ float a[kMaxRegions];
+    score_regions<kScoreThreads>(*qd, total, validc, act, score, a);
+}
+
+// compaction: chunk counts -> one-workgroup scan -> scatter
+__global__ __launch_bounds__(kBlock) void k_stage_count(const uint8_t* __restrict__ flags, uint32_t n, uint32_t* tmp)
+{
+    __shared__ uint32_t sbuf[kBlock / 64 + 1];
+    const uint32_t c = blockIdx.x;
+    uint32_t cnt     = 0;
+    for(uint32_t k = 0; k < kChunk / kBlock; ++k)
+        {
+            const uint32_t i = c * kChunk + k * kBlock + threadIdx.x;
+            cnt += (i < n && flags[i]) ? 1u : 0u;
+        }
+    uint32_t tot;
+    block_excl_scan<kBlock>(cnt, sbuf, tot);
+    if(threadIdx.x == 0) tmp[c] = tot;
+}
+
+__global__ __launch_bounds__(kScoreThreads) void k_stage_scan(uint32_t* tmp, uint32_t m, uint32_t* total)
+{
+    __shared__ uint32_t sbuf[kScoreThreads / 64 + 1];
+    const uint32_t t = block_scan_array<kScoreThreads>(tmp, m, sbuf);
+    if(threadIdx.x == 0) *total = t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_stage_scatter(const uint8_t* __restrict__ flags, uint32_t n,
+                                                          const uint32_t* __restrict__ tmp, uint32_t* out)
+{
+    __shared__ uint32_t sbuf[kBlock / 64 + 1];
+    const uint32_t c = blockIdx.x;
+    uint32_t run     = tmp[c];
+    for(uint32_t k = 0; k < kChunk / kBlock; ++k)
+        {
+            const uint32_t i = c * kChunk + k * kBlock + threadIdx.x;
+            const bool f     = i < n && flags[i] != 0;
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan<kBlock>(f ? 1u : 0u, sbuf, tot);
+            if(f) out[run + ex] = i;
+            run += tot;
+        }
+}
+
+// segmented repeat: chunk sums -> scan -> write
+__global__ __launch_bounds__(kBlock) void k_stage_sum(const uint32_t* __restrict__ counts, uint32_t n, uint32_t* tmp)
+{
+    __shared__ uint32_t sbuf[kBlock / 64 + 1];
+    const uint32_t c = blockIdx.x;
+    uint32_t sum     = 0;
+    for(uint32_t k = 0; k < kChunk / kBlock; ++k)
+        {
+            const uint32_t i = c * kChunk + k * kBlock + threadIdx.x;
+            sum += i < n ? counts[i] : 0u;
+        }
+    uint32_t tot;
+    block_excl_scan<kBlock>(sum, sbuf, tot);
+    if(threadIdx.x == 0) tmp[c] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_stage_repeat(const uint32_t* __restrict__ ids,
+                                                         const uint32_t* __restrict__ counts, uint32_t n,
+                                                         const uint32_t* __restrict__ tmp, uint32_t* out,
+                                                         uint32_t outCap)
+{
+    __shared__ uint32_t sbuf[kBlock / 64 + 1];
+    const uint32_t c = blockIdx.x;
+    uint32_t run     = tmp[c];
+    for(uint32_t k = 0; k < kChunk / kBlock; ++k)
+        {
+            const uint32_t i   = c * kChunk + k * kBlock + threadIdx.x;
+            const uint32_t cnt = i < n ? counts[i] : 0u;
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan<kBlock>(cnt, sbuf, tot);
+            for(uint32_t m = 0; m < cnt; ++m)
+                if(run + ex + m < outCap) out[run + ex + m] = ids[i];
+            run += tot;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_init(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, uint32_t nq,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(k_init, dim3(nq), dim3(kBlock), 0, s, qds, sts, P, W, nq);
+    return hipGetLastError();
+}
+
+hipError_t launch_propagate(int model, const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
+                            uint32_t nq, uint32_t maxObs, const Launch& L, hipStream_t s)
+{
+    const size_t lds = (size_t)maxObs * 2 * sizeof(float4);
+    const dim3 g(L.propGrid), b(kTile);
+    switch(model)
+        {
+            case KGMT_MODEL_DOUBLE_INTEGRATOR:
+                hipLaunchKernelGGL(k_propagate<KGMT_MODEL_DOUBLE_INTEGRATOR>, g, b, lds, s, qds, sts, P, W, nq);
+                break;
+            case KGMT_MODEL_DUBINS_AIRPLANE:
+                hipLaunchKernelGGL(k_propagate<KGMT_MODEL_DUBINS_AIRPLANE>, g, b, lds, s, qds, sts, P, W, nq);
+                break;
+            case KGMT_MODEL_DUBINS_5D:
+                hipLaunchKernelGGL(k_propagate<KGMT_MODEL_DUBINS_5D>, g, b, lds, s, qds, sts, P, W, nq);
+                break;
+            case KGMT_MODEL_QUADCOPTER:
+                hipLaunchKernelGGL(k_propagate<KGMT_MODEL_QUADCOPTER>, g, b, lds, s, qds, sts, P, W, nq);
+                break;
+            default: hipLaunchKernelGGL(k_propagate<-1>, g, b, lds, s, qds, sts, P, W, nq); break;
+        }
+    return hipGetLastError();
+}
+
+hipError_t launch_score(const QueryDesc* qds, QueryState* sts, const Pools& P, uint32_t nq, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_score, dim3(nq), dim3(kScoreThreads), 0, s, qds, sts, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_append(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, uint32_t nq,
+                         const Launch& L, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_append, dim3(L.chunkGrid), dim3(kBlock), 0, s, qds, sts, P, W, nq);
+    return hipGetLastError();
+}
+
+hipError_t launch_expand(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, uint32_t nq,
+                         const Launch& L, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_expand, dim3(L.chunkGrid), dim3(kBlock), 0, s, qds, sts, P, W, nq);
+    return hipGetLastError();
+}
+
+hipError_t launch_plan(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, uint32_t nq,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(k_plan, dim3(nq), dim3(kBlock), 0, s, qds, sts, P, W, nq);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_draw(const QueryDesc* qd, int C, uint32_t query, uint32_t itr, uint32_t u0, uint32_t n,
+                             float* controls, int32_t* steps, float* ufront, hipStream_t s)
+{
+    const dim3 g((n + kBlock - 1) / kBlock), b(kBlock);
+    if(n == 0) return hipSuccess;
+    switch(C)
+        {
+            case 3: hipLaunchKernelGGL(k_stage_draw<3>, g, b, 0, s, qd, query, itr, u0, n, controls, steps, ufront); break;
+            case 4: hipLaunchKernelGGL(k_stage_draw<4>, g, b, 0, s, qd, query, itr, u0, n, controls, steps, ufront); break;
+            default: return hipErrorInvalidValue;
+        }
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_propagate(int model, const QueryDesc* qd, const int32_t* parentIdx, const float* parentStates,
+                                  uint32_t nParents, const float* controls, const int32_t* steps, uint32_t n,
+                                  const float4* obs, uint32_t nObs, float* child, uint8_t* valid, int32_t* r1,
+                                  int32_t* r2, hipStream_t s)
+{
+    if(n == 0) return hipSuccess;
+    const dim3 g((n + kBlock - 1) / kBlock), b(kBlock);
+    const size_t lds = (size_t)(nObs ? nObs : 1) * 2 * sizeof(float4);
+#define KGMT_STAGE_PROP(M)                                                                                     \
+    hipLaunchKernelGGL(k_stage_propagate<M>, g, b, lds, s, qd, parentIdx, parentStates, nParents, controls, steps, \
+                       n, obs, nObs, child, valid, r1, r2)
+    switch(model)
+        {
+            case KGMT_MODEL_DOUBLE_INTEGRATOR: KGMT_STAGE_PROP(KGMT_MODEL_DOUBLE_INTEGRATOR); break;
+            case KGMT_MODEL_DUBINS_AIRPLANE: KGMT_STAGE_PROP(KGMT_MODEL_DUBINS_AIRPLANE); break;
+            case KGMT_MODEL_DUBINS_5D: KGMT_STAGE_PROP(KGMT_MODEL_DUBINS_5D); break;
+            case KGMT_MODEL_QUADCOPTER: KGMT_STAGE_PROP(KGMT_MODEL_QUADCOPTER); break;
+            default: return hipErrorInvalidValue;
+        }
+#undef KGMT_STAGE_PROP
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_region_update(const QueryDesc* qd, const int32_t* r1, const int32_t* r2, const uint8_t* valid,
+                                      const float* ufront, uint32_t n, const float* score, const uint32_t* actPrev,
+                                      uint32_t* actNext, int32_t* total, int32_t* validc, uint8_t* fnext,
+                                      hipStream_t s)
+{
+    if(n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stage_region_update, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, qd, r1, r2, valid,
+                       ufront, n, score, actPrev, actNext, total, validc, fnext);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_score(const QueryDesc* qd, const int32_t* total, const int32_t* validc, const uint32_t* act,
+                              float* score, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_stage_score, dim3(1), dim3(kScoreThreads), 0, s, qd, total, validc, act, score);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_compact(const uint8_t* flags, uint32_t n, uint32_t* out, uint32_t* count, uint32_t* tmp,
+                                hipStream_t s)
+{
+    const uint32_t m = (n + kChunk - 1) / kChunk;
+    if(m) hipLaunchKernelGGL(k_stage_count, dim3(m), dim3(kBlock), 0, s, flags, n, tmp);
+    hipLaunchKernelGGL(k_stage_scan, dim3(1), dim3(kScoreThreads), 0, s, tmp, m, count);
+    if(m) hipLaunchKernelGGL(k_stage_scatter, dim3(m), dim3(kBlock), 0, s, flags, n, tmp, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_repeat(const uint32_t* ids, const uint32_t* counts, uint32_t n, uint32_t* out,
+                               uint32_t outCap, uint32_t* total, uint32_t* tmp, hipStream_t s)
+{
+    const uint32_t m = (n + kChunk - 1) / kChunk;
+    if(m) hipLaunchKernelGGL(k_stage_sum, dim3(m), dim3(kBlock), 0, s, counts, n, tmp);
+    hipLaunchKernelGGL(k_stage_scan, dim3(1), dim3(kScoreThreads), 0, s, tmp, m, total);
+    if(m) hipLaunchKernelGGL(k_stage_repeat, dim3(m), dim3(kBlock), 0, s, ids, counts, n, tmp, out, outCap);
+    return hipGetLastError();
+}
+
+}  // namespace kgmt

@@ -1,0 +1,50 @@
+// kgmt_launch.hpp — host-side launchers of the kernels in kgmt_kernels.hip.
+// All launchers are asynchronous on `stream` and return the launch status.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kgmt_device.hpp"
+
+namespace kgmt
+{
+
+struct Launch
+{
+    uint32_t propGrid  = 1024;  // persistent workgroups of the propagate kernel
+    uint32_t propBlock = 256;   // threads per propagate workgroup (= kTile)
+    uint32_t chunkGrid = 1024;  // persistent workgroups of append / expand
+};
+
+// planner iteration kernels; `model` = KGMT_MODEL_* when every query shares it, -1 otherwise
+hipError_t launch_init(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, uint32_t nq,
+                       hipStream_t s);
+hipError_t launch_propagate(int model, const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
+                            uint32_t nq, uint32_t maxObs, const Launch& L, hipStream_t s);
+hipError_t launch_score(const QueryDesc* qds, QueryState* sts, const Pools& P, uint32_t nq, hipStream_t s);
+hipError_t launch_append(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, uint32_t nq,
+                         const Launch& L, hipStream_t s);
+hipError_t launch_expand(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, uint32_t nq,
+                         const Launch& L, hipStream_t s);
+hipError_t launch_plan(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, uint32_t nq,
+                       hipStream_t s);
+
+// stage-level kernels (one query, caller buffers)
+hipError_t launch_stage_draw(const QueryDesc* qd, int C, uint32_t query, uint32_t itr, uint32_t u0, uint32_t n,
+                             float* controls, int32_t* steps, float* ufront, hipStream_t s);
+hipError_t launch_stage_propagate(int model, const QueryDesc* qd, const int32_t* parentIdx, const float* parentStates,
+                                  uint32_t nParents, const float* controls, const int32_t* steps, uint32_t n,
+                                  const float4* obs, uint32_t nObs, float* child, uint8_t* valid, int32_t* r1,
+                                  int32_t* r2, hipStream_t s);
+hipError_t launch_stage_region_update(const QueryDesc* qd, const int32_t* r1, const int32_t* r2, const uint8_t* valid,
+                                      const float* ufront, uint32_t n, const float* score, const uint32_t* actPrev,
+                                      uint32_t* actNext, int32_t* total, int32_t* validc, uint8_t* fnext,
+                                      hipStream_t s);
+hipError_t launch_stage_score(const QueryDesc* qd, const int32_t* total, const int32_t* validc, const uint32_t* act,
+                              float* score, hipStream_t s);
+hipError_t launch_stage_compact(const uint8_t* flags, uint32_t n, uint32_t* out, uint32_t* count, uint32_t* tmp,
+                                hipStream_t s);
+hipError_t launch_stage_repeat(const uint32_t* ids, const uint32_t* counts, uint32_t n, uint32_t* out,
+                               uint32_t outCap, uint32_t* total, uint32_t* tmp, hipStream_t s);
+
+}  // namespace kgmt

@@ -1,0 +1,211 @@
+"""Python mirror of the reference planner interface over the C-ABI.
+
+    Planner  include/planners/Planner.cuh:19-44   (abstract plan + result members)
+    KGMT     include/planners/KGMT.cuh:5-36        (plan, planBench)
+
+Member names follow the reference so that code written against it reads the
+same: after plan(), h_treeSize_, h_itr_, h_pathToGoal_ (0 = not found, else
+the goal node index) and h_controlPathToGoal_ (rows goal -> root, D floats
+each) hold the results; tree arrays come from tree().
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _lib
+from .abi import KGMT_MAX_SAMPLE, KgmtConfig, KgmtResult, dims
+from .workloads import config as workload_config
+
+
+def _f32(a, n=None) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    if n is not None and a.size < n:
+        b = np.zeros(n, np.float32)
+        b[: a.size] = a.ravel()
+        a = b
+    return a
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _obstacle_arg(obstacles, n):
+    """(pointer, on_device, keepalive) for host arrays, torch tensors or raw
+    device addresses (ints, like the reference's d_obstacles_ptr)."""
+    if obstacles is None:
+        return None, 0, None
+    if isinstance(obstacles, int):
+        return C.c_void_p(obstacles), 1, None
+    mod = type(obstacles).__module__
+    if mod.startswith("torch"):
+        t = obstacles.contiguous().float()
+        return C.c_void_p(t.data_ptr()), 1 if t.is_cuda else 0, t
+    a = _f32(obstacles).reshape(-1)
+    if a.size < 6 * n:
+        raise ValueError("obstacle array shorter than 6 * count")
+    return _ptr(a), 0, a
+
+
+class Planner:
+    """include/planners/Planner.cuh:19-44"""
+
+    def __init__(self):
+        self.h_treeSize_ = 0
+        self.h_itr_ = 0
+        self.h_costToGoal_ = 0.0
+        self.h_pathToGoal_ = 0
+        self.h_controlPathToGoal_ = np.zeros((0, 0), np.float32)
+
+    def plan(self, h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount):  # pragma: no cover - abstract
+        raise NotImplementedError
+
+
+class KGMT(Planner):
+    """include/planners/KGMT.cuh:5-36, backed by libkgmt (HIP, gfx950)."""
+
+    def __init__(self, cfg: KgmtConfig | None = None):
+        super().__init__()
+        self.cfg = (cfg or workload_config("reference_quad")).copy()
+        self.S, self.C, self.D = dims(self.cfg.model)
+        L = _lib.lib()
+        h = C.c_void_p()
+        _lib.check(L.kgmt_create(C.byref(self.cfg), C.byref(h)))
+        self._h = h
+        self.result = KgmtResult()
+        self.h_pathRows_ = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().kgmt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, hip_stream: int | None):
+        _lib.check(_lib.lib().kgmt_set_stream(self._h, C.c_void_p(hip_stream or 0)))
+
+    # KGMT::plan, src/planners/KGMT.cu:41-105
+    def plan(self, h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount) -> KgmtResult:
+        start = _f32(h_initial, self.D)
+        goal = _f32(h_goal, self.D)
+        n = int(h_obstaclesCount)
+        p, on_dev, keep = _obstacle_arg(d_obstacles_ptr, n)
+        res = KgmtResult()
+        _lib.check(_lib.lib().kgmt_plan(self._h, _ptr(start), _ptr(goal), p, n, on_dev, C.byref(res)))
+        del keep
+        self.result = res
+        self.h_treeSize_ = res.tree_size
+        self.h_itr_ = res.iterations
+        self.h_pathToGoal_ = res.goal_index if res.success else 0
+        self.h_costToGoal_ = res.path_cost if res.success else 0.0
+        self.h_pathRows_ = res.path_rows
+        self.h_controlPathToGoal_ = self.path() if res.success else np.zeros((0, self.D), np.float32)
+        return res
+
+    # KGMT::planBench, src/planners/KGMT.cu:107-165: plan, then dump the tree
+    # as CSV under out_dir like writeDeviceVectorsToCSV (KGMT.cu:405-488)
+    def planBench(self, h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount, benchItr: int,
+                  out_dir: str = "Data") -> KgmtResult:
+        res = self.plan(h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount)
+        from .export import write_plan_csv
+        write_plan_csv(self, out_dir, benchItr)
+        return res
+
+    def tree(self):
+        """(samples (n, D), parents (n,), costs (n,)) of the last plan."""
+        L = _lib.lib()
+        n = C.c_uint32()
+        _lib.check(L.kgmt_get_tree(self._h, None, None, None, 0, C.byref(n)))
+        k = n.value
+        s = np.zeros((k, self.D), np.float32)
+        p = np.zeros(k, np.int32)
+        c = np.zeros(k, np.float32)
+        _lib.check(L.kgmt_get_tree(self._h, _ptr(s), _ptr(p), _ptr(c), k, C.byref(n)))
+        return s, p, c
+
+    def path(self) -> np.ndarray:
+        L = _lib.lib()
+        cap = self.cfg.max_iter + 2
+        rows = np.zeros((cap, self.D), np.float32)
+        n = C.c_uint32()
+        _lib.check(L.kgmt_get_path(self._h, _ptr(rows), cap, C.byref(n)))
+        return rows[: n.value].copy()
+
+    def regions(self):
+        L = _lib.lib()
+        n = C.c_uint32()
+        _lib.check(L.kgmt_get_regions(self._h, None, None, None, 0, C.byref(n)))
+        k = n.value
+        t = np.zeros(k, np.int32)
+        v = np.zeros(k, np.int32)
+        s = np.zeros(k, np.float32)
+        _lib.check(L.kgmt_get_regions(self._h, _ptr(t), _ptr(v), _ptr(s), k, C.byref(n)))
+        return t, v, s
+
+
+class BatchKGMT:
+    """Many independent queries served by one launch sequence per iteration."""
+
+    def __init__(self, cfgs):
+        self.cfgs = [c.copy() for c in cfgs]
+        arr = (KgmtConfig * len(self.cfgs))(*self.cfgs)
+        h = C.c_void_p()
+        _lib.check(_lib.lib().kgmt_batch_create(arr, len(self.cfgs), C.byref(h)))
+        self._h = h
+        self.n = len(self.cfgs)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().kgmt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_launch(self, propagate_grid=0, chunk_grid=0):
+        _lib.check(_lib.lib().kgmt_batch_set_launch(self._h, 0, int(propagate_grid), int(chunk_grid)))
+
+    def plan(self, starts, goals, obstacles, offsets, counts, use_graph=True):
+        starts = _f32(starts).reshape(self.n, KGMT_MAX_SAMPLE)
+        goals = _f32(goals).reshape(self.n, KGMT_MAX_SAMPLE)
+        obs = _f32(obstacles).reshape(-1)
+        if obs.size == 0:
+            obs = np.zeros(6, np.float32)
+        offs = np.ascontiguousarray(offsets, np.uint32)
+        cnts = np.ascontiguousarray(counts, np.uint32)
+        res = (KgmtResult * self.n)()
+        fin = np.zeros(self.n, np.float32)
+        _lib.check(_lib.lib().kgmt_batch_plan(self._h, _ptr(starts), _ptr(goals), _ptr(obs), _ptr(offs),
+                                              _ptr(cnts), 1 if use_graph else 0, res, _ptr(fin)))
+        return [res[i] for i in range(self.n)], fin
+
+    def path(self, q: int) -> np.ndarray:
+        D = dims(self.cfgs[q].model)[2]
+        cap = self.cfgs[q].max_iter + 2
+        rows = np.zeros((cap, D), np.float32)
+        n = C.c_uint32()
+        _lib.check(_lib.lib().kgmt_batch_get_path(self._h, q, _ptr(rows), cap, C.byref(n)))
+        return rows[: n.value].copy()
+
+    def tree(self, q: int):
+        L = _lib.lib()
+        D = dims(self.cfgs[q].model)[2]
+        n = C.c_uint32()
+        _lib.check(L.kgmt_batch_get_tree(self._h, q, None, None, None, 0, C.byref(n)))
+        k = n.value
+        s = np.zeros((k, D), np.float32)
+        p = np.zeros(k, np.int32)
+        c = np.zeros(k, np.float32)
+        _lib.check(L.kgmt_batch_get_tree(self._h, q, _ptr(s), _ptr(p), _ptr(c), k, C.byref(n)))
+        return s, p, c

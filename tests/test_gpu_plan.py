@@ -1,0 +1,105 @@
+"""Whole plans on the GPU against the CPU oracle: same tree, node by node.
+
+Small capacities keep the oracle to seconds; the GPU runs exactly the same
+iterations, so every sample, parent, cost, region counter and score must be
+identical, and so must the path."""
+import numpy as np
+import pytest
+
+from kgmt_amd import workloads as W
+from kgmt_amd.abi import KGMT_DI_EXACT
+from kgmt_amd.planner import KGMT, BatchKGMT
+
+pytestmark = pytest.mark.gpu
+
+
+def _same_plan(gpu: KGMT, res, ref):
+    st = ref["stats"]
+    assert res.iterations == st["iterations"]
+    assert res.tree_size == st["tree_size"]
+    assert res.expanded == st["expanded"]
+    assert (res.goal_index if res.success else -1) == st["goal_index"]
+    assert res.path_rows == st["path_rows"]
+    s, p, c = gpu.tree()
+    assert np.array_equal(p, ref["parents"])
+    assert np.array_equal(s.view(np.uint32), ref["samples"].view(np.uint32))
+    assert np.array_equal(c.view(np.uint32), ref["costs"].view(np.uint32))
+    t, v, sc = gpu.regions()
+    assert np.array_equal(t, ref["total"]) and np.array_equal(v, ref["valid"])
+    assert np.array_equal(sc.view(np.uint32), ref["score"].view(np.uint32))
+    if res.success:
+        assert np.array_equal(gpu.path().view(np.uint32), ref["path"].view(np.uint32))
+        assert np.float32(res.path_cost) == np.float32(st["path_cost"])
+        assert np.float32(res.path_length) == np.float32(st["path_length"])
+
+
+PLAN_CASES = [
+    ("di6d", 0, dict(capacity=1 << 14)),
+    ("di6d", 1, dict(capacity=1 << 15, integrator=KGMT_DI_EXACT)),
+    ("dubins5d", 0, dict(capacity=1 << 15, max_iter=40)),
+    ("dubins5d", 2, dict(capacity=5000, max_iter=30)),
+    ("quad12d", 0, dict(capacity=1 << 13, max_iter=12)),
+]
+
+
+@pytest.mark.parametrize("name,seed,kw", PLAN_CASES)
+def test_plan_matches_oracle(name, seed, kw, orc):
+    q = W.make_query(name, seed, **kw)
+    ref = orc.plan(q.cfg, q.start, q.goal, q.obstacles)
+    g = KGMT(q.cfg)
+    res = g.plan(q.start, q.goal, q.obstacles, len(q.obstacles))
+    _same_plan(g, res, ref)
+    # a second plan on the same context starts from scratch
+    res2 = g.plan(q.start, q.goal, q.obstacles, len(q.obstacles))
+    _same_plan(g, res2, ref)
+
+
+def test_plan_device_obstacles(orc):
+    torch = pytest.importorskip("torch")
+    q = W.make_query("di6d", 4, capacity=1 << 14)
+    ref = orc.plan(q.cfg, q.start, q.goal, q.obstacles)
+    g = KGMT(q.cfg)
+    d_obs = torch.from_numpy(q.obstacles.reshape(-1)).cuda()
+    res = g.plan(q.start, q.goal, int(d_obs.data_ptr()), len(q.obstacles))
+    _same_plan(g, res, ref)
+
+
+def test_plan_no_obstacles_and_unreachable(orc):
+    q = W.make_query("di6d", 5, capacity=1 << 13, max_iter=15)
+    empty = np.zeros((0, 6), np.float32)
+    ref = orc.plan(q.cfg, q.start, q.goal, empty)
+    g = KGMT(q.cfg)
+    res = g.plan(q.start, q.goal, empty, 0)
+    _same_plan(g, res, ref)
+    # goal enclosed by a box: never reached, runs max_iter iterations
+    box = np.array([[0.85, 0.85, 0.85, 1.0, 1.0, 1.0]], np.float32)
+    ref = orc.plan(q.cfg, q.start, q.goal, box)
+    res = g.plan(q.start, q.goal, box, 1)
+    assert not res.success and res.iterations == q.cfg.max_iter
+    _same_plan(g, res, ref)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_matches_oracle(use_graph, orc):
+    qs = W.make_batch(12, seed=3, max_iter=25)
+    for q in qs:  # keep the oracle fast
+        q.cfg = q.cfg.replace(capacity=min(q.cfg.capacity, 1 << 13))
+    cfgs, starts, goals, obs, offs, cnts = W.pack_batch(qs)
+    b = BatchKGMT(cfgs)
+    results, fin = b.plan(starts, goals, obs, offs, cnts, use_graph=use_graph)
+    for i, q in enumerate(qs):
+        ref = orc.plan(q.cfg, q.start, q.goal, q.obstacles, query=i)
+        st = ref["stats"]
+        r = results[i]
+        assert r.iterations == st["iterations"], i
+        assert r.tree_size == st["tree_size"], i
+        assert (r.goal_index if r.success else -1) == st["goal_index"], i
+        s, p, c = b.tree(i)
+        assert np.array_equal(p, ref["parents"]), i
+        assert np.array_equal(s.view(np.uint32), ref["samples"].view(np.uint32)), i
+        if r.success:
+            assert np.array_equal(b.path(i).view(np.uint32), ref["path"].view(np.uint32)), i
+    assert np.all(fin > 0)
+    # replaying the same batch gives the same answers
+    results2, _ = b.plan(starts, goals, obs, offs, cnts, use_graph=use_graph)
+    assert [r.tree_size for r in results2] == [r.tree_size for r in results]
