@@ -115,7 +115,7 @@ typedef struct kgmt_config {
     int32_t  sparse_valid_below; /* region is sparse while valid count < this (10) */
     float    quad_mass, quad_ix, quad_iy, quad_iz;
     float    quad_nu, quad_mu, gravity;
-    uint32_t reserved0;
+    uint32_t query_id;           /* Philox counter word: queries sharing a seed     */
     uint64_t seed;               /* replaces the wall-clock seed of KGMT.cu:80-81  */
 } kgmt_config;
 
@@ -130,6 +130,17 @@ typedef struct kgmt_result {
     float    time_ms;     /* wall time of the plan call (host clock)                  */
     uint64_t expanded;    /* children propagated over all iterations                 */
 } kgmt_result;
+
+/* Per-kernel device time accumulated while profiling is on (HIP events
+ * around every launch of the plain, non-graph iteration loop). Kernel order:
+ * 0 propagate, 1 score, 2 append, 3 expand, 4 plan. */
+#define KGMT_PROFILE_KERNELS 5
+typedef struct kgmt_profile {
+    uint64_t launches[KGMT_PROFILE_KERNELS];
+    double   ms[KGMT_PROFILE_KERNELS];
+    uint64_t children;   /* children propagated by the profiled launches */
+    uint64_t iterations; /* profiled iterations */
+} kgmt_profile;
 
 typedef struct kgmt_ctx kgmt_ctx;
 typedef struct kgmt_batch kgmt_batch;
@@ -147,6 +158,8 @@ int kgmt_config_check(const kgmt_config* cfg);
 /* ---- single-query planner (KGMT::plan) ---------------------------------- */
 int kgmt_create(const kgmt_config* cfg, kgmt_ctx** out);
 int kgmt_destroy(kgmt_ctx* ctx);
+/* Seed of the next plans (replaces the wall-clock seed of KGMT.cu:80-81). */
+int kgmt_set_seed(kgmt_ctx* ctx, uint64_t seed);
 /* Use an existing hipStream_t for every launch of this context (NULL: own stream). */
 int kgmt_set_stream(kgmt_ctx* ctx, void* hip_stream);
 /* start, goal: host arrays of D floats (only state[0..S) of start and
@@ -167,6 +180,10 @@ int kgmt_get_path(kgmt_ctx* ctx, float* rows, uint32_t cap_rows, uint32_t* n_row
 int kgmt_get_regions(kgmt_ctx* ctx, int32_t* total, int32_t* valid, float* score, uint32_t cap,
                      uint32_t* n_regions);
 
+/* Turn per-kernel event timing on/off (it also resets the accumulators). */
+int kgmt_set_profiling(kgmt_ctx* ctx, int32_t on);
+int kgmt_get_profile(kgmt_ctx* ctx, kgmt_profile* out);
+
 /* ---- batched planner ----------------------------------------------------- */
 /* n_q queries, each with its own configuration (capacity, model, ...) and its
  * own obstacle set. obstacle_offsets[q] (in boxes) indexes `obstacles`;
@@ -185,6 +202,8 @@ int kgmt_batch_get_tree(kgmt_batch* b, uint32_t q, float* samples, int32_t* pare
 /* Launch shape of the persistent kernels (0 = autotuned/default). */
 int kgmt_batch_set_launch(kgmt_batch* b, uint32_t propagate_block, uint32_t propagate_grid,
                           uint32_t chunk_grid);
+int kgmt_batch_set_profiling(kgmt_batch* b, int32_t on);
+int kgmt_batch_get_profile(kgmt_batch* b, kgmt_profile* out);
 
 /* ---- stage-level entry points (device pointers, asynchronous on `stream`) -
  * These run one stage of an iteration on caller-owned device buffers so that

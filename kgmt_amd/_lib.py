@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-from .abi import KgmtConfig, KgmtResult
+from .abi import KgmtConfig, KgmtProfile, KgmtResult
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libkgmt.so")
 
@@ -38,11 +38,16 @@ SIGNATURES = {
     "kgmt_config_check": (C.c_int, [_cfgp]),
     "kgmt_create": (C.c_int, [_cfgp, C.POINTER(_P)]),
     "kgmt_destroy": (C.c_int, [_P]),
+    "kgmt_set_seed": (C.c_int, [_P, C.c_uint64]),
     "kgmt_set_stream": (C.c_int, [_P, _P]),
     "kgmt_plan": (C.c_int, [_P, _P, _P, _P, _u32, _i32, _resp]),
     "kgmt_get_tree": (C.c_int, [_P, _P, _P, _P, _u32, C.POINTER(_u32)]),
     "kgmt_get_path": (C.c_int, [_P, _P, _u32, C.POINTER(_u32)]),
     "kgmt_get_regions": (C.c_int, [_P, _P, _P, _P, _u32, C.POINTER(_u32)]),
+    "kgmt_set_profiling": (C.c_int, [_P, _i32]),
+    "kgmt_get_profile": (C.c_int, [_P, C.POINTER(KgmtProfile)]),
+    "kgmt_batch_set_profiling": (C.c_int, [_P, _i32]),
+    "kgmt_batch_get_profile": (C.c_int, [_P, C.POINTER(KgmtProfile)]),
     "kgmt_batch_create": (C.c_int, [_cfgp, _u32, C.POINTER(_P)]),
     "kgmt_batch_destroy": (C.c_int, [_P]),
     "kgmt_batch_set_stream": (C.c_int, [_P, _P]),

@@ -81,7 +81,7 @@ class KgmtConfig(C.Structure):
         ("quad_nu", C.c_float),
         ("quad_mu", C.c_float),
         ("gravity", C.c_float),
-        ("reserved0", C.c_uint32),
+        ("query_id", C.c_uint32),
         ("seed", C.c_uint64),
     ]
 
@@ -128,6 +128,27 @@ class KgmtResult(C.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+KGMT_PROFILE_KERNELS = 5
+PROFILE_KERNEL_NAMES = ("propagate", "score", "append", "expand", "plan")
+
+
+class KgmtProfile(C.Structure):
+    _fields_ = [
+        ("launches", C.c_uint64 * KGMT_PROFILE_KERNELS),
+        ("ms", C.c_double * KGMT_PROFILE_KERNELS),
+        ("children", C.c_uint64),
+        ("iterations", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {
+            "kernels": {n: {"launches": int(self.launches[i]), "ms": float(self.ms[i])}
+                        for i, n in enumerate(PROFILE_KERNEL_NAMES)},
+            "children": int(self.children),
+            "iterations": int(self.iterations),
+        }
 
 
 assert C.sizeof(KgmtConfig) == 160, C.sizeof(KgmtConfig)

@@ -65,6 +65,19 @@ def build_lib(verbose: bool = False) -> str:
     return LIB
 
 
+def build_dropin(verbose: bool = False) -> str:
+    """tests/cpp/dropin_main: a C++ caller of include/kgmt_planner.hpp linked
+    against libkgmt.so (host code only)."""
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_main.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "dropin_main")
+    cmd = [_hipcc(), "-O2", "-std=c++17", src, "-o", out, f"-L{HERE}", "-lkgmt", f"-Wl,-rpath,{HERE}",
+           "-Wl,-rpath,$ORIGIN/../../kgmt_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return out
+
+
 def build_oracle(verbose: bool = False) -> str:
     cmd = ["make", "-s", "-C", ORACLE_DIR]
     if verbose:
@@ -82,6 +95,7 @@ def main(argv=None) -> int:
     both = not (a.lib or a.oracle)
     if a.lib or both:
         print(build_lib(a.verbose))
+        print(build_dropin(a.verbose))
     if a.oracle or both:
         print(build_oracle(a.verbose))
     return 0

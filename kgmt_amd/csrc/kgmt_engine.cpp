@@ -175,9 +175,15 @@ struct Engine
     std::vector<double> iterMs;
     uint32_t itersRun = 0;
     int device        = 0;
+    // profiling: 6 timing events per iteration (boundaries of the 5 kernels)
+    bool profiling = false;
+    std::vector<hipEvent_t> profEv;
+    kgmt_profile prof{};
 
     ~Engine()
     {
+        for(auto& e : profEv)
+            if(e) (void)hipEventDestroy(e);
         if(graphExec) (void)hipGraphExecDestroy(graphExec);
         if(graph) (void)hipGraphDestroy(graph);
         for(auto& e : ev)
@@ -293,13 +299,51 @@ struct Engine
         return KGMT_OK;
     }
 
-    int enqueue_iteration()
+    int enqueue_iteration(hipEvent_t* marks = nullptr)
     {
+        if(marks) HIP_TRY(hipEventRecord(marks[0], stream));
         HIP_TRY(launch_propagate(model, dq, ds, P, W, nq, maxObs, L, stream));
+        if(marks) HIP_TRY(hipEventRecord(marks[1], stream));
         HIP_TRY(launch_score(dq, ds, P, nq, stream));
+        if(marks) HIP_TRY(hipEventRecord(marks[2], stream));
         HIP_TRY(launch_append(dq, ds, P, W, nq, L, stream));
+        if(marks) HIP_TRY(hipEventRecord(marks[3], stream));
         HIP_TRY(launch_expand(dq, ds, P, W, nq, L, stream));
+        if(marks) HIP_TRY(hipEventRecord(marks[4], stream));
         HIP_TRY(launch_plan(dq, ds, P, W, nq, stream));
+        if(marks) HIP_TRY(hipEventRecord(marks[5], stream));
+        return KGMT_OK;
+    }
+
+    int set_profiling(bool on)
+    {
+        profiling = on;
+        std::memset(&prof, 0, sizeof(prof));
+        const size_t need = (size_t)(maxIter + 1) * 6;
+        if(on && profEv.size() < need)
+            {
+                const size_t old = profEv.size();
+                profEv.resize(need, nullptr);
+                for(size_t i = old; i < need; ++i) HIP_TRY(hipEventCreate(&profEv[i]));
+            }
+        return KGMT_OK;
+    }
+
+    int collect_profile(uint32_t iters)
+    {
+        for(uint32_t it = 0; it < iters; ++it)
+            {
+                hipEvent_t* m = &profEv[(size_t)it * 6];
+                for(int k = 0; k < KGMT_PROFILE_KERNELS; ++k)
+                    {
+                        float ms = 0.0f;
+                        HIP_TRY(hipEventElapsedTime(&ms, m[k], m[k + 1]));
+                        prof.ms[k] += ms;
+                        prof.launches[k] += 1;
+                    }
+            }
+        prof.iterations += iters;
+        for(uint32_t q = 0; q < nq; ++q) prof.children += hs[q].expanded;
         return KGMT_OK;
     }
 
@@ -346,7 +390,7 @@ struct Engine
                     HIP_TRY(hipGraphLaunch(graphExec, stream));
                 else
                     {
-                        int rc = enqueue_iteration();
+                        int rc = enqueue_iteration(profiling ? &profEv[(size_t)it * 6] : nullptr);
                         if(rc) return rc;
                         HIP_TRY(hipMemcpyAsync(&pinned[slot], W.allDone, sizeof(uint32_t), hipMemcpyDeviceToHost,
                                                stream));
@@ -374,6 +418,7 @@ struct Engine
         HIP_TRY(hipStreamSynchronize(stream));
         itersRun = it;
         HIP_TRY(hipMemcpy(hs.data(), ds, sizeof(QueryState) * nq, hipMemcpyDeviceToHost));
+        if(profiling && !useGraph) return collect_profile(it);
         return KGMT_OK;
     }
 
@@ -547,6 +592,13 @@ int kgmt_destroy(kgmt_ctx* ctx)
     return KGMT_OK;
 }
 
+int kgmt_set_seed(kgmt_ctx* ctx, uint64_t seed)
+{
+    if(!ctx) return fail(KGMT_E_ARG, "null context");
+    ctx->eng.hq[0].cfg.seed = seed;
+    return KGMT_OK;
+}
+
 int kgmt_set_stream(kgmt_ctx* ctx, void* s)
 {
     if(!ctx) return fail(KGMT_E_ARG, "null context");
@@ -603,6 +655,32 @@ int kgmt_plan(kgmt_ctx* ctx, const float* start, const float* goal, const float*
     if(rc) return rc;
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     e.result(0, out, ms);
+    return KGMT_OK;
+}
+
+int kgmt_set_profiling(kgmt_ctx* ctx, int32_t on)
+{
+    if(!ctx) return fail(KGMT_E_ARG, "null context");
+    return ctx->eng.set_profiling(on != 0);
+}
+
+int kgmt_get_profile(kgmt_ctx* ctx, kgmt_profile* out)
+{
+    if(!ctx || !out) return fail(KGMT_E_ARG, "null argument");
+    *out = ctx->eng.prof;
+    return KGMT_OK;
+}
+
+int kgmt_batch_set_profiling(kgmt_batch* b, int32_t on)
+{
+    if(!b) return fail(KGMT_E_ARG, "null batch");
+    return b->eng.set_profiling(on != 0);
+}
+
+int kgmt_batch_get_profile(kgmt_batch* b, kgmt_profile* out)
+{
+    if(!b || !out) return fail(KGMT_E_ARG, "null argument");
+    *out = b->eng.prof;
     return KGMT_OK;
 }
 

@@ -237,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void k_init(const QueryDesc* __restrict__ q
 // propagate: persistent over 256-child tiles
 // ---------------------------------------------------------------------------
 template <int MODEL>
-__device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState& st, uint32_t q, uint32_t u,
+__device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState& st, uint32_t u,
                                            const Pools& P, const float4* sObs)
 {
     constexpr int S = ModelDims<MODEL>::S, C = ModelDims<MODEL>::C, D = S + C + 1;
@@ -258,7 +258,7 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
             float uc[C];
             int32_t steps;
             float uf;
-            draw_child<C>(qd.cfg, q, st.itr, u, uc, steps, uf);
+            draw_child<C>(qd.cfg, qd.cfg.query_id, st.itr, u, uc, steps, uf);
             ok       = propagate<MODEL>(qd, s, uc, steps, sObs, (int)qd.nObs);
             float* U = P.childS + qd.sampleOff;
 #pragma unroll
@@ -306,19 +306,19 @@ __global__ __launch_bounds__(kTile) void k_propagate(const QueryDesc* __restrict
                 }
             const uint32_t u = (t - W.tileStart[q]) * kTile + threadIdx.x;
             if constexpr(MODEL >= 0)
-                child_tile<MODEL>(qd, sts[q], q, u, P, sObs);
+                child_tile<MODEL>(qd, sts[q], u, P, sObs);
             else
                 {
                     switch(qd.cfg.model)
                         {
                             case KGMT_MODEL_DOUBLE_INTEGRATOR:
-                                child_tile<KGMT_MODEL_DOUBLE_INTEGRATOR>(qd, sts[q], q, u, P, sObs);
+                                child_tile<KGMT_MODEL_DOUBLE_INTEGRATOR>(qd, sts[q], u, P, sObs);
                                 break;
                             case KGMT_MODEL_DUBINS_AIRPLANE:
-                                child_tile<KGMT_MODEL_DUBINS_AIRPLANE>(qd, sts[q], q, u, P, sObs);
+                                child_tile<KGMT_MODEL_DUBINS_AIRPLANE>(qd, sts[q], u, P, sObs);
                                 break;
-                            case KGMT_MODEL_DUBINS_5D: child_tile<KGMT_MODEL_DUBINS_5D>(qd, sts[q], q, u, P, sObs); break;
-                            default: child_tile<KGMT_MODEL_QUADCOPTER>(qd, sts[q], q, u, P, sObs); break;
+                            case KGMT_MODEL_DUBINS_5D: child_tile<KGMT_MODEL_DUBINS_5D>(qd, sts[q], u, P, sObs); break;
+                            default: child_tile<KGMT_MODEL_QUADCOPTER>(qd, sts[q], u, P, sObs); break;
                         }
                 }
         }
@@ -398,7 +398,7 @@ __device__ void append_children(const QueryDesc& qd, QueryState& st, uint32_t c,
     if(threadIdx.x == 0) P.partC[qd.chunkOff + c] = tot;
 }
 
-__device__ void reactivate(const QueryDesc& qd, QueryState& st, uint32_t q, uint32_t o, const Pools& P,
+__device__ void reactivate(const QueryDesc& qd, QueryState& st, uint32_t o, const Pools& P,
                            uint32_t* sbuf)
 {
     const uint32_t tree = st.treeSize, itr = st.itr;
@@ -413,7 +413,7 @@ __device__ void reactivate(const QueryDesc& qd, QueryState& st, uint32_t q, uint
                     bool fl = false;
                     if(P.frontier[nodeO + i] == 0)
                         {
-                            const float Ur = draw_react(qd.cfg, q, itr, i);
+                            const float Ur = draw_react(qd.cfg, qd.cfg.query_id, itr, i);
                             if(Ur <= P.score[qd.regOff + P.r1node[nodeO + i]] + fA)
                                 {
                                     fl                    = true;
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_append(const QueryDesc* __restrict__
             if(local < nCC)
                 append_children(qd, st, local, P, sbuf);
             else
-                reactivate(qd, st, q, local - nCC, P, sbuf);
+                reactivate(qd, st, local - nCC, P, sbuf);
             if(last_arrival(&st.jobsDone, jobsQ, &sflag))
                 {
                     const uint32_t nOC    = (st.treeSize + kChunk - 1) / kChunk;

@@ -16,7 +16,7 @@ import os
 import numpy as np
 
 from . import _lib
-from .abi import KGMT_MAX_SAMPLE, KgmtConfig, KgmtResult, dims
+from .abi import KGMT_MAX_SAMPLE, KgmtConfig, KgmtProfile, KgmtResult, dims
 from .workloads import config as workload_config
 
 
@@ -91,6 +91,19 @@ class KGMT(Planner):
 
     def set_stream(self, hip_stream: int | None):
         _lib.check(_lib.lib().kgmt_set_stream(self._h, C.c_void_p(hip_stream or 0)))
+
+    def set_seed(self, seed: int):
+        """Seed of the following plans (the reference seeds from the clock)."""
+        self.cfg.seed = int(seed)
+        _lib.check(_lib.lib().kgmt_set_seed(self._h, int(seed)))
+
+    def set_profiling(self, on: bool = True):
+        _lib.check(_lib.lib().kgmt_set_profiling(self._h, 1 if on else 0))
+
+    def profile(self) -> dict:
+        p = KgmtProfile()
+        _lib.check(_lib.lib().kgmt_get_profile(self._h, C.byref(p)))
+        return p.as_dict()
 
     # KGMT::plan, src/planners/KGMT.cu:41-105
     def plan(self, h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount) -> KgmtResult:
@@ -172,6 +185,14 @@ class BatchKGMT:
             self.close()
         except Exception:
             pass
+
+    def set_profiling(self, on: bool = True):
+        _lib.check(_lib.lib().kgmt_batch_set_profiling(self._h, 1 if on else 0))
+
+    def profile(self) -> dict:
+        p = KgmtProfile()
+        _lib.check(_lib.lib().kgmt_batch_get_profile(self._h, C.byref(p)))
+        return p.as_dict()
 
     def set_launch(self, propagate_grid=0, chunk_grid=0):
         _lib.check(_lib.lib().kgmt_batch_set_launch(self._h, 0, int(propagate_grid), int(chunk_grid)))

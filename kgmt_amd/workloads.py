@@ -159,7 +159,7 @@ def make_batch(n_queries: int = 512, seed: int = 0, max_iter: int = 300):
             g = rng.uniform(0.1, 0.9, 3)
             if np.linalg.norm(s - g) >= 0.6:
                 break
-        cfg = config(model, capacity=cap, seed=seed * 100003 + q + 1, max_iter=max_iter)
+        cfg = config(model, capacity=cap, seed=seed * 100003 + q + 1, max_iter=max_iter, query_id=q)
         clear = [(s, 0.06), (g, cfg.goal_thresh + 0.01)]
         obs = random_boxes(rng, n_obs, 0.02, 0.06, clear)
         out.append(Query(cfg, _sample(cfg, s), _sample(cfg, g), obs, f"batch/{q}", {"model": model}))

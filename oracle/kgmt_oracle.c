@@ -699,14 +699,15 @@ static float dist3(const float* a, const float* b)
     return sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
-int orc_plan(const kgmt_config* cfg, uint32_t query, const float* start, const float* goal, const float* obs,
+int orc_plan(const kgmt_config* cfg, const float* start, const float* goal, const float* obs,
              uint32_t n_obs, float* samples, int32_t* parents, float* costs, float* path, int32_t* total_out,
              int32_t* valid_out, float* score_out, uint32_t* iter_log, orc_stats* st)
 {
     int S, C;
     if(dims(cfg->model, &S, &C)) return KGMT_E_UNSUPPORTED;
-    const int D      = S + C + 1;
-    const uint32_t N = cfg->capacity, B = cfg->children_per_entry;
+    const int D          = S + C + 1;
+    const uint32_t N     = cfg->capacity, B = cfg->children_per_entry;
+    const uint32_t query = cfg->query_id;
     int32_t R1, R2, P;
     orc_num_regions(cfg, &R1, &R2, &P);
 
@@ -916,7 +917,7 @@ int orc_plan_many(const kgmt_config* cfgs, uint32_t n_q, const float* starts, co
                 if(rc != KGMT_OK) continue;
                 struct timespec t0, t1;
                 clock_gettime(CLOCK_MONOTONIC, &t0);
-                int r = orc_plan(&cfgs[q], (uint32_t)q, starts + (size_t)q * KGMT_MAX_SAMPLE,
+                int r = orc_plan(&cfgs[q], starts + (size_t)q * KGMT_MAX_SAMPLE,
                                  goals + (size_t)q * KGMT_MAX_SAMPLE, obs + 6 * (size_t)obs_off[q], obs_cnt[q], samples,
                                  parents, costs, path, NULL, NULL, NULL, NULL, &st_out[q]);
                 if(r != KGMT_OK) rc = r;

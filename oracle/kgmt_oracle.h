@@ -61,15 +61,16 @@ typedef struct orc_stats {
     uint64_t expanded;
 } orc_stats;
 
-/* samples: capacity*D floats (row-major), parents/costs: capacity entries,
+/* Random draws use cfg->query_id as the query word of the Philox counter.
+ * samples: capacity*D floats (row-major), parents/costs: capacity entries,
  * path: (max_iter+1)*D floats. iter_log (optional, may be NULL): per
  * iteration 4 uint32 {R, n_children, next, n_react}. */
-int orc_plan(const kgmt_config* cfg, uint32_t query, const float* start, const float* goal,
+int orc_plan(const kgmt_config* cfg, const float* start, const float* goal,
              const float* obs, uint32_t n_obs, float* samples, int32_t* parents, float* costs,
              float* path, int32_t* total_out, int32_t* valid_out, float* score_out,
              uint32_t* iter_log, orc_stats* st);
 
-/* Plan n_q queries (configuration cfgs[q], query id q) on `threads` OpenMP
+/* Plan n_q queries (configuration cfgs[q]) on `threads` OpenMP
  * threads, one query per thread at a time; the CPU baseline. starts/goals:
  * n_q rows of KGMT_MAX_SAMPLE floats. Per-query wall ms in ms_out. */
 int orc_plan_many(const kgmt_config* cfgs, uint32_t n_q, const float* starts, const float* goals,

@@ -50,7 +50,7 @@ _SIG = {
     "orc_region_update": (None, [_cfgp, _P, _P, _P, _P, _u32, _P, _P, _P, _P, _P, _P]),
     "orc_score": (None, [_cfgp, _P, _P, _P, _P]),
     "orc_faccept": (C.c_float, [_cfgp, _u32, _u32, _u32]),
-    "orc_plan": (C.c_int, [_cfgp, _u32, _P, _P, _P, _u32, _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(OrcStats)]),
+    "orc_plan": (C.c_int, [_cfgp, _P, _P, _P, _u32, _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(OrcStats)]),
     "orc_plan_many": (C.c_int, [_cfgp, _u32, _P, _P, _P, _P, _P, C.c_int, C.POINTER(OrcStats), _P]),
     "orc_sincosf_n": (None, [_P, _u32, _P, _P]),
     "orc_philox_n": (None, [_P, _P, _u32, _P]),
@@ -214,7 +214,7 @@ def faccept(cfg, itr, tree, nxt) -> float:
     return lib().orc_faccept(C.byref(cfg), itr, tree, nxt)
 
 
-def plan(cfg: KgmtConfig, start, goal, obs, query: int = 0, log_iters: bool = False):
+def plan(cfg: KgmtConfig, start, goal, obs, log_iters: bool = False):
     """Full planner on the CPU. Returns dict with stats, tree, path, regions."""
     S, Cd, D = dims(cfg.model)
     N = cfg.capacity
@@ -233,7 +233,7 @@ def plan(cfg: KgmtConfig, start, goal, obs, query: int = 0, log_iters: bool = Fa
     g = np.zeros(KGMT_MAX_SAMPLE, np.float32)
     g[:D] = np.asarray(goal, np.float32)[:D]
     ob = _c(obs, np.float32).reshape(-1)
-    rc = lib().orc_plan(C.byref(cfg), query, _p(s), _p(g), _p(ob) if ob.size else None, ob.size // 6, _p(samples),
+    rc = lib().orc_plan(C.byref(cfg), _p(s), _p(g), _p(ob) if ob.size else None, ob.size // 6, _p(samples),
                         _p(parents), _p(costs), _p(path), _p(total), _p(valid), _p(score_),
                         _p(log) if log is not None else None, C.byref(st))
     assert rc == 0, rc

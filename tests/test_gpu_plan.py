@@ -88,7 +88,7 @@ def test_batch_matches_oracle(use_graph, orc):
     b = BatchKGMT(cfgs)
     results, fin = b.plan(starts, goals, obs, offs, cnts, use_graph=use_graph)
     for i, q in enumerate(qs):
-        ref = orc.plan(q.cfg, q.start, q.goal, q.obstacles, query=i)
+        ref = orc.plan(q.cfg, q.start, q.goal, q.obstacles)
         st = ref["stats"]
         r = results[i]
         assert r.iterations == st["iterations"], i

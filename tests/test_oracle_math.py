@@ -1,0 +1,414 @@
+"""Pins of the CPU oracle's building blocks (no GPU).
+
+Each check is independent of the oracle's own formula: published known-answer
+vectors, libm in double precision, closed-form integrals, brute force on small
+lattices, an independent matrix formulation of the quadcopter dynamics
+integrated by scipy, and exhaustive index maps.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from kgmt_amd import workloads as W
+from kgmt_amd.abi import KGMT_DI_EXACT, KGMT_MODEL_DUBINS_AIRPLANE, dims
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+# --- Philox4x32-10 ------------------------------------------------------------
+def test_philox_known_answers(orc):
+    kat = json.load(open(os.path.join(HERE, "golden", "philox_kat.json")))["vectors"]
+    for ctr, key, want in kat:
+        got = orc.philox(ctr, key)[0]
+        assert [int(x) for x in got] == want
+
+
+def test_u01_range_and_exactness(orc):
+    assert orc.u01(0) == 2.0 ** -24
+    assert orc.u01(0xFFFFFFFF) == 1.0
+    assert orc.u01(0x00000100) == 2.0 * 2.0 ** -24
+    xs = np.random.default_rng(0).integers(0, 2 ** 32, 2000, dtype=np.uint64)
+    for x in xs[:200]:
+        v = orc.u01(int(x))
+        assert v == ((int(x) >> 8) + 1) / 2.0 ** 24
+
+
+# --- sin / cos ----------------------------------------------------------------
+def test_sincos_accuracy_against_libm(orc):
+    x = np.concatenate([np.linspace(-50, 50, 400001), np.linspace(-4000, 4000, 200001),
+                        np.array([0.0, -0.0, np.pi / 2, np.pi, -np.pi, 1e-30, 5e-8])]).astype(np.float32)
+    s, c = orc.sincos(x)
+    xd = x.astype(np.float64)
+    es, ec = np.sin(xd), np.cos(xd)
+    # error relative to the largest representable magnitude near 1 (absolute
+    # error bound) plus 2 ulps of the value itself
+    ulp = lambda v: np.spacing(np.abs(v).astype(np.float32)).astype(np.float64)
+    assert np.all(np.abs(s - es) <= 2 * ulp(es) + 6e-8)
+    assert np.all(np.abs(c - ec) <= 2 * ulp(ec) + 6e-8)
+    # near zero sin is relative-accurate
+    small = np.abs(xd) < 0.5
+    assert np.all(np.abs(s[small] - es[small]) <= 2 * ulp(es[small]) + 1e-45)
+    assert np.max(np.abs(s.astype(np.float64) ** 2 + c.astype(np.float64) ** 2 - 1)) < 3e-7
+
+
+# --- per-child draws ----------------------------------------------------------
+@pytest.mark.parametrize("name", ["di6d", "dubins5d", "quad12d"])
+def test_draw_ranges_and_uniformity(name, orc):
+    cfg = W.config(name)
+    S, Cd, D = dims(cfg.model)
+    ctrl, steps, uf = orc.draw_children(cfg, 0, 1, 0, 200000)
+    for i in range(Cd):
+        assert ctrl[i].min() >= cfg.ctrl_min[i] and ctrl[i].max() <= cfg.ctrl_max[i]
+        # mean of a uniform draw
+        assert abs(ctrl[i].mean() - (cfg.ctrl_min[i] + cfg.ctrl_max[i]) / 2) < 0.01 * (cfg.ctrl_max[i] - cfg.ctrl_min[i])
+    k = cfg.steps_max - cfg.steps_min + 1
+    h = np.bincount(steps - cfg.steps_min, minlength=k)
+    assert h.sum() == len(steps) and h.min() > 0 and len(h) == k
+    chi2 = ((h - len(steps) / k) ** 2 / (len(steps) / k)).sum()
+    assert chi2 < k + 6 * math.sqrt(2 * k)
+    assert uf.min() > 0 and uf.max() <= 1
+    # distinct iterations / queries / indices give distinct streams
+    c2, _, _ = orc.draw_children(cfg, 0, 2, 0, 1000)
+    c3, _, _ = orc.draw_children(cfg, 1, 1, 0, 1000)
+    assert not np.array_equal(ctrl[:, :1000], c2) and not np.array_equal(ctrl[:, :1000], c3)
+
+
+# --- AABB segment test ------------------------------------------------------
+def _lattice_hit(p0, p1, box):
+    """Brute force: does the closed AABB of the segment meet the open box?
+    All coordinates are multiples of 0.25, so testing the lattice points and
+    midpoints of the segment's box (step 0.125) decides it exactly."""
+    lo = np.minimum(p0, p1)
+    hi = np.maximum(p0, p1)
+    axes = []
+    for d in range(3):
+        pts = np.arange(lo[d], hi[d] + 1e-9, 0.125) if hi[d] > lo[d] else np.array([lo[d]])
+        axes.append(pts)
+    for x in axes[0]:
+        for y in axes[1]:
+            for z in axes[2]:
+                q = np.array([x, y, z])
+                if np.all(q > box[:3]) and np.all(q < box[3:]):
+                    return True
+    return False
+
+
+def test_segment_test_brute_force_lattice(orc):
+    rng = np.random.default_rng(1)
+    grid = np.arange(0, 1.01, 0.25)
+    boxes = []
+    for _ in range(60):
+        a = rng.choice(grid, 3)
+        b = rng.choice(grid, 3)
+        lo, hi = np.minimum(a, b), np.maximum(a, b)
+        hi = np.where(hi == lo, np.minimum(lo + 0.25, 1.0), hi)
+        boxes.append(np.concatenate([lo, hi]))
+    for box in boxes:
+        p0 = rng.choice(grid, (80, 3))
+        p1 = rng.choice(grid, (80, 3))
+        got = orc.segment_valid(p0, p1, box[None])
+        want = np.array([not _lattice_hit(a, b, box) for a, b in zip(p0, p1)], np.uint8)
+        assert np.array_equal(got, want)
+
+
+def test_segment_test_properties(orc):
+    rng = np.random.default_rng(2)
+    obs = W.random_boxes(rng, 30, 0.05, 0.3)
+    p0 = rng.uniform(0, 1, (5000, 3)).astype(np.float32)
+    p1 = (p0 + rng.uniform(-0.1, 0.1, (5000, 3))).astype(np.float32)
+    v = orc.segment_valid(p0, p1, obs)
+    assert np.array_equal(v, orc.segment_valid(p1, p0, obs))          # endpoint order
+    assert np.all(orc.segment_valid(p0, p1, obs[:0]) == 1)             # no boxes: free
+    # a box set is hit iff one of its boxes is hit
+    each = np.stack([orc.segment_valid(p0, p1, obs[i:i + 1]) for i in range(len(obs))])
+    assert np.array_equal(v, each.min(axis=0))
+    # touching a face is not a collision (strict inequalities)
+    box = np.array([[0.2, 0.2, 0.2, 0.4, 0.4, 0.4]], np.float32)
+    assert orc.segment_valid([0.4, 0.3, 0.3], [0.5, 0.3, 0.3], box)[0] == 1
+    assert orc.segment_valid([0.39, 0.3, 0.3], [0.5, 0.3, 0.3], box)[0] == 0
+    assert orc.segment_valid([0.1, 0.1, 0.1], [0.2, 0.2, 0.2], box)[0] == 1
+
+
+# --- regions ------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["di6d", "dubins5d", "quad12d", "reference_quad"])
+def test_region_maps_exhaustive(name, orc):
+    cfg = W.config(name)
+    L, L2 = cfg.r1_len, cfg.r2_len
+    lo, hi = cfg.ws_min, cfg.ws_max
+    size = (hi - lo) / L
+    idx = np.stack(np.meshgrid(np.arange(L), np.arange(L), np.arange(L), indexing="ij"), -1).reshape(-1, 3)
+    centres = (lo + (idx + 0.5) * size).astype(np.float32)
+    r1, r2 = orc.regions(cfg, centres)
+    assert np.array_equal(r1, (idx[:, 0] * L + idx[:, 1]) * L + idx[:, 2])        # z fastest
+    # subregion centres
+    sub = np.stack(np.meshgrid(np.arange(L2), np.arange(L2), np.arange(L2), indexing="ij"), -1).reshape(-1, 3)
+    for cell in idx[:: max(1, len(idx) // 50)]:
+        pts = (lo + cell * size + (sub + 0.5) * size / L2).astype(np.float32)
+        a, b = orc.regions(cfg, pts)
+        r = (cell[0] * L + cell[1]) * L + cell[2]
+        assert np.all(a == r)
+        assert np.array_equal(b, r * L2 ** 3 + (sub[:, 0] * L2 + sub[:, 1]) * L2 + sub[:, 2])
+    # outside the workspace clamps to the border regions; NaN maps to index 0
+    far = np.array([[lo - 5, hi + 5, (lo + hi) / 2], [np.nan, lo, lo], [np.inf, -np.inf, lo]], np.float32)
+    a, _ = orc.regions(cfg, far)
+    assert a[0] == (0 * L + (L - 1)) * L + L // 2 and a[1] == 0 and a[2] == (L - 1) * L * L
+
+
+def test_region_boundaries_are_floors(orc):
+    cfg = W.config("di6d")
+    L = cfg.r1_len
+    for k in range(1, L):
+        b = np.float32(k / L)
+        below = np.nextafter(b, np.float32(0))
+        above = np.nextafter(b, np.float32(1))
+        r_b, _ = orc.regions(cfg, np.array([[0.01, 0.01, below]], np.float32))
+        r_a, _ = orc.regions(cfg, np.array([[0.01, 0.01, above]], np.float32))
+        # index = floor(L * z) computed in float: the cell changes within one ulp of the boundary
+        assert r_a[0] - r_b[0] == 1
+        assert r_b[0] == int(np.floor(np.float32(L) * below))
+
+
+# --- dynamics: closed forms and independent integrators ----------------------
+def _no_bounds(cfg, **kw):
+    return cfg.replace(ws_min=-1e6, ws_max=1e6, v_min=-1e6, v_max=1e6, pitch_min=-1e6, pitch_max=1e6, **kw)
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_double_integrator_closed_form(exact, orc):
+    cfg = _no_bounds(W.config("di6d"), integrator=KGMT_DI_EXACT if exact else 0)
+    rng = np.random.default_rng(4)
+    n = 400
+    x0 = np.concatenate([rng.uniform(0, 1, (n, 3)), rng.uniform(-0.5, 0.5, (n, 3))], 1)
+    a = rng.uniform(-1, 1, (n, 3))
+    steps = rng.integers(1, 30, n).astype(np.int32)
+    h = float(np.float32(cfg.dt))
+    k = steps[:, None].astype(np.float64)
+    if exact:
+        want = x0[:, :3] + x0[:, 3:] * k * h + a * (k * h) ** 2 / 2
+    else:  # statePropagator.cu:97-99 literally: x += (5 v + 2 a h) h / 6 per step
+        want = x0[:, :3] + 5.0 / 6.0 * h * (k * x0[:, 3:] + a * h * k * (k - 1) / 2) + k / 3.0 * a * h * h
+    vwant = x0[:, 3:] + a * k * h
+    x1, valid = orc.propagate_f64(cfg, x0, a, steps, np.zeros((0, 6)))
+    assert np.all(valid == 1)
+    assert np.allclose(x1[:, :3], want, rtol=0, atol=1e-12)
+    assert np.allclose(x1[:, 3:6], vwant, rtol=0, atol=1e-12)
+    # FP32 restatement within 1e-5 relative of the FP64 twin
+    X = x0.T.astype(np.float32)
+    child, v32, _, _ = orc.propagate_children(cfg, np.arange(n, dtype=np.int32), X, a.T.astype(np.float32), steps,
+                                              np.zeros((0, 6), np.float32))
+    x1_32 = np.asarray(orc.propagate_f64(cfg, X.T.astype(np.float64), a.astype(np.float32).astype(np.float64), steps,
+                                         np.zeros((0, 6)))[0])
+    err = np.abs(child[:6].T - x1_32[:, :6]) / np.maximum(np.abs(x1_32[:, :6]), 1.0)
+    assert err.max() < 1e-5
+    assert np.allclose(child[9], h * steps, rtol=1e-6)
+
+
+def _dubins_exact(x0, yr, pr, v, T):
+    """Closed form of x' = v cos(p) cos(y), y' = v cos(p) sin(y), z' = v sin(p)
+    with p = p0 + pr t, y = y0 + yr t (product-to-sum integrals)."""
+    y0, p0 = x0[3], x0[4]
+
+    def I_cos(w, phase):
+        return T * math.cos(phase) if abs(w) < 1e-14 else (math.sin(phase + w * T) - math.sin(phase)) / w
+
+    def I_sin(w, phase):
+        return T * math.sin(phase) if abs(w) < 1e-14 else (math.cos(phase) - math.cos(phase + w * T)) / w
+
+    x = x0[0] + v * 0.5 * (I_cos(pr - yr, p0 - y0) + I_cos(pr + yr, p0 + y0))
+    y = x0[1] + v * 0.5 * (I_sin(pr + yr, p0 + y0) - I_sin(pr - yr, p0 - y0))
+    z = x0[2] + v * I_sin(pr, p0)
+    return np.array([x, y, z, y0 + yr * T, p0 + pr * T])
+
+
+@pytest.mark.parametrize("model", ["dubins5d", "dubins6d"])
+def test_dubins_against_closed_form(model, orc):
+    base = W.config("dubins5d")
+    if model == "dubins6d":
+        base = base.replace(model=KGMT_MODEL_DUBINS_AIRPLANE)
+    cfg = _no_bounds(base)
+    rng = np.random.default_rng(5)
+    n = 300
+    S, Cd, D = dims(cfg.model)
+    x0 = np.zeros((n, S))
+    x0[:, :3] = rng.uniform(0, 1, (n, 3))
+    x0[:, 3] = rng.uniform(-3, 3, n)
+    x0[:, 4] = rng.uniform(-0.8, 0.8, n)
+    u = np.zeros((n, Cd))
+    u[:, 0] = rng.uniform(-1, 1, n)
+    u[:, 1] = rng.uniform(-0.5, 0.5, n)
+    u[::7, :2] = 0.0                                  # straight lines
+    if model == "dubins5d":
+        u[:, 2] = rng.uniform(0.2, 0.4, n)
+    else:
+        x0[:, 5] = rng.uniform(0.2, 0.4, n)
+        u[:, 2] = 0.0                                 # constant speed: closed form applies
+    steps = rng.integers(1, 26, n).astype(np.int32)
+    h = float(np.float32(cfg.dt))
+    x1, valid = orc.propagate_f64(cfg, x0, u, steps, np.zeros((0, 6)))
+    assert np.all(valid == 1)
+    for i in range(n):
+        v = u[i, 2] if model == "dubins5d" else x0[i, 5]
+        want = _dubins_exact(x0[i], u[i, 0], u[i, 1], v, h * steps[i])
+        # RK4 global error O(h^4): tiny at h = 0.02
+        assert np.allclose(x1[i, :5], want, rtol=0, atol=1e-9), i
+    # FP32 oracle within 1e-5 of the FP64 twin
+    X = x0.T.astype(np.float32)
+    child, _, _, _ = orc.propagate_children(cfg, np.arange(n, dtype=np.int32), X, u.T.astype(np.float32), steps,
+                                            np.zeros((0, 6), np.float32))
+    ref, _ = orc.propagate_f64(cfg, X.T.astype(np.float64), u.astype(np.float32).astype(np.float64), steps,
+                               np.zeros((0, 6)))
+    assert np.max(np.abs(child[:5].T - ref[:, :5]) / np.maximum(np.abs(ref[:, :5]), 1.0)) < 1e-5
+
+
+def _quad_ode_matrix(x, u, cfg):
+    """Rigid-body quadrotor in matrix form, written independently of the
+    reference's expanded scalar expressions: world position rate R v_b with
+    R = Rz(psi) Ry(theta) Rx(phi); body acceleration -w x v + R^T g + drag +
+    thrust; Euler-angle rates from the body rates; Euler's equations."""
+    phi, th, psi = x[3:6]
+    vb = x[6:9]
+    w = x[9:12]
+    cx, sx, cy, sy, cz, sz = math.cos(phi), math.sin(phi), math.cos(th), math.sin(th), math.cos(psi), math.sin(psi)
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    R = Rz @ Ry @ Rx
+    E = np.array([[1, sx * math.tan(th), cx * math.tan(th)], [0, cx, -sx], [0, sx / cy, cx / cy]])
+    m, g = cfg.quad_mass, cfg.gravity
+    I = np.array([cfg.quad_ix, cfg.quad_iy, cfg.quad_iz])
+    acc = -np.cross(w, vb) + R.T @ np.array([0, 0, g]) - cfg.quad_nu * np.linalg.norm(vb) * vb / m
+    acc[2] += u[0] / m
+    tau = np.array(u[1:4])
+    wdot = (-np.cross(w, I * w) - cfg.quad_mu * np.linalg.norm(w) * w + tau) / I
+    return np.concatenate([R @ vb, E @ w, acc, wdot])
+
+
+def test_quad_ode_matches_matrix_form(orc):
+    cfg = W.config("quad12d")
+    rng = np.random.default_rng(6)
+    for _ in range(200):
+        x = np.concatenate([rng.uniform(0, 1, 3), rng.uniform(-1.2, 1.2, 3), rng.uniform(-3, 3, 3), rng.uniform(-5, 5, 3)])
+        u = np.array([rng.uniform(0, 20), *rng.uniform(-0.05, 0.05, 3)])
+        got = orc.quad_ode_f64(cfg, x, u)
+        want = _quad_ode_matrix(x, u, cfg)
+        assert np.allclose(got, want, rtol=1e-11, atol=1e-11)
+
+
+def test_quad_invariants(orc):
+    cfg = W.config("quad12d")
+    hover = np.zeros(12)
+    hover[:3] = 0.5
+    d = orc.quad_ode_f64(cfg, hover, np.array([-cfg.gravity * cfg.quad_mass, 0, 0, 0]))
+    assert np.allclose(d, 0, atol=1e-15)
+    d = orc.quad_ode_f64(cfg, hover, np.zeros(4))                     # free fall
+    assert abs(d[8] - cfg.gravity) < 1e-6 and np.allclose(np.delete(d, 8), 0, atol=1e-15)
+    spin = hover.copy()
+    spin[11] = 2.0                                                      # pure yaw spin
+    d = orc.quad_ode_f64(cfg, spin, np.array([-cfg.gravity, 0, 0, 0]))
+    assert abs(d[5] - 2.0) < 1e-15 and abs(d[11] + cfg.quad_mu * 2.0 * 2.0 / cfg.quad_iz) < 1e-12
+
+
+def test_quad_rk4_against_dop853(orc):
+    scipy_integrate = pytest.importorskip("scipy.integrate")
+    cfg = _no_bounds(W.config("quad12d"))
+    rng = np.random.default_rng(7)
+    for _ in range(12):
+        x0 = np.concatenate([rng.uniform(0.3, 0.7, 3), rng.uniform(-0.5, 0.5, 3), rng.uniform(-1, 1, 3),
+                             rng.uniform(-2, 2, 3)])
+        u = np.array([rng.uniform(0, 19.62), *rng.uniform(-0.05, 0.05, 3)])
+        steps = int(rng.integers(10, 51))
+        h = float(np.float32(cfg.dt))
+        sol = scipy_integrate.solve_ivp(lambda t, x: _quad_ode_matrix(x, u, cfg), (0, h * steps), x0,
+                                        method="DOP853", rtol=1e-12, atol=1e-12)
+        x1, valid = orc.propagate_f64(cfg, x0[None], u[None], np.array([steps], np.int32), np.zeros((0, 6)))
+        assert valid[0] == 1
+        err_h = np.max(np.abs(x1[0, :12] - sol.y[:, -1]))
+        assert err_h < 5e-6
+        # fourth order: half the step, 2x the steps -> error / 16
+        half = cfg.replace(dt=h / 2)
+        x2, _ = orc.propagate_f64(half, x0[None], u[None], np.array([2 * steps], np.int32), np.zeros((0, 6)))
+        err_h2 = np.max(np.abs(x2[0, :12] - sol.y[:, -1]))
+        if err_h > 1e-9:
+            assert 10.0 < err_h / err_h2 < 24.0, (err_h, err_h2)
+        # FP32 restatement within 1e-5 relative (per component, floor 1) of FP64
+        c32, _, _, _ = orc.propagate_children(cfg, np.zeros(1, np.int32), x0[:, None].astype(np.float32),
+                                              u[:, None].astype(np.float32), np.array([steps], np.int32),
+                                              np.zeros((0, 6), np.float32))
+        r64, _ = orc.propagate_f64(cfg, x0.astype(np.float32).astype(np.float64)[None],
+                                   u.astype(np.float32).astype(np.float64)[None], np.array([steps], np.int32),
+                                   np.zeros((0, 6)))
+        assert np.max(np.abs(c32[:12, 0] - r64[0, :12]) / np.maximum(np.abs(r64[0, :12]), 1.0)) < 1e-5
+
+
+def test_fp32_states_match_fp64_with_obstacles(orc):
+    """Validity verdicts of FP32 and FP64 agree except where the FP64
+    trajectory passes within float resolution of a box face or bound."""
+    for name in ("di6d", "dubins5d", "quad12d"):
+        q = W.make_query(name, 2)
+        cfg = q.cfg
+        S, Cd, D = dims(cfg.model)
+        n = 3000
+        rng = np.random.default_rng(8)
+        X = np.zeros((S, n), np.float32)
+        X[:3] = rng.uniform(0.1, 0.9, (3, n))
+        ctrl, steps, _ = orc.draw_children(cfg, 0, 1, 0, n)
+        c32, v32, _, _ = orc.propagate_children(cfg, np.arange(n, dtype=np.int32), X, ctrl, steps, q.obstacles)
+        c64, v64 = orc.propagate_f64(cfg, X.T.astype(np.float64), ctrl.T.astype(np.float64), steps,
+                                     q.obstacles.astype(np.float64))
+        agree = (v32 == v64)
+        assert agree.mean() > 0.995, name
+        both = agree & (v32 == 1)
+        rel = np.abs(c32[:S, both].T - c64[both, :S]) / np.maximum(np.abs(c64[both, :S]), 1.0)
+        assert rel.max() < 1e-5, name
+
+
+# --- scores -------------------------------------------------------------------
+def test_score_closed_form(orc):
+    for name in ("di6d", "dubins5d", "quad12d"):
+        cfg = W.config(name)
+        R1, P = cfg.r1_len ** 3, cfg.r2_len ** 3
+        rng = np.random.default_rng(9)
+        total = rng.integers(0, 100000, R1)
+        total[rng.random(R1) < 0.4] = 0
+        valid = (total * rng.random(R1)).astype(np.int64)
+        act = (rng.random(R1 * P) < 0.3).astype(np.uint8)
+        got = orc.score(cfg, total, valid, act)
+        eps = cfg.epsilon
+        vol = ((cfg.ws_max - cfg.ws_min) / cfg.r1_len) ** 3
+        cov = act.reshape(R1, P).sum(1) / P
+        fv = (eps + valid) / (eps + total) * vol
+        s = np.where(valid > 0, fv ** 4 / ((1 + cov) * (1 + total.astype(np.float64) ** 2)), 0.0)
+        want = np.where(valid > 0, eps + s / s.sum(), 1.0)
+        assert np.allclose(got, want, rtol=2e-6, atol=0)
+
+
+def test_faccept_closed_form(orc):
+    cfg = W.config("di6d")
+    for itr, tree, nxt in [(1, 1, 100), (10, 5000, 300), (300, 100000, 31072), (7, 0, 0)]:
+        want = itr * cfg.epsilon * (1 - (tree + nxt) / cfg.capacity) ** 5
+        assert abs(orc.faccept(cfg, itr, tree, nxt) - want) <= 1e-6 * max(1.0, abs(want))
+
+
+def test_region_update_rules(orc):
+    cfg = W.config("di6d")
+    R1, R2 = cfg.r1_len ** 3, cfg.r1_len ** 3 * cfg.r2_len ** 3
+    r1 = np.array([3, 3, 3, 5, 7], np.int32)
+    r2 = np.array([3 * 64, 3 * 64, 3 * 64 + 1, 5 * 64, 7 * 64], np.int32)
+    valid = np.array([1, 1, 1, 0, 1], np.uint8)
+    uf = np.array([0.9, 0.9, 0.9, 0.0, 0.1], np.float32)
+    score = np.zeros(R1, np.float32)
+    score[7] = 0.5
+    ap = np.zeros(R2, np.uint8)
+    ap[3 * 64 + 1] = 1
+    ap[7 * 64] = 1
+    an = ap.copy()
+    total = np.zeros(R1, np.int32)
+    vc = np.zeros(R1, np.int32)
+    fn = orc.region_update(cfg, r1, r2, valid, uf, score, ap, an, total, vc)
+    # snapshot semantics: both children in the fresh subregion see it inactive
+    assert list(fn) == [1, 1, 0, 0, 1]
+    assert total[3] == 3 and vc[3] == 3 and total[5] == 1 and vc[5] == 0 and total[7] == 1
+    assert an[3 * 64] == 1 and an[5 * 64] == 0
