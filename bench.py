@@ -142,7 +142,21 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=512, help="mixed batch size (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-queries", type=int, default=0, help="CPU sample size (default: one per thread)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of 256 MiB three times (scales the FETCH_SIZE/WRITE_SIZE counters)")
     a = ap.parse_args(argv)
+
+    if a.copy_calibration:
+        import torch
+        n = 64 * 1024 * 1024  # floats: 256 MiB
+        x = torch.ones(n, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": n * 4, "launches": 3}), flush=True)
+        return 0
 
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1",

@@ -133,8 +133,8 @@ typedef struct kgmt_result {
 
 /* Per-kernel device time accumulated while profiling is on (HIP events
  * around every launch of the plain, non-graph iteration loop). Kernel order:
- * 0 propagate, 1 score, 2 append, 3 expand, 4 plan. */
-#define KGMT_PROFILE_KERNELS 5
+ * 0 propagate, 1 score, 2 append, 3 expand (which also closes the iteration). */
+#define KGMT_PROFILE_KERNELS 4
 typedef struct kgmt_profile {
     uint64_t launches[KGMT_PROFILE_KERNELS];
     double   ms[KGMT_PROFILE_KERNELS];
@@ -160,6 +160,9 @@ int kgmt_create(const kgmt_config* cfg, kgmt_ctx** out);
 int kgmt_destroy(kgmt_ctx* ctx);
 /* Seed of the next plans (replaces the wall-clock seed of KGMT.cu:80-81). */
 int kgmt_set_seed(kgmt_ctx* ctx, uint64_t seed);
+/* Replay iterations from a hipGraph captured at the first plan (on != 0) or
+ * launch them one by one (0, the default; needed for kgmt_set_profiling). */
+int kgmt_set_graph(kgmt_ctx* ctx, int32_t on);
 /* Use an existing hipStream_t for every launch of this context (NULL: own stream). */
 int kgmt_set_stream(kgmt_ctx* ctx, void* hip_stream);
 /* start, goal: host arrays of D floats (only state[0..S) of start and

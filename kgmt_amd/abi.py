@@ -130,8 +130,8 @@ class KgmtResult(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
-KGMT_PROFILE_KERNELS = 5
-PROFILE_KERNEL_NAMES = ("propagate", "score", "append", "expand", "plan")
+KGMT_PROFILE_KERNELS = 4
+PROFILE_KERNEL_NAMES = ("propagate", "score", "append", "expand")
 
 
 class KgmtProfile(C.Structure):

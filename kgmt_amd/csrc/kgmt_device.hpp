@@ -23,7 +23,11 @@ namespace kgmt
 constexpr uint32_t kChunk   = 1024;  // elements per bookkeeping chunk (one workgroup job)
 constexpr uint32_t kTile    = 256;   // children per propagate tile
 constexpr uint32_t kBlock   = 256;   // threads of chunk / tile workgroups
-constexpr uint32_t kMaxObs  = 2048;  // obstacles a query may carry (LDS: 32 B each)
+constexpr uint32_t kMaxObs  = 1500;  // obstacles a query may carry
+constexpr uint32_t kGroup   = 8;     // boxes per bounding group of a scene
+constexpr uint32_t kGroupBatch = 4;  // groups tested together (scenes pad to a multiple)
+// LDS of one scene: G groups + 8 G boxes, G = ceil(n / 32) * 4, as (lo, hi)
+// float4 pairs: at most 1692 pairs = 54.1 KB (+ 8.5 KB of tile state)
 constexpr int32_t kNoGoal   = 0x7fffffff;
 
 // -------------------------------------------------------------------------
@@ -40,13 +44,13 @@ struct QueryDesc
     uint32_t chunkOff;               // offset of per-chunk arrays (nChunks + 1 entries)
     uint32_t regOff;                 // offset of per-region arrays (R1 entries)
     uint32_t subOff;                 // offset of subregion bitset words (R2words entries)
-    uint32_t obsOff, nObs;           // first box and box count in the obstacle pool
+    uint32_t obsOff, nObs, nGrp;     // first pair, boxes, groups (multiple of 4) of the scene in the pool
     uint32_t pathOff;                // float offset of the path rows ((max_iter+2)*D)
     float r1size, vol;               // W_R1_SIZE, W_R1_VOL
     float goal[3];
     float start[KGMT_MAX_SAMPLE];
     float qminv, qax, qbx, qay, qby, qaz, qbz;  // quad: 1/m, (Iy-Iz)/Ix, 1/Ix, ...
-    uint32_t pad[3];
+    uint32_t pad[2];
 };
 
 struct QueryState
@@ -94,17 +98,24 @@ struct Pools
     float* score;        // per region score
     uint32_t* actPrev;   // subregion bitset as of iteration start
     uint32_t* actNext;   // subregion bitset being built
-    float4* obs;         // obstacle pool, 2 float4 per box (lo, hi)
+    float4* obs;         // scenes: per query nGrp group bounds then nObs boxes, (lo, hi) float4 pairs
     float* path;         // goal->root rows
 };
 
-// Work lists over queries (prefix sums, nq + 1 entries each).
+// Work lists over queries, double-buffered by iteration parity: the kernels
+// of iteration i read buffer (i-1)&1 while the last finisher of that
+// iteration writes the other one. Prefix sums of nq + 1 entries each.
 struct WorkLists
 {
-    uint32_t* tileStart;   // propagate tiles
-    uint32_t* jobStart;    // append / expand chunk jobs
-    uint32_t* counter;     // last-block counter of the plan kernel
-    uint32_t* allDone;     // 1 once every query is done
+    uint32_t* tileStart;   // 2 x (nq + 1): propagate tiles
+    uint32_t* jobStart;    // 2 x (nq + 1): append / expand chunk jobs
+    uint32_t* active;      // 2: queries not done in each list
+    uint32_t* counter;     // arrivals of query finishers (last-block pattern)
+    uint32_t* allDone;     // host-mapped word: 1 once every query is done
+    uint32_t nq;
+
+    __device__ __forceinline__ const uint32_t* tiles(uint32_t parity) const { return tileStart + parity * (nq + 1); }
+    __device__ __forceinline__ const uint32_t* jobs(uint32_t parity) const { return jobStart + parity * (nq + 1); }
 };
 
 // -------------------------------------------------------------------------
@@ -204,22 +215,53 @@ __device__ __forceinline__ void sincos_k(float x, float& s, float& c)
 // -------------------------------------------------------------------------
 // Collision: segment AABB vs boxes staged in LDS as (lo, hi) float4 pairs.
 // A box is hit unless some axis separates (bbMax <= lo or hi <= bbMin),
-// exactly the reference's test (NaN coordinates count as a hit, like it).
+// exactly the reference's test (a NaN face never separates, like there).
+// The scene holds the boxes in groups of kGroup with a bounding box per group
+// (componentwise min of lo / max of hi, NaN if a member has NaN); a group
+// that no lane of the wave can touch is skipped as a whole. Groups are padded
+// to whole batches with dummy boxes lo = +inf, hi = -inf, which separate from
+// every segment that has one non-NaN axis (a segment with NaN on all axes
+// already hits every real box of that group). Skipping and padding are exact:
+// a box hit implies a hit of its group's bounds, and the verdict is an OR
+// over boxes, independent of their order.
 // -------------------------------------------------------------------------
+struct Scene
+{
+    const float4* grp;  // nGrp (lo, hi) pairs, then kGroup * nGrp boxes
+    int nGrp;
+};
+
+__device__ __forceinline__ bool separated(float mnx, float mny, float mnz, float mxx, float mxy, float mxz, float4 lo,
+                                          float4 hi)
+{
+    return (mxx <= lo.x) | (hi.x <= mnx) | (mxy <= lo.y) | (hi.y <= mny) | (mxz <= lo.z) | (hi.z <= mnz);
+}
+
 __device__ __forceinline__ bool segment_free(float ax, float ay, float az, float bx, float by, float bz,
-                                             const float4* __restrict__ sObs, int nObs)
+                                             const Scene& sc)
 {
     const float mnx = ax > bx ? bx : ax, mxx = ax > bx ? ax : bx;
     const float mny = ay > by ? by : ay, mxy = ay > by ? ay : by;
     const float mnz = az > bz ? bz : az, mxz = az > bz ? az : bz;
-    bool hit = false;
-    for(int o = 0; o < nObs; ++o)
+    const float4* box = sc.grp + 2 * sc.nGrp;
+    bool hit          = false;
+    for(int g0 = 0; g0 < sc.nGrp; g0 += (int)kGroupBatch)
         {
-            const float4 lo = sObs[2 * o];
-            const float4 hi = sObs[2 * o + 1];
-            const bool sep  = (mxx <= lo.x) | (hi.x <= mnx) | (mxy <= lo.y) | (hi.y <= mny) | (mxz <= lo.z) |
-                             (hi.z <= mnz);
-            hit |= !sep;
+            bool near[kGroupBatch];
+#pragma unroll
+            for(int k = 0; k < (int)kGroupBatch; ++k)
+                near[k] = !separated(mnx, mny, mnz, mxx, mxy, mxz, sc.grp[2 * (g0 + k)], sc.grp[2 * (g0 + k) + 1]);
+#pragma unroll
+            for(int k = 0; k < (int)kGroupBatch; ++k)
+                {
+                    if(__any(near[k] && !hit))
+                        {
+                            const float4* b = box + 2 * (int)kGroup * (g0 + k);
+#pragma unroll 4
+                            for(int m = 0; m < (int)kGroup; ++m)
+                                hit |= !separated(mnx, mny, mnz, mxx, mxy, mxz, b[2 * m], b[2 * m + 1]);
+                        }
+                }
         }
     return !hit;
 }
@@ -258,7 +300,7 @@ struct ModelDims<KGMT_MODEL_DUBINS_5D>
 
 // statePropagator.cu:76-142
 __device__ __forceinline__ bool prop_di(const QueryDesc& qd, float* s, const float* u, int steps,
-                                        const float4* sObs, int nObs)
+                                        const Scene& sc)
 {
     const kgmt_config& c = qd.cfg;
     const float h        = c.dt;
@@ -297,7 +339,7 @@ __device__ __forceinline__ bool prop_di(const QueryDesc& qd, float* s, const flo
                     valid = false;
                     break;
                 }
-            if(!segment_free(px, py, pz, x, y, z, sObs, nObs))
+            if(!segment_free(px, py, pz, x, y, z, sc))
                 {
                     valid = false;
                     break;
@@ -310,7 +352,7 @@ __device__ __forceinline__ bool prop_di(const QueryDesc& qd, float* s, const flo
 // statePropagator.cu:147-241; FIVE_D: speed is control u[2], no acceleration.
 template <bool FIVE_D>
 __device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const float* u, int steps,
-                                            const float4* sObs, int nObs)
+                                            const Scene& sc)
 {
     const kgmt_config& c = qd.cfg;
     const float h        = c.dt;
@@ -319,15 +361,18 @@ __device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const
     float x = s[0], y = s[1], z = s[2], yaw = s[3], pitch = s[4];
     float v    = FIVE_D ? u[2] : s[5];
     bool valid = true;
+    // sin/cos of the step-start angles; the step-end angles pitch + h*pr and
+    // yaw + h*yr are the next step's start angles (same expression, same bits)
+    float sp0, cp0, sy0, cy0;
+    sincos_k(pitch, sp0, cp0);
+    sincos_k(yaw, sy0, cy0);
     for(int i = 0; i < steps; ++i)
         {
             const float px = x, py = y, pz = z;
             const float hh = 0.5f * h;
             const float vm = v + hh * a, pm = pitch + hh * pr, ym = yaw + hh * yr;
             const float ve = v + h * a, pe = pitch + h * pr, ye = yaw + h * yr;
-            float sp0, cp0, sy0, cy0, spm, cpm, sym, cym, spe, cpe, sye, cye;
-            sincos_k(pitch, sp0, cp0);
-            sincos_k(yaw, sy0, cy0);
+            float spm, cpm, sym, cym, spe, cpe, sye, cye;
             sincos_k(pm, spm, cpm);
             sincos_k(ym, sym, cym);
             sincos_k(pe, spe, cpe);
@@ -341,6 +386,7 @@ __device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const
             yaw            = yaw + h * yr;
             pitch          = pitch + h * pr;
             v              = v + (h / 6.0f) * (a + 2.0f * (a + a) + a);
+            sp0 = spe, cp0 = cpe, sy0 = sye, cy0 = cye;
             if(!FIVE_D && (v < c.v_min || v > c.v_max))
                 {
                     valid = false;
@@ -356,7 +402,7 @@ __device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const
                     valid = false;
                     break;
                 }
-            if(!segment_free(px, py, pz, x, y, z, sObs, nObs))
+            if(!segment_free(px, py, pz, x, y, z, sc))
                 {
                     valid = false;
                     break;
@@ -397,7 +443,7 @@ __device__ __forceinline__ void quad_ode(const QueryDesc& qd, const float* st, c
 
 // statePropagator.cu:246-323
 __device__ __forceinline__ bool prop_quad(const QueryDesc& qd, float* s, const float* uc, int steps,
-                                          const float4* sObs, int nObs)
+                                          const Scene& sc)
 {
     const kgmt_config& c = qd.cfg;
     const float h = c.dt, hh = 0.5f * h, h6 = h / 6.0f;
@@ -429,7 +475,7 @@ __device__ __forceinline__ bool prop_quad(const QueryDesc& qd, float* s, const f
                     valid = false;
                     break;
                 }
-            if(!segment_free(px, py, pz, s[0], s[1], s[2], sObs, nObs))
+            if(!segment_free(px, py, pz, s[0], s[1], s[2], sc))
                 {
                     valid = false;
                     break;
@@ -440,16 +486,16 @@ __device__ __forceinline__ bool prop_quad(const QueryDesc& qd, float* s, const f
 
 template <int MODEL>
 __device__ __forceinline__ bool propagate(const QueryDesc& qd, float* s, const float* u, int steps,
-                                          const float4* sObs, int nObs)
+                                          const Scene& sc)
 {
     if constexpr(MODEL == KGMT_MODEL_DOUBLE_INTEGRATOR)
-        return prop_di(qd, s, u, steps, sObs, nObs);
+        return prop_di(qd, s, u, steps, sc);
     else if constexpr(MODEL == KGMT_MODEL_DUBINS_AIRPLANE)
-        return prop_dubins<false>(qd, s, u, steps, sObs, nObs);
+        return prop_dubins<false>(qd, s, u, steps, sc);
     else if constexpr(MODEL == KGMT_MODEL_DUBINS_5D)
-        return prop_dubins<true>(qd, s, u, steps, sObs, nObs);
+        return prop_dubins<true>(qd, s, u, steps, sc);
     else
-        return prop_quad(qd, s, u, steps, sObs, nObs);
+        return prop_quad(qd, s, u, steps, sc);
 }
 
 // -------------------------------------------------------------------------

@@ -108,24 +108,23 @@ void fill_desc(QueryDesc& qd, const kgmt_config& c)
     qd.qaz = (c.quad_ix - c.quad_iy) / c.quad_iz, qd.qbz = 1.0f / c.quad_iz;
 }
 
-// obstacle rows (xmin ymin zmin xmax ymax zmax) -> (lo, hi) float4 pairs
-void pack_obstacles(const float* rows, uint32_t n, float4* out)
+// groups of a scene of n boxes: whole batches of kGroupBatch groups
+uint32_t scene_groups(uint32_t n)
 {
-    for(uint32_t i = 0; i < n; ++i)
-        {
-            const float* r = rows + 6 * (size_t)i;
-            out[2 * i]     = make_float4(r[0], r[1], r[2], 0.0f);
-            out[2 * i + 1] = make_float4(r[3], r[4], r[5], 0.0f);
-        }
+    const uint32_t per = kGroup * kGroupBatch;
+    return (n + per - 1) / per * kGroupBatch;
 }
 
-__global__ void k_pack_obstacles(const float* __restrict__ rows, uint32_t n, float4* __restrict__ out)
+uint32_t scene_pairs(uint32_t n)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n) return;
-    const float* r = rows + 6 * (size_t)i;
-    out[2 * i]     = make_float4(r[0], r[1], r[2], 0.0f);
-    out[2 * i + 1] = make_float4(r[3], r[4], r[5], 0.0f);
+    return scene_groups(n) * (1 + kGroup);
+}
+
+uint32_t morton10(uint32_t x, uint32_t y, uint32_t z)
+{
+    uint32_t code = 0;
+    for(int b = 9; b >= 0; --b) code = (code << 3) | (((x >> b) & 1u) << 2) | (((y >> b) & 1u) << 1) | ((z >> b) & 1u);
+    return code;
 }
 
 template <typename T>
@@ -141,9 +140,77 @@ int dev_alloc(T*& p, size_t count, std::vector<void*>& owned)
 }
 
 constexpr int kRing = 8;  // completion polls in flight
-constexpr int kLag  = 2;  // iterations the host stays behind the GPU
+constexpr int kLag  = 2;  // launch units the host stays behind the GPU
 
 }  // namespace
+
+uint32_t kgmt::build_scene(const float* rows, uint32_t n, float4* out)
+{
+    if(n == 0) return 0;
+    const uint32_t nGrp = scene_groups(n);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for(uint32_t i = 0; i < n; ++i)
+        for(int d = 0; d < 3; ++d)
+            {
+                const float c = 0.5f * (rows[6 * (size_t)i + d] + rows[6 * (size_t)i + 3 + d]);
+                if(std::isfinite(c)) lo[d] = std::min(lo[d], c), hi[d] = std::max(hi[d], c);
+            }
+    std::vector<std::pair<uint32_t, uint32_t>> key(n);
+    for(uint32_t i = 0; i < n; ++i)
+        {
+            uint32_t q[3];
+            for(int d = 0; d < 3; ++d)
+                {
+                    const float c = 0.5f * (rows[6 * (size_t)i + d] + rows[6 * (size_t)i + 3 + d]);
+                    float t       = hi[d] > lo[d] ? (c - lo[d]) / (hi[d] - lo[d]) : 0.0f;
+                    if(!(t >= 0.0f)) t = 0.0f;
+                    if(t > 1.0f) t = 1.0f;
+                    q[d] = (uint32_t)(t * 1023.0f);
+                }
+            key[i] = {morton10(q[0], q[1], q[2]), i};
+        }
+    std::sort(key.begin(), key.end());
+    // boxes in Morton order, then never-hit dummies up to nGrp * kGroup
+    float4* box = out + 2 * nGrp;
+    for(uint32_t k = 0; k < nGrp * kGroup; ++k)
+        {
+            if(k < n)
+                {
+                    const float* r = rows + 6 * (size_t)key[k].second;
+                    box[2 * k]     = make_float4(r[0], r[1], r[2], 0.0f);
+                    box[2 * k + 1] = make_float4(r[3], r[4], r[5], 0.0f);
+                }
+            else
+                {
+                    box[2 * k]     = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+                    box[2 * k + 1] = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+                }
+        }
+    for(uint32_t g = 0; g < nGrp; ++g)
+        {
+            float gl[3] = {INFINITY, INFINITY, INFINITY}, gh[3] = {-INFINITY, -INFINITY, -INFINITY};
+            bool nanAxis[3] = {false, false, false};
+            const uint32_t b1 = std::min(n, (g + 1) * kGroup);
+            for(uint32_t k = g * kGroup; k < b1; ++k)
+                {
+                    const float l[3] = {box[2 * k].x, box[2 * k].y, box[2 * k].z};
+                    const float h[3] = {box[2 * k + 1].x, box[2 * k + 1].y, box[2 * k + 1].z};
+                    for(int d = 0; d < 3; ++d)
+                        {
+                            if(std::isnan(l[d]) || std::isnan(h[d])) nanAxis[d] = true;
+                            gl[d] = std::min(gl[d], l[d]);
+                            gh[d] = std::max(gh[d], h[d]);
+                        }
+                }
+            // a NaN member makes its group unskippable on that axis; a group of
+            // dummies only keeps lo = +inf, hi = -inf and is never near
+            for(int d = 0; d < 3; ++d)
+                if(nanAxis[d]) gl[d] = gh[d] = NAN;
+            out[2 * g]     = make_float4(gl[0], gl[1], gl[2], 0.0f);
+            out[2 * g + 1] = make_float4(gh[0], gh[1], gh[2], 0.0f);
+        }
+    return nGrp;
+}
 
 // ---------------------------------------------------------------------------
 // Engine: device pools and the iteration loop for a table of queries
@@ -158,14 +225,14 @@ struct Engine
     QueryState* ds = nullptr;
     Pools P{};
     WorkLists W{};
-    uint32_t obsCap = 0;     // boxes the obstacle pool holds
+    uint32_t obsCap = 0;     // (lo, hi) pairs the obstacle pool holds
     float4* obsPool = nullptr;
-    uint32_t maxObs = 0;     // largest per-query box count of the current plan
+    uint32_t maxPairs = 0;   // largest per-query scene (pairs) of the current plan
     int model       = -1;    // common model or -1
     uint32_t maxIter = 0;
     hipStream_t stream = nullptr;
     bool ownStream     = false;
-    uint32_t* pinned   = nullptr;
+    uint32_t* pinned   = nullptr;  // host-mapped completion word (written by the kernels)
     hipEvent_t ev[kRing] = {};
     Launch L;
     hipGraph_t graph         = nullptr;
@@ -239,15 +306,17 @@ struct Engine
            (rc = dev_alloc(P.partC, chunks, owned)) || (rc = dev_alloc(P.total, regs, owned)) ||
            (rc = dev_alloc(P.validc, regs, owned)) || (rc = dev_alloc(P.score, regs, owned)) ||
            (rc = dev_alloc(P.actPrev, subs, owned)) || (rc = dev_alloc(P.actNext, subs, owned)) ||
-           (rc = dev_alloc(P.path, paths, owned)) || (rc = dev_alloc(W.tileStart, n + 1, owned)) ||
-           (rc = dev_alloc(W.jobStart, n + 1, owned)) || (rc = dev_alloc(W.counter, 1, owned)) ||
-           (rc = dev_alloc(W.allDone, 1, owned)))
+           (rc = dev_alloc(P.path, paths, owned)) || (rc = dev_alloc(W.tileStart, 2 * (size_t)(n + 1), owned)) ||
+           (rc = dev_alloc(W.jobStart, 2 * (size_t)(n + 1), owned)) || (rc = dev_alloc(W.active, 2, owned)) ||
+           (rc = dev_alloc(W.counter, 1, owned)))
             return rc;
+        W.nq = n;
         HIP_TRY(hipMemset(W.counter, 0, sizeof(uint32_t)));
-        HIP_TRY(hipMemset(W.allDone, 0, sizeof(uint32_t)));
         HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         ownStream = true;
-        HIP_TRY(hipHostMalloc((void**)&pinned, kRing * sizeof(uint32_t), hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc((void**)&pinned, sizeof(uint32_t), hipHostMallocMapped));
+        *pinned = 0;
+        HIP_TRY(hipHostGetDevicePointer((void**)&W.allDone, pinned, 0));
         for(auto& e : ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         // launch shape: persistent workgroups sized to the device
         hipDeviceProp_t prop;
@@ -279,10 +348,10 @@ struct Engine
         graphExec = nullptr, graph = nullptr, graphObs = 0xffffffffu;
     }
 
-    int ensure_obstacles(uint32_t boxes)
+    int ensure_obstacles(uint32_t pairs)
     {
-        if(boxes <= obsCap && obsPool) return KGMT_OK;
-        const uint32_t cap = std::max<uint32_t>(boxes, 64);
+        if(pairs <= obsCap && obsPool) return KGMT_OK;
+        const uint32_t cap = std::max<uint32_t>(pairs, 64);
         float4* p          = nullptr;
         if(hipMalloc((void**)&p, (size_t)cap * 2 * sizeof(float4)) != hipSuccess)
             return fail(KGMT_E_NOMEM, "obstacle pool allocation failed");
@@ -299,19 +368,18 @@ struct Engine
         return KGMT_OK;
     }
 
-    int enqueue_iteration(hipEvent_t* marks = nullptr)
+    // One iteration: 4 launches. marks: 5 timing events around them (profiling).
+    int enqueue_iteration(uint32_t parity, hipEvent_t* marks = nullptr)
     {
         if(marks) HIP_TRY(hipEventRecord(marks[0], stream));
-        HIP_TRY(launch_propagate(model, dq, ds, P, W, nq, maxObs, L, stream));
+        HIP_TRY(launch_propagate(model, dq, ds, P, W, parity, maxPairs, L, stream));
         if(marks) HIP_TRY(hipEventRecord(marks[1], stream));
         HIP_TRY(launch_score(dq, ds, P, nq, stream));
         if(marks) HIP_TRY(hipEventRecord(marks[2], stream));
-        HIP_TRY(launch_append(dq, ds, P, W, nq, L, stream));
+        HIP_TRY(launch_append(dq, ds, P, W, parity, L, stream));
         if(marks) HIP_TRY(hipEventRecord(marks[3], stream));
-        HIP_TRY(launch_expand(dq, ds, P, W, nq, L, stream));
+        HIP_TRY(launch_expand(dq, ds, P, W, parity, L, stream));
         if(marks) HIP_TRY(hipEventRecord(marks[4], stream));
-        HIP_TRY(launch_plan(dq, ds, P, W, nq, stream));
-        if(marks) HIP_TRY(hipEventRecord(marks[5], stream));
         return KGMT_OK;
     }
 
@@ -319,7 +387,7 @@ struct Engine
     {
         profiling = on;
         std::memset(&prof, 0, sizeof(prof));
-        const size_t need = (size_t)(maxIter + 1) * 6;
+        const size_t need = (size_t)(maxIter + 1) * (KGMT_PROFILE_KERNELS + 1);
         if(on && profEv.size() < need)
             {
                 const size_t old = profEv.size();
@@ -333,7 +401,7 @@ struct Engine
     {
         for(uint32_t it = 0; it < iters; ++it)
             {
-                hipEvent_t* m = &profEv[(size_t)it * 6];
+                hipEvent_t* m = &profEv[(size_t)it * (KGMT_PROFILE_KERNELS + 1)];
                 for(int k = 0; k < KGMT_PROFILE_KERNELS; ++k)
                     {
                         float ms = 0.0f;
@@ -347,68 +415,70 @@ struct Engine
         return KGMT_OK;
     }
 
+    // graph of two iterations (parities 0 and 1): no memset or copy nodes
     int build_graph()
     {
         drop_graph();
         HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_iteration();
-        hipError_t e = hipMemcpyAsync(pinned, W.allDone, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
-        hipGraph_t g = nullptr;
+        int rc = enqueue_iteration(0);
+        if(!rc) rc = enqueue_iteration(1);
+        hipGraph_t g  = nullptr;
         hipError_t e2 = hipStreamEndCapture(stream, &g);
         if(rc) return rc;
-        if(e != hipSuccess) return fail(KGMT_E_HIP, std::string("capture memcpy: ") + hipGetErrorString(e));
         if(e2 != hipSuccess) return fail(KGMT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
         graph = g;
         HIP_TRY(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0));
-        graphObs    = maxObs;
+        graphObs    = maxPairs;
         graphLaunch = L;
         return KGMT_OK;
     }
 
     // Upload descriptors, run init + iterations until every query is done.
+    // The host enqueues launch units (one iteration, or the two-iteration
+    // graph) and only waits for units kLag behind the newest one, so the GPU
+    // always has queued work; kernels of finished queries exit at once.
     int run(bool useGraph)
     {
         const auto t0 = std::chrono::steady_clock::now();
         HIP_TRY(hipMemcpyAsync(dq, hq.data(), sizeof(QueryDesc) * nq, hipMemcpyHostToDevice, stream));
         HIP_TRY(hipMemsetAsync(W.counter, 0, sizeof(uint32_t), stream));
-        HIP_TRY(hipMemsetAsync(W.allDone, 0, sizeof(uint32_t), stream));
-        HIP_TRY(launch_init(dq, ds, P, W, nq, stream));
-        if(useGraph && (!graphExec || graphObs != maxObs || graphLaunch.propGrid != L.propGrid ||
+        __atomic_store_n(pinned, 0u, __ATOMIC_RELEASE);
+        HIP_TRY(launch_init(dq, ds, P, W, stream));
+        if(useGraph && (!graphExec || graphObs != maxPairs || graphLaunch.propGrid != L.propGrid ||
                         graphLaunch.chunkGrid != L.chunkGrid))
             {
                 int rc = build_graph();
                 if(rc) return rc;
             }
-        iterMs.assign(maxIter + 1, 0.0);
-        for(int k = 0; k < kRing; ++k) pinned[k] = 0;
-        uint32_t it = 0, observed = 0;
+        const bool prof        = profiling && !useGraph;
+        const uint32_t perUnit = useGraph ? 2 : 1;
+        iterMs.assign(maxIter + 2, 0.0);
+        uint32_t it = 0, units = 0, observed = 0;
         bool finished = false;
         while(it < maxIter && !finished)
             {
-                const int slot = (int)(it % kRing);
+                const int slot = (int)(units % kRing);
                 if(useGraph)
                     HIP_TRY(hipGraphLaunch(graphExec, stream));
                 else
                     {
-                        int rc = enqueue_iteration(profiling ? &profEv[(size_t)it * 6] : nullptr);
+                        int rc = enqueue_iteration(it & 1u,
+                                                   prof ? &profEv[(size_t)it * (KGMT_PROFILE_KERNELS + 1)] : nullptr);
                         if(rc) return rc;
-                        HIP_TRY(hipMemcpyAsync(&pinned[slot], W.allDone, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                               stream));
                     }
                 HIP_TRY(hipEventRecord(ev[slot], stream));
-                ++it;
-                // observe iterations that are kLag behind
-                while(observed + kLag < it || (it == maxIter && observed < it))
+                it += perUnit;
+                ++units;
+                while(observed + kLag < units || (it >= maxIter && observed < units))
                     {
-                        const int os = (int)(observed % kRing);
                         hipError_t q;
-                        while((q = hipEventQuery(ev[os])) == hipErrorNotReady) {}
+                        while((q = hipEventQuery(ev[observed % kRing])) == hipErrorNotReady) {}
                         if(q != hipSuccess) return fail(KGMT_E_HIP, std::string("iteration: ") + hipGetErrorString(q));
-                        iterMs[observed] =
+                        const double t =
                           std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-                        const uint32_t v = useGraph ? pinned[0] : pinned[os];
+                        for(uint32_t k = 0; k < perUnit; ++k) iterMs[std::min<size_t>(observed * perUnit + k, maxIter + 1)] = t;
                         ++observed;
-                        if(v)
+                        if(__atomic_load_n(pinned, __ATOMIC_ACQUIRE))
                             {
                                 finished = true;
                                 break;
@@ -416,9 +486,9 @@ struct Engine
                     }
             }
         HIP_TRY(hipStreamSynchronize(stream));
-        itersRun = it;
+        itersRun = std::min(it, maxIter);
         HIP_TRY(hipMemcpy(hs.data(), ds, sizeof(QueryState) * nq, hipMemcpyDeviceToHost));
-        if(profiling && !useGraph) return collect_profile(it);
+        if(prof) return collect_profile(it);
         return KGMT_OK;
     }
 
@@ -473,6 +543,7 @@ struct Engine
 struct kgmt_ctx
 {
     Engine eng;
+    bool useGraph = false;
 };
 
 struct kgmt_batch
@@ -599,40 +670,64 @@ int kgmt_set_seed(kgmt_ctx* ctx, uint64_t seed)
     return KGMT_OK;
 }
 
+int kgmt_set_graph(kgmt_ctx* ctx, int32_t on)
+{
+    if(!ctx) return fail(KGMT_E_ARG, "null context");
+    ctx->useGraph = on != 0;
+    return KGMT_OK;
+}
+
 int kgmt_set_stream(kgmt_ctx* ctx, void* s)
 {
     if(!ctx) return fail(KGMT_E_ARG, "null context");
     return ctx->eng.set_stream(s);
 }
 
+// Obstacle rows of every query -> scenes (grouped, Morton-sorted) in the pool.
+// rows may be device memory (the reference passes a device pointer); they are
+// read back once per plan (6 floats per box) to build the groups on the host.
 static int upload_obstacles(Engine& e, const float* obstacles, const uint32_t* offsets, const uint32_t* counts,
                             uint32_t total, int onDevice)
 {
-    int rc = e.ensure_obstacles(total);
-    if(rc) return rc;
-    e.maxObs = 0;
+    if(total && !obstacles) return fail(KGMT_E_ARG, "obstacles pointer is null");
+    std::vector<float> hostRows;
+    const float* rows = obstacles;
+    if(onDevice && total)
+        {
+            hostRows.resize((size_t)total * 6);
+            HIP_TRY(hipMemcpyAsync(hostRows.data(), obstacles, hostRows.size() * sizeof(float), hipMemcpyDeviceToHost,
+                                   e.stream));
+            HIP_TRY(hipStreamSynchronize(e.stream));
+            rows = hostRows.data();
+        }
+    uint32_t pairs = 0;
     for(uint32_t q = 0; q < e.nq; ++q)
         {
-            const uint32_t c = counts ? counts[q] : total;
-            if(c > kMaxObs) return fail(KGMT_E_CAPACITY, "more than 2048 obstacles in one query");
-            e.hq[q].obsOff = offsets ? offsets[q] : 0;
-            e.hq[q].nObs   = c;
-            if((uint64_t)e.hq[q].obsOff + c > total) return fail(KGMT_E_ARG, "obstacle range out of bounds");
-            e.maxObs = std::max(e.maxObs, c);
+            const uint32_t c   = counts ? counts[q] : total;
+            const uint32_t off = offsets ? offsets[q] : 0;
+            if(c > kMaxObs) return fail(KGMT_E_CAPACITY, "more than 1500 obstacles in one query");
+            if((uint64_t)off + c > total) return fail(KGMT_E_ARG, "obstacle range out of bounds");
+            pairs += scene_pairs(c);
         }
-    if(total == 0) return KGMT_OK;
-    if(!obstacles) return fail(KGMT_E_ARG, "obstacles pointer is null");
-    if(onDevice)
+    int rc = e.ensure_obstacles(pairs);
+    if(rc) return rc;
+    std::vector<float4> packed((size_t)std::max<uint32_t>(pairs, 1) * 2);
+    uint32_t p = 0;
+    e.maxPairs = 0;
+    for(uint32_t q = 0; q < e.nq; ++q)
         {
-            hipLaunchKernelGGL(k_pack_obstacles, dim3((total + 255) / 256), dim3(256), 0, e.stream, obstacles, total,
-                               e.obsPool);
-            HIP_TRY(hipGetLastError());
+            const uint32_t c   = counts ? counts[q] : total;
+            const uint32_t off = offsets ? offsets[q] : 0;
+            QueryDesc& qd      = e.hq[q];
+            qd.obsOff          = p;
+            qd.nObs            = c;
+            qd.nGrp            = build_scene(rows + 6 * (size_t)off, c, packed.data() + 2 * (size_t)p);
+            p += scene_pairs(c);
+            e.maxPairs = std::max(e.maxPairs, scene_pairs(c));
         }
-    else
+    if(pairs)
         {
-            std::vector<float4> packed((size_t)total * 2);
-            pack_obstacles(obstacles, total, packed.data());
-            HIP_TRY(hipMemcpyAsync(e.obsPool, packed.data(), packed.size() * sizeof(float4), hipMemcpyHostToDevice,
+            HIP_TRY(hipMemcpyAsync(e.obsPool, packed.data(), (size_t)pairs * 2 * sizeof(float4), hipMemcpyHostToDevice,
                                    e.stream));
             HIP_TRY(hipStreamSynchronize(e.stream));
         }
@@ -651,7 +746,7 @@ int kgmt_plan(kgmt_ctx* ctx, const float* start, const float* goal, const float*
     qd.goal[0] = goal[0], qd.goal[1] = goal[1], qd.goal[2] = goal[2];
     int rc = upload_obstacles(e, obstacles, nullptr, nullptr, n_obs, on_device);
     if(rc) return rc;
-    rc = e.run(false);
+    rc = e.run(ctx->useGraph && !e.profiling);
     if(rc) return rc;
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     e.result(0, out, ms);
@@ -844,18 +939,27 @@ int kgmt_propagate_collide(const kgmt_config* cfg, const int32_t* d_parent_idx, 
     if(n_children && (!d_parent_idx || !d_parent_states || !d_controls || !d_steps || !d_child_samples || !d_valid ||
                       !d_r1 || !d_r2))
         return fail(KGMT_E_ARG, "null buffer");
-    if(n_obs > kMaxObs) return fail(KGMT_E_CAPACITY, "more than 2048 obstacles");
+    if(n_obs > kMaxObs) return fail(KGMT_E_CAPACITY, "more than 1500 obstacles");
     if(n_obs && !d_obstacles) return fail(KGMT_E_ARG, "obstacles pointer is null");
     StageDesc sd((hipStream_t)stream);
     int rc = sd.make(cfg);
     if(rc) return rc;
-    float4* packed = nullptr;
-    HIP_TRY(hipMallocAsync((void**)&packed, sizeof(float4) * 2 * std::max<uint32_t>(n_obs, 1), sd.s));
-    if(n_obs) hipLaunchKernelGGL(k_pack_obstacles, dim3((n_obs + 255) / 256), dim3(256), 0, sd.s, d_obstacles, n_obs, packed);
+    std::vector<float> rows((size_t)n_obs * 6);
+    if(n_obs)
+        {
+            HIP_TRY(hipMemcpyAsync(rows.data(), d_obstacles, rows.size() * sizeof(float), hipMemcpyDeviceToHost, sd.s));
+            HIP_TRY(hipStreamSynchronize(sd.s));
+        }
+    std::vector<float4> scene((size_t)std::max<uint32_t>(scene_pairs(n_obs), 1) * 2);
+    const uint32_t nGrp = build_scene(rows.data(), n_obs, scene.data());
+    float4* d_scene     = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&d_scene, scene.size() * sizeof(float4), sd.s));
+    HIP_TRY(hipMemcpyAsync(d_scene, scene.data(), scene.size() * sizeof(float4), hipMemcpyHostToDevice, sd.s));
     hipError_t e = launch_stage_propagate(cfg->model, sd.d, d_parent_idx, d_parent_states, n_parents, d_controls,
-                                          d_steps, n_children, packed, n_obs, d_child_samples, d_valid, d_r1, d_r2,
-                                          sd.s);
-    (void)hipFreeAsync(packed, sd.s);
+                                          d_steps, n_children, d_scene, nGrp, n_obs, d_child_samples, d_valid, d_r1,
+                                          d_r2, sd.s);
+    (void)hipFreeAsync(d_scene, sd.s);
+    if(e == hipSuccess) e = hipStreamSynchronize(sd.s);  // the host scene buffer must outlive the copy
     if(e != hipSuccess) return fail(KGMT_E_HIP, std::string("propagate: ") + hipGetErrorString(e));
     return KGMT_OK;
 }

@@ -15,11 +15,12 @@
 //                of old nodes. The last job of a query scans its partials.
 //                (KGMT.cu:305-378)
 //   k_expand     same jobs: writes the next expansion list (segmented repeat).
-//                (KGMT.cu:170-181, Planner.cu:52-80)
-//   k_plan       one workgroup per query: tree size, goal path, next launch
-//                regime (KGMT.cu:183-201); the last workgroup rebuilds the
-//                work lists of all queries.
-// All sizes stay on the device: the host only polls a completion word.
+//                (KGMT.cu:170-181, Planner.cu:52-80). The last job of each
+//                query closes its iteration: tree size, goal path, next
+//                launch regime (KGMT.cu:84-95, 183-201, 342-361, 399-402); the
+//                last query to close rebuilds the work lists of all queries.
+// All sizes stay on the device: the host only polls a completion word that
+// the kernels write into host memory.
 #include "kgmt_launch.hpp"
 
 namespace kgmt
@@ -28,10 +29,11 @@ namespace kgmt
 // ---------------------------------------------------------------------------
 // shared helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void stage_obstacles(const float4* __restrict__ pool, uint32_t off, uint32_t n,
-                                                float4* sObs)
+// copy `pairs` (lo, hi) float4 pairs of a scene into LDS
+__device__ __forceinline__ void stage_scene(const float4* __restrict__ pool, uint32_t off, uint32_t pairs,
+                                            float4* sObs)
 {
-    for(uint32_t i = threadIdx.x; i < 2 * n; i += blockDim.x) sObs[i] = pool[2 * (size_t)off + i];
+    for(uint32_t i = threadIdx.x; i < 2 * pairs; i += blockDim.x) sObs[i] = pool[2 * (size_t)off + i];
 }
 
 // total[key]++ for lanes with `has`, valid[key]++ for lanes with `has && ok`:
@@ -120,10 +122,14 @@ __device__ __forceinline__ void launch_regime(const QueryDesc& qd, uint32_t R, u
     nChildren         = nc > qd.cap ? qd.cap : (uint32_t)nc;
 }
 
-// Work lists for the next launches (block of NT threads, last workgroup).
+// Work lists of the next iteration into buffer `parity` (block of NT threads,
+// run by the last workgroup to arrive).
 template <int NT>
-__device__ void build_worklists(const QueryState* sts, const WorkLists& W, uint32_t nq, uint32_t* sbuf)
+__device__ void build_worklists(const QueryState* sts, const WorkLists& W, uint32_t parity, uint32_t* sbuf)
 {
+    const uint32_t nq = W.nq;
+    uint32_t* tileStart = W.tileStart + parity * (nq + 1);
+    uint32_t* jobStart  = W.jobStart + parity * (nq + 1);
     uint32_t carryT = 0, carryJ = 0, carryA = 0;
     for(uint32_t base = 0; base < nq; base += NT)
         {
@@ -149,17 +155,18 @@ __device__ void build_worklists(const QueryState* sts, const WorkLists& W, uint3
             block_excl_scan<NT>(alive, sbuf, tA);
             if(q < nq)
                 {
-                    W.tileStart[q] = carryT + eT;
-                    W.jobStart[q]  = carryJ + eJ;
+                    tileStart[q] = carryT + eT;
+                    jobStart[q]  = carryJ + eJ;
                 }
             carryT += tT, carryJ += tJ, carryA += tA;
         }
     if(threadIdx.x == 0)
         {
-            W.tileStart[nq] = carryT;
-            W.jobStart[nq]  = carryJ;
-            *W.allDone      = carryA == 0 ? 1u : 0u;
-            *W.counter      = 0;
+            tileStart[nq]     = carryT;
+            jobStart[nq]      = carryJ;
+            W.active[parity]  = carryA;
+            *W.counter        = 0;
+            __hip_atomic_store(W.allDone, carryA == 0 ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
 }
 
@@ -230,22 +237,70 @@ __global__ __launch_bounds__(kBlock) void k_init(const QueryDesc* __restrict__ q
             st.expanded  = 0;
             st.finishItr = 0;
         }
-    if(last_arrival(W.counter, nq, &sflag)) build_worklists<kBlock>(sts, W, nq, sbuf);
+    if(last_arrival(W.counter, nq, &sflag)) build_worklists<kBlock>(sts, W, 0, sbuf);
 }
 
 // ---------------------------------------------------------------------------
 // propagate: persistent over 256-child tiles
 // ---------------------------------------------------------------------------
+// LDS of one propagate tile: the draws of its 256 children and the order in
+// which the lanes take them.
+struct TileLds
+{
+    uint32_t hist[kTile];      // children per step bucket, then bucket offsets
+    uint16_t slot[kTile];      // lane -> child of the tile
+    float ctrl[KGMT_MAX_CONTROL][kTile];
+    int32_t steps[kTile];
+    float ufront[kTile];
+    uint32_t sbuf[kTile / 64 + 1];
+};
+
+// One tile of 256 children u = base .. base + 255 of a query. The children
+// are drawn in tile order, then handed to the lanes longest-first (counting
+// sort on the step count) so that each wave's lanes integrate about the same
+// number of steps. Each child still writes only its own slot and the
+// counters are integer atomics, so the outputs do not depend on the order.
 template <int MODEL>
-__device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState& st, uint32_t u,
-                                           const Pools& P, const float4* sObs)
+__device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState& st, uint32_t base,
+                                           const Pools& P, const float4* sObs, TileLds& L)
 {
     constexpr int S = ModelDims<MODEL>::S, C = ModelDims<MODEL>::C, D = S + C + 1;
-    const uint32_t nC   = st.nChildren;
-    const bool has      = u < nC;
-    const uint32_t cap  = qd.cap;
-    const size_t nodeO  = qd.nodeOff;
-    int r1              = 0;
+    const uint32_t nC  = st.nChildren;
+    const uint32_t cap = qd.cap;
+    const size_t nodeO = qd.nodeOff;
+    const uint32_t t   = threadIdx.x;
+    // draws of the child in tile order
+    {
+        const uint32_t un = base + t;
+        float uc[C];
+        int32_t steps = 0;
+        float uf      = 0.0f;
+        uint32_t bucket = kTile - 1;  // past the end: last
+        if(un < nC)
+            {
+                draw_child<C>(qd.cfg, qd.cfg.query_id, st.itr, un, uc, steps, uf);
+                const int32_t k = steps - qd.cfg.steps_min;
+                bucket          = (uint32_t)(kTile - 2) - (uint32_t)(k < (int32_t)kTile - 2 ? k : (int32_t)kTile - 2);
+            }
+        L.hist[t] = 0;
+        __syncthreads();
+        const uint32_t rank = atomicAdd(&L.hist[bucket], 1u);
+        __syncthreads();
+        uint32_t tot;
+        const uint32_t off = block_excl_scan<kTile>(L.hist[t], L.sbuf, tot);
+        L.hist[t]          = off;
+        __syncthreads();
+        L.slot[L.hist[bucket] + rank] = (uint16_t)t;
+#pragma unroll
+        for(int k = 0; k < C; ++k) L.ctrl[k][t] = uc[k];
+        L.steps[t]  = steps;
+        L.ufront[t] = uf;
+        __syncthreads();
+    }
+    const uint32_t i   = L.slot[t];
+    const uint32_t u   = base + i;
+    const bool has     = u < nC;
+    int r1             = 0;
     bool ok = false, fn = false;
     if(has)
         {
@@ -256,15 +311,17 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
 #pragma unroll
             for(int d = 0; d < S; ++d) s[d] = T[(size_t)d * cap + p];
             float uc[C];
-            int32_t steps;
-            float uf;
-            draw_child<C>(qd.cfg, qd.cfg.query_id, st.itr, u, uc, steps, uf);
-            ok       = propagate<MODEL>(qd, s, uc, steps, sObs, (int)qd.nObs);
+#pragma unroll
+            for(int k = 0; k < C; ++k) uc[k] = L.ctrl[k][i];
+            const int32_t steps = L.steps[i];
+            const float uf      = L.ufront[i];
+            const Scene sc{sObs, (int)qd.nGrp};
+            ok       = propagate<MODEL>(qd, s, uc, steps, sc);
             float* U = P.childS + qd.sampleOff;
 #pragma unroll
             for(int d = 0; d < S; ++d) U[(size_t)d * cap + u] = s[d];
 #pragma unroll
-            for(int i = 0; i < C; ++i) U[(size_t)(S + i) * cap + u] = uc[i];
+            for(int k = 0; k < C; ++k) U[(size_t)(S + k) * cap + u] = uc[k];
             U[(size_t)(D - 1) * cap + u] = qd.cfg.dt * (float)steps;
             P.parentU[nodeO + u]          = (int32_t)p;
             if(u % per == 0) P.frontier[nodeO + p] = 0;
@@ -281,44 +338,52 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
             P.fnext[nodeO + u] = fn ? 1 : 0;
         }
     wave_count(P.total + qd.regOff, P.validc + qd.regOff, r1, has, ok);
+    // the tile lies inside one 1024-element chunk
     const uint64_t fm = __ballot(fn);
-    if(lane_id() == 0 && fm) atomicAdd(&P.partA[qd.chunkOff + u / kChunk], (uint32_t)__popcll(fm));
+    if(lane_id() == 0 && fm) atomicAdd(&P.partA[qd.chunkOff + base / kChunk], (uint32_t)__popcll(fm));
+    __syncthreads();  // L is reused by the next tile
 }
 
 template <int MODEL>
 __global__ __launch_bounds__(kTile) void k_propagate(const QueryDesc* __restrict__ qds,
                                                      const QueryState* __restrict__ sts, Pools P, WorkLists W,
-                                                     uint32_t nq)
+                                                     uint32_t parity)
 {
     extern __shared__ float4 sObs[];
-    const uint32_t nTiles = W.tileStart[nq];
-    uint32_t staged       = 0xffffffffu;
+    __shared__ TileLds L;
+    const uint32_t nq        = W.nq;
+    const uint32_t* tileList = W.tiles(parity);
+    const uint32_t nTiles    = tileList[nq];
+    uint32_t staged          = 0xffffffffu;
     for(uint32_t t = blockIdx.x; t < nTiles; t += gridDim.x)
         {
-            const uint32_t q    = locate(W.tileStart, nq, t);
+            const uint32_t q    = locate(tileList, nq, t);
+            if(sts[q].done) continue;  // stale list entry of a finished query
             const QueryDesc& qd = qds[q];
             if(q != staged)
                 {
                     __syncthreads();
-                    stage_obstacles(P.obs, qd.obsOff, qd.nObs, sObs);
+                    stage_scene(P.obs, qd.obsOff, qd.nGrp * (1 + kGroup), sObs);
                     __syncthreads();
                     staged = q;
                 }
-            const uint32_t u = (t - W.tileStart[q]) * kTile + threadIdx.x;
+            const uint32_t base = (t - tileList[q]) * kTile;
             if constexpr(MODEL >= 0)
-                child_tile<MODEL>(qd, sts[q], u, P, sObs);
+                child_tile<MODEL>(qd, sts[q], base, P, sObs, L);
             else
                 {
                     switch(qd.cfg.model)
                         {
                             case KGMT_MODEL_DOUBLE_INTEGRATOR:
-                                child_tile<KGMT_MODEL_DOUBLE_INTEGRATOR>(qd, sts[q], u, P, sObs);
+                                child_tile<KGMT_MODEL_DOUBLE_INTEGRATOR>(qd, sts[q], base, P, sObs, L);
                                 break;
                             case KGMT_MODEL_DUBINS_AIRPLANE:
-                                child_tile<KGMT_MODEL_DUBINS_AIRPLANE>(qd, sts[q], u, P, sObs);
+                                child_tile<KGMT_MODEL_DUBINS_AIRPLANE>(qd, sts[q], base, P, sObs, L);
                                 break;
-                            case KGMT_MODEL_DUBINS_5D: child_tile<KGMT_MODEL_DUBINS_5D>(qd, sts[q], u, P, sObs); break;
-                            default: child_tile<KGMT_MODEL_QUADCOPTER>(qd, sts[q], u, P, sObs); break;
+                            case KGMT_MODEL_DUBINS_5D:
+                                child_tile<KGMT_MODEL_DUBINS_5D>(qd, sts[q], base, P, sObs, L);
+                                break;
+                            default: child_tile<KGMT_MODEL_QUADCOPTER>(qd, sts[q], base, P, sObs, L); break;
                         }
                 }
         }
@@ -430,18 +495,21 @@ __device__ void reactivate(const QueryDesc& qd, QueryState& st, uint32_t o, cons
 }
 
 __global__ __launch_bounds__(kBlock) void k_append(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
-                                                   Pools P, WorkLists W, uint32_t nq)
+                                                   Pools P, WorkLists W, uint32_t parity)
 {
     __shared__ uint32_t sbuf[kBlock / 64 + 1];
     __shared__ uint32_t sflag;
-    const uint32_t nJobs = W.jobStart[nq];
+    const uint32_t nq       = W.nq;
+    const uint32_t* jobList = W.jobs(parity);
+    const uint32_t nJobs    = jobList[nq];
     for(uint32_t j = blockIdx.x; j < nJobs; j += gridDim.x)
         {
-            const uint32_t q     = locate(W.jobStart, nq, j);
-            const QueryDesc& qd  = qds[q];
+            const uint32_t q     = locate(jobList, nq, j);
             QueryState& st       = sts[q];
-            const uint32_t local = j - W.jobStart[q];
-            const uint32_t jobsQ = W.jobStart[q + 1] - W.jobStart[q];
+            if(st.done) continue;  // stale list entry of a finished query
+            const QueryDesc& qd  = qds[q];
+            const uint32_t local = j - jobList[q];
+            const uint32_t jobsQ = jobList[q + 1] - jobList[q];
             const uint32_t nCC   = (st.nChildren + kChunk - 1) / kChunk;
             if(local < nCC)
                 append_children(qd, st, local, P, sbuf);
@@ -467,18 +535,87 @@ __global__ __launch_bounds__(kBlock) void k_append(const QueryDesc* __restrict__
 // expand: next expansion list (re-activated ascending, then new nodes each
 // repeated repU times), truncated at capacity entries
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__ qds,
-                                                   const QueryState* __restrict__ sts, Pools P, WorkLists W,
-                                                   uint32_t nq)
+// End of an iteration for one query (all threads of the workgroup; thread 0
+// does the bookkeeping): tree size, goal path, stop rules, next regime.
+__device__ void finish_query(const QueryDesc& qd, QueryState& st, const Pools& P)
+{
+    for(uint32_t c = threadIdx.x; c <= qd.nChunks; c += kBlock) P.partA[qd.chunkOff + c] = 0u;
+    if(threadIdx.x != 0) return;
+    const uint32_t tree = st.treeSize + st.next;
+    st.treeSize         = tree;
+    st.expanded += st.nChildren;
+    st.jobsDone = 0;
+    if(st.goalIdx != kNoGoal)
+        {
+            // goal -> root rows, exact depth (Q13)
+            const uint32_t cap = qd.cap;
+            const int D        = qd.D;
+            const float* T     = P.treeS + qd.sampleOff;
+            float* out         = P.path + qd.pathOff;
+            int32_t n          = st.goalIdx;
+            uint32_t rows      = 0;
+            float pc = 0.0f, pl = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
+            const uint32_t maxRows = qd.cfg.max_iter + 2;
+            while(n != -1 && rows < maxRows)
+                {
+                    for(int d = 0; d < D; ++d) out[rows * D + d] = T[(size_t)d * cap + n];
+                    const float x = T[n], y = T[(size_t)cap + n], z = T[2 * (size_t)cap + n];
+                    const int32_t par = P.parent[qd.nodeOff + n];
+                    if(par != -1) pc = pc + T[(size_t)(D - 1) * cap + n];
+                    if(rows > 0)
+                        {
+                            const float dx = px - x, dy = py - y, dz = pz - z;
+                            pl             = pl + __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+                        }
+                    px = x, py = y, pz = z;
+                    ++rows;
+                    n = par;
+                }
+            st.pathRows  = rows;
+            st.pathCost  = pc;
+            st.pathLen   = pl;
+            st.success   = 1;
+            st.done      = 1;
+            st.finishItr = st.itr;
+        }
+    else if(st.itr >= qd.cfg.max_iter || tree == qd.cap)
+        {
+            // out of iterations, or a full tree that can take no further node
+            st.done      = 1;
+            st.finishItr = st.itr;
+        }
+    else
+        {
+            uint32_t per, nc;
+            launch_regime(qd, st.R, tree, per, nc);
+            st.per       = per;
+            st.nChildren = nc;
+            st.itr       = st.itr + 1;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// expand: next expansion list (re-activated ascending, then new nodes each
+// repeated repU times), truncated at capacity entries; the last job of a query
+// closes its iteration, the last query to close rebuilds the work lists.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
+                                                   Pools P, WorkLists W, uint32_t parity)
 {
     __shared__ uint32_t sbuf[kBlock / 64 + 1];
-    const uint32_t nJobs = W.jobStart[nq];
+    __shared__ uint32_t sflag;
+    const uint32_t nq       = W.nq;
+    const uint32_t* jobList = W.jobs(parity);
+    const uint32_t nJobs    = jobList[nq];
+    const uint32_t active   = W.active[parity];
     for(uint32_t j = blockIdx.x; j < nJobs; j += gridDim.x)
         {
-            const uint32_t q      = locate(W.jobStart, nq, j);
+            const uint32_t q      = locate(jobList, nq, j);
+            QueryState& st        = sts[q];
+            if(st.done) continue;  // stale list entry of a finished query
             const QueryDesc& qd   = qds[q];
-            const QueryState& st  = sts[q];
-            const uint32_t local  = j - W.jobStart[q];
+            const uint32_t local  = j - jobList[q];
+            const uint32_t jobsQ  = jobList[q + 1] - jobList[q];
             const uint32_t nC     = st.nChildren;
             const uint32_t nCC    = (nC + kChunk - 1) / kChunk;
             const uint32_t cap    = qd.cap, tree = st.treeSize;
@@ -523,78 +660,13 @@ __global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__
                             run += tot;
                         }
                 }
-        }
-}
-
-// ---------------------------------------------------------------------------
-// plan: per-query bookkeeping of the iteration end; last workgroup rebuilds
-// the work lists (KGMT.cu:84-95, 183-201, 342-361, 399-402)
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_plan(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
-                                                 Pools P, WorkLists W, uint32_t nq)
-{
-    __shared__ uint32_t sbuf[kBlock / 64 + 1];
-    __shared__ uint32_t sflag;
-    const uint32_t q    = blockIdx.x;
-    const QueryDesc& qd = qds[q];
-    QueryState& st      = sts[q];
-    if(!st.done)
-        {
-            for(uint32_t c = threadIdx.x; c <= qd.nChunks; c += kBlock) P.partA[qd.chunkOff + c] = 0u;
-            if(threadIdx.x == 0)
+            if(last_arrival(&st.jobsDone, jobsQ, &sflag))
                 {
-                    const uint32_t tree = st.treeSize + st.next;
-                    st.treeSize         = tree;
-                    st.expanded += st.nChildren;
-                    if(st.goalIdx != kNoGoal)
-                        {
-                            // goal -> root rows, exact depth (Q13)
-                            const uint32_t cap = qd.cap;
-                            const int D        = qd.D;
-                            const float* T     = P.treeS + qd.sampleOff;
-                            float* out         = P.path + qd.pathOff;
-                            int32_t n          = st.goalIdx;
-                            uint32_t rows      = 0;
-                            float pc = 0.0f, pl = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
-                            const uint32_t maxRows = qd.cfg.max_iter + 2;
-                            while(n != -1 && rows < maxRows)
-                                {
-                                    for(int d = 0; d < D; ++d) out[rows * D + d] = T[(size_t)d * cap + n];
-                                    const float x = T[n], y = T[(size_t)cap + n], z = T[2 * (size_t)cap + n];
-                                    const int32_t par = P.parent[qd.nodeOff + n];
-                                    if(par != -1) pc = pc + T[(size_t)(D - 1) * cap + n];
-                                    if(rows > 0)
-                                        {
-                                            const float dx = px - x, dy = py - y, dz = pz - z;
-                                            pl             = pl + __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
-                                        }
-                                    px = x, py = y, pz = z;
-                                    ++rows;
-                                    n = par;
-                                }
-                            st.pathRows  = rows;
-                            st.pathCost  = pc;
-                            st.pathLen   = pl;
-                            st.success   = 1;
-                            st.done      = 1;
-                            st.finishItr = st.itr;
-                        }
-                    else if(st.itr >= qd.cfg.max_iter)
-                        {
-                            st.done      = 1;
-                            st.finishItr = st.itr;
-                        }
-                    else
-                        {
-                            uint32_t per, nc;
-                            launch_regime(qd, st.R, tree, per, nc);
-                            st.per       = per;
-                            st.nChildren = nc;
-                            st.itr       = st.itr + 1;
-                        }
+                    finish_query(qd, st, P);
+                    if(last_arrival(W.counter, active, &sflag)) build_worklists<kBlock>(sts, W, parity ^ 1u, sbuf);
                 }
+            __syncthreads();
         }
-    if(last_arrival(W.counter, nq, &sflag)) build_worklists<kBlock>(sts, W, nq, sbuf);
 }
 
 // ---------------------------------------------------------------------------
@@ -623,13 +695,14 @@ __global__ __launch_bounds__(kBlock) void k_stage_propagate(const QueryDesc* __r
                                                             const float* __restrict__ parentStates,
                                                             uint32_t nParents, const float* __restrict__ controls,
                                                             const int32_t* __restrict__ steps, uint32_t n,
-                                                            const float4* __restrict__ obs, uint32_t nObs,
+                                                            const float4* __restrict__ obs, uint32_t nGrp,
+                                                            uint32_t nObs,
                                                             float* child, uint8_t* valid, int32_t* r1o, int32_t* r2o)
 {
     constexpr int S = ModelDims<MODEL>::S, C = ModelDims<MODEL>::C, D = S + C + 1;
     extern __shared__ float4 sObs[];
     const QueryDesc& qd = *qdp;
-    stage_obstacles(obs, 0, nObs, sObs);
+    stage_scene(obs, 0, nGrp * (1 + kGroup), sObs);
     __syncthreads();
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if(i >= n) return;
@@ -649,7 +722,9 @@ __global__ __launch_bounds__(kBlock) void k_stage_propagate(const QueryDesc* __r
 #pragma unroll
     for(int k = 0; k < C; ++k) uc[k] = controls[(size_t)k * n + i];
     const int32_t st = steps[i];
-    const bool ok    = propagate<MODEL>(qd, s, uc, st, sObs, (int)nObs);
+    const Scene sc{sObs, (int)nGrp};
+    (void)nObs;
+    const bool ok    = propagate<MODEL>(qd, s, uc, st, sc);
 #pragma unroll
     for(int d = 0; d < S; ++d) child[(size_t)d * n + i] = s[d];
 #pragma unroll
@@ -780,33 +855,32 @@ __global__ __launch_bounds__(kBlock) void k_stage_repeat(const uint32_t* __restr
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_init(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, uint32_t nq,
-                       hipStream_t s)
+hipError_t launch_init(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_init, dim3(nq), dim3(kBlock), 0, s, qds, sts, P, W, nq);
+    hipLaunchKernelGGL(k_init, dim3(W.nq), dim3(kBlock), 0, s, qds, sts, P, W, W.nq);
     return hipGetLastError();
 }
 
 hipError_t launch_propagate(int model, const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
-                            uint32_t nq, uint32_t maxObs, const Launch& L, hipStream_t s)
+                            uint32_t parity, uint32_t maxPairs, const Launch& L, hipStream_t s)
 {
-    const size_t lds = (size_t)maxObs * 2 * sizeof(float4);
+    const size_t lds = (size_t)maxPairs * 2 * sizeof(float4);
     const dim3 g(L.propGrid), b(kTile);
     switch(model)
         {
             case KGMT_MODEL_DOUBLE_INTEGRATOR:
-                hipLaunchKernelGGL(k_propagate<KGMT_MODEL_DOUBLE_INTEGRATOR>, g, b, lds, s, qds, sts, P, W, nq);
+                hipLaunchKernelGGL(k_propagate<KGMT_MODEL_DOUBLE_INTEGRATOR>, g, b, lds, s, qds, sts, P, W, parity);
                 break;
             case KGMT_MODEL_DUBINS_AIRPLANE:
-                hipLaunchKernelGGL(k_propagate<KGMT_MODEL_DUBINS_AIRPLANE>, g, b, lds, s, qds, sts, P, W, nq);
+                hipLaunchKernelGGL(k_propagate<KGMT_MODEL_DUBINS_AIRPLANE>, g, b, lds, s, qds, sts, P, W, parity);
                 break;
             case KGMT_MODEL_DUBINS_5D:
-                hipLaunchKernelGGL(k_propagate<KGMT_MODEL_DUBINS_5D>, g, b, lds, s, qds, sts, P, W, nq);
+                hipLaunchKernelGGL(k_propagate<KGMT_MODEL_DUBINS_5D>, g, b, lds, s, qds, sts, P, W, parity);
                 break;
             case KGMT_MODEL_QUADCOPTER:
-                hipLaunchKernelGGL(k_propagate<KGMT_MODEL_QUADCOPTER>, g, b, lds, s, qds, sts, P, W, nq);
+                hipLaunchKernelGGL(k_propagate<KGMT_MODEL_QUADCOPTER>, g, b, lds, s, qds, sts, P, W, parity);
                 break;
-            default: hipLaunchKernelGGL(k_propagate<-1>, g, b, lds, s, qds, sts, P, W, nq); break;
+            default: hipLaunchKernelGGL(k_propagate<-1>, g, b, lds, s, qds, sts, P, W, parity); break;
         }
     return hipGetLastError();
 }
@@ -817,24 +891,17 @@ hipError_t launch_score(const QueryDesc* qds, QueryState* sts, const Pools& P, u
     return hipGetLastError();
 }
 
-hipError_t launch_append(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, uint32_t nq,
-                         const Launch& L, hipStream_t s)
+hipError_t launch_append(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
+                         uint32_t parity, const Launch& L, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_append, dim3(L.chunkGrid), dim3(kBlock), 0, s, qds, sts, P, W, nq);
+    hipLaunchKernelGGL(k_append, dim3(L.chunkGrid), dim3(kBlock), 0, s, qds, sts, P, W, parity);
     return hipGetLastError();
 }
 
-hipError_t launch_expand(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, uint32_t nq,
-                         const Launch& L, hipStream_t s)
+hipError_t launch_expand(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
+                         uint32_t parity, const Launch& L, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_expand, dim3(L.chunkGrid), dim3(kBlock), 0, s, qds, sts, P, W, nq);
-    return hipGetLastError();
-}
-
-hipError_t launch_plan(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, uint32_t nq,
-                       hipStream_t s)
-{
-    hipLaunchKernelGGL(k_plan, dim3(nq), dim3(kBlock), 0, s, qds, sts, P, W, nq);
+    hipLaunchKernelGGL(k_expand, dim3(L.chunkGrid), dim3(kBlock), 0, s, qds, sts, P, W, parity);
     return hipGetLastError();
 }
 
@@ -854,15 +921,15 @@ hipError_t launch_stage_draw(const QueryDesc* qd, int C, uint32_t query, uint32_
 
 hipError_t launch_stage_propagate(int model, const QueryDesc* qd, const int32_t* parentIdx, const float* parentStates,
                                   uint32_t nParents, const float* controls, const int32_t* steps, uint32_t n,
-                                  const float4* obs, uint32_t nObs, float* child, uint8_t* valid, int32_t* r1,
-                                  int32_t* r2, hipStream_t s)
+                                  const float4* obs, uint32_t nGrp, uint32_t nObs, float* child, uint8_t* valid,
+                                  int32_t* r1, int32_t* r2, hipStream_t s)
 {
     if(n == 0) return hipSuccess;
     const dim3 g((n + kBlock - 1) / kBlock), b(kBlock);
-    const size_t lds = (size_t)(nObs ? nObs : 1) * 2 * sizeof(float4);
+    const size_t lds = (size_t)(nGrp ? nGrp * (1 + kGroup) : 1) * 2 * sizeof(float4);
 #define KGMT_STAGE_PROP(M)                                                                                     \
     hipLaunchKernelGGL(k_stage_propagate<M>, g, b, lds, s, qd, parentIdx, parentStates, nParents, controls, steps, \
-                       n, obs, nObs, child, valid, r1, r2)
+                       n, obs, nGrp, nObs, child, valid, r1, r2)
     switch(model)
         {
             case KGMT_MODEL_DOUBLE_INTEGRATOR: KGMT_STAGE_PROP(KGMT_MODEL_DOUBLE_INTEGRATOR); break;

@@ -16,26 +16,29 @@ struct Launch
     uint32_t chunkGrid = 1024;  // persistent workgroups of append / expand
 };
 
-// planner iteration kernels; `model` = KGMT_MODEL_* when every query shares it, -1 otherwise
-hipError_t launch_init(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, uint32_t nq,
-                       hipStream_t s);
+// Scene of one query: boxes sorted along a Morton curve of their centres,
+// grouped by kGroup, written as nGrp group bounds followed by the boxes, each
+// a (lo, hi) float4 pair. Returns nGrp. rows: n x 6 floats.
+uint32_t build_scene(const float* rows, uint32_t n, float4* out);
+
+// planner iteration kernels; `model` = KGMT_MODEL_* when every query shares
+// it, -1 otherwise; `parity` = (iteration - 1) & 1 selects the work lists
+hipError_t launch_init(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, hipStream_t s);
 hipError_t launch_propagate(int model, const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
-                            uint32_t nq, uint32_t maxObs, const Launch& L, hipStream_t s);
+                            uint32_t parity, uint32_t maxPairs, const Launch& L, hipStream_t s);
 hipError_t launch_score(const QueryDesc* qds, QueryState* sts, const Pools& P, uint32_t nq, hipStream_t s);
-hipError_t launch_append(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, uint32_t nq,
-                         const Launch& L, hipStream_t s);
-hipError_t launch_expand(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, uint32_t nq,
-                         const Launch& L, hipStream_t s);
-hipError_t launch_plan(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, uint32_t nq,
-                       hipStream_t s);
+hipError_t launch_append(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
+                         uint32_t parity, const Launch& L, hipStream_t s);
+hipError_t launch_expand(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
+                         uint32_t parity, const Launch& L, hipStream_t s);
 
 // stage-level kernels (one query, caller buffers)
 hipError_t launch_stage_draw(const QueryDesc* qd, int C, uint32_t query, uint32_t itr, uint32_t u0, uint32_t n,
                              float* controls, int32_t* steps, float* ufront, hipStream_t s);
 hipError_t launch_stage_propagate(int model, const QueryDesc* qd, const int32_t* parentIdx, const float* parentStates,
                                   uint32_t nParents, const float* controls, const int32_t* steps, uint32_t n,
-                                  const float4* obs, uint32_t nObs, float* child, uint8_t* valid, int32_t* r1,
-                                  int32_t* r2, hipStream_t s);
+                                  const float4* obs, uint32_t nGrp, uint32_t nObs, float* child, uint8_t* valid,
+                                  int32_t* r1, int32_t* r2, hipStream_t s);
 hipError_t launch_stage_region_update(const QueryDesc* qd, const int32_t* r1, const int32_t* r2, const uint8_t* valid,
                                       const float* ufront, uint32_t n, const float* score, const uint32_t* actPrev,
                                       uint32_t* actNext, int32_t* total, int32_t* validc, uint8_t* fnext,

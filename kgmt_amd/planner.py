@@ -105,6 +105,10 @@ class KGMT(Planner):
         _lib.check(_lib.lib().kgmt_get_profile(self._h, C.byref(p)))
         return p.as_dict()
 
+    def set_graph(self, on: bool = True):
+        """Replay iterations from a captured hipGraph (ignored while profiling)."""
+        _lib.check(_lib.lib().kgmt_set_graph(self._h, 1 if on else 0))
+
     # KGMT::plan, src/planners/KGMT.cu:41-105
     def plan(self, h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount) -> KgmtResult:
         start = _f32(h_initial, self.D)

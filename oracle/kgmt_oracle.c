@@ -18,7 +18,8 @@
  *   Q5  expansion multiset = {i repeated repeat[i] times, repeat[i] > 0}, ascending i;
  *   Q6  region scores use the closed formula with a fixed summation order;
  *   Q8  the last frontierNext flag is counted;
- *   Q9  appended nodes are clamped to the remaining capacity;
+ *   Q9  appended nodes are clamped to the remaining capacity, and a plan ends
+ *       (unsuccessfully) once the tree is full;
  *   Q10 among goal hits of one iteration the smallest tree index wins;
  *   Q11/Q12 randomness is Philox4x32-10 keyed by (seed) with counter
  *       (index, iteration, stream, query) — no per-thread RNG state;
@@ -847,6 +848,9 @@ int orc_plan(const kgmt_config* cfg, const float* start, const float* goal, cons
                     goalIdx = best;
                     break;
                 }
+            /* a full tree can never take another node, so the goal is out of
+             * reach: the plan ends unsuccessfully */
+            if(treeSize == N) break;
         }
 
     /* path goal->root (KGMT.cu:342-361), exact depth (Q13) */

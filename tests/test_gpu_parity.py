@@ -193,3 +193,56 @@ def test_compact_and_repeat(n):
     call("kgmt_repeat", ptr(d_ids), ptr(d_counts), n, ptr(o2), cap, ptr(tot), stream())
     assert host(tot)[0] == len(rep)
     assert np.array_equal(host(o2)[:cap].view(np.uint32), rep[:cap])
+
+
+@pytest.mark.parametrize("kind", ["edge_boxes", "max_boxes", "reference_house"])
+def test_scene_grouping_keeps_verdicts(kind, orc):
+    """Boxes are grouped and reordered on the device side; verdicts must stay
+    identical to the oracle's plain loop over all boxes, including inverted
+    boxes, NaN coordinates, boxes beyond the workspace and the maximum count."""
+    import os
+    cfg = CASES["di_ref"]()
+    rng = np.random.default_rng(21)
+    if kind == "edge_boxes":
+        obs = W.random_boxes(rng, 40, 0.05, 0.3)
+        obs[3] = [0.6, 0.2, 0.2, 0.4, 0.5, 0.5]          # inverted on x
+        obs[7] = [np.nan, 0.1, 0.1, 0.3, 0.3, 0.3]       # NaN face: that face never separates
+        obs[11] = [-5, -5, -5, -4, -4, -4]               # far outside
+        obs[19] = [0.2, 0.2, 0.2, 0.2, 0.9, 0.9]         # zero thickness
+        obs = obs.astype(np.float32)
+    elif kind == "max_boxes":
+        obs = W.random_boxes(rng, 1500, 0.005, 0.03)
+    else:
+        here = os.path.dirname(os.path.abspath(__file__))
+        obs = W.read_obstacles_csv(os.path.join(here, "golden", "obstacles", "house.csv"))
+    S, Cd, D = dims(cfg.model)
+    n, npar = 30000, 900
+    X = parent_states(cfg, npar, seed=9)
+    pidx = rng.integers(0, npar, n).astype(np.int32)
+    ctrl, steps, _ = orc.draw_children(cfg, 0, 2, 0, n)
+    want = orc.propagate_children(cfg, pidx, X, ctrl, steps, obs)
+    child = empty((D, n), torch.float32)
+    valid = empty(n, torch.uint8)
+    r1 = empty(n, torch.int32)
+    r2 = empty(n, torch.int32)
+    d_obs, d_pidx, d_X, d_ctrl, d_steps = dev(obs.reshape(-1)), dev(pidx), dev(X), dev(ctrl), dev(steps)
+    call("kgmt_propagate_collide", C.byref(cfg), ptr(d_pidx), ptr(d_X), npar, ptr(d_ctrl), ptr(d_steps),
+         n, ptr(d_obs), len(obs), ptr(child), ptr(valid), ptr(r1), ptr(r2), stream())
+    assert np.array_equal(host(valid), want[1])
+    assert np.array_equal(bits(host(child)), bits(want[0]))
+    if kind == "edge_boxes":
+        # a NaN face never separates, so the box acts as if that face were at -inf
+        alt = obs.copy()
+        alt[7, 0] = -np.inf
+        assert np.array_equal(orc.propagate_children(cfg, pidx, X, ctrl, steps, alt)[1], want[1])
+
+
+def test_too_many_obstacles_is_an_error():
+    from kgmt_amd import _lib
+    cfg = CASES["di_ref"]()
+    obs = np.zeros((1501, 6), np.float32)
+    d_obs = dev(obs.reshape(-1))
+    buf = empty(16, torch.float32)
+    rc = _lib.lib().kgmt_propagate_collide(C.byref(cfg), ptr(buf), ptr(buf), 1, ptr(buf), ptr(buf), 1,
+                                           ptr(d_obs), 1501, ptr(buf), ptr(buf), ptr(buf), ptr(buf), stream())
+    assert rc == -4

@@ -49,9 +49,23 @@ def test_plan_matches_oracle(name, seed, kw, orc):
     g = KGMT(q.cfg)
     res = g.plan(q.start, q.goal, q.obstacles, len(q.obstacles))
     _same_plan(g, res, ref)
-    # a second plan on the same context starts from scratch
+    # a second plan on the same context starts from scratch; then graph replay
     res2 = g.plan(q.start, q.goal, q.obstacles, len(q.obstacles))
     _same_plan(g, res2, ref)
+    g.set_graph(True)
+    for _ in range(2):
+        res3 = g.plan(q.start, q.goal, q.obstacles, len(q.obstacles))
+        _same_plan(g, res3, ref)
+
+
+def test_full_tree_ends_the_plan(orc):
+    q = W.make_query("dubins5d", 1, capacity=3000, max_iter=60)
+    box = np.array([[0.8, 0.8, 0.8, 1.0, 1.0, 1.0]], np.float32)
+    ref = orc.plan(q.cfg, q.start, q.goal, box)
+    assert ref["stats"]["tree_size"] == 3000 and ref["stats"]["iterations"] < 60
+    g = KGMT(q.cfg)
+    res = g.plan(q.start, q.goal, box, 1)
+    _same_plan(g, res, ref)
 
 
 def test_plan_device_obstacles(orc):
@@ -71,12 +85,19 @@ def test_plan_no_obstacles_and_unreachable(orc):
     g = KGMT(q.cfg)
     res = g.plan(q.start, q.goal, empty, 0)
     _same_plan(g, res, ref)
-    # goal enclosed by a box: never reached, runs max_iter iterations
+    # goal enclosed by a box: never reached; the plan ends when the tree is full
     box = np.array([[0.85, 0.85, 0.85, 1.0, 1.0, 1.0]], np.float32)
     ref = orc.plan(q.cfg, q.start, q.goal, box)
     res = g.plan(q.start, q.goal, box, 1)
-    assert not res.success and res.iterations == q.cfg.max_iter
+    assert not res.success and res.tree_size == q.cfg.capacity
     _same_plan(g, res, ref)
+    # with a large capacity it runs all max_iter iterations instead
+    big = q.cfg.replace(capacity=1 << 17, max_iter=6)
+    ref = orc.plan(big, q.start, q.goal, box)
+    g2 = KGMT(big)
+    res = g2.plan(q.start, q.goal, box, 1)
+    assert not res.success and res.iterations == 6
+    _same_plan(g2, res, ref)
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
