@@ -142,6 +142,22 @@ typedef struct kgmt_profile {
     uint64_t iterations; /* profiled iterations */
 } kgmt_profile;
 
+/* Trajectory refinement (differentiable RK4 rollout of the quadcopter ODE).
+ * For trajectory k with T steps of dt, controls u_t (t < T), states
+ * x_{t+1} = RK4(x_t, u_t), positions p_t = x_t[0..3):
+ *   J = w_effort * sum_t |u_t - u_ref|^2
+ *     + w_obstacle * sum_{t=1..T} sum_boxes tau * softplus((margin - sd(p_t, box)) / tau)
+ *     + w_goal * |p_T - goal|^2
+ * sd = signed distance from a point to an axis-aligned box. */
+typedef struct kgmt_adjoint_params {
+    float    w_effort, w_obstacle, w_goal;
+    float    margin, tau;
+    float    u_ref[4];     /* control the effort is measured from (e.g. hover thrust) */
+    float    lr, beta1, beta2, adam_eps;  /* Adam step of kgmt_refine */
+    uint32_t iterations;   /* Adam steps of kgmt_refine */
+    int32_t  clamp;        /* clamp controls to cfg ctrl_min/max after each step */
+} kgmt_adjoint_params;
+
 typedef struct kgmt_ctx kgmt_ctx;
 typedef struct kgmt_batch kgmt_batch;
 
@@ -244,6 +260,24 @@ int kgmt_compact(const uint8_t* d_flags, uint32_t n, uint32_t* d_out, uint32_t* 
  * *d_total = sum of counts. */
 int kgmt_repeat(const uint32_t* d_ids, const uint32_t* d_counts, uint32_t n, uint32_t* d_out,
                 uint32_t out_cap, uint32_t* d_total, void* stream);
+
+/* ---- trajectory refinement (quadcopter RK4 adjoint) ----------------------
+ * No counterpart in the reference planner: an extension that differentiates
+ * its quadcopter rollout (statePropagator.cu:246-370) to refine control
+ * sequences. Device buffers, SoA over trajectories k < n:
+ *   steps [n] int32 (T_k <= max_steps), x0 [12][n], goal [3][n],
+ *   u / grad [max_steps][4][n] (element (t, c, k) at (t*4 + c)*n + k),
+ *   obstacles: 6 floats per box (shared by all trajectories).
+ * cfg supplies the quad parameters, dt and (for clamping) ctrl_min/max. */
+/* J_k -> cost[n] and dJ_k/du -> grad (zero past T_k). */
+int kgmt_rk4_adjoint(const kgmt_config* cfg, const kgmt_adjoint_params* ap, uint32_t n, uint32_t max_steps,
+                     const int32_t* d_steps, const float* d_x0, const float* d_goal, const float* d_u,
+                     const float* d_obstacles, uint32_t n_obs, float* d_cost, float* d_grad, void* stream);
+/* ap->iterations Adam steps on u in place; cost_history [(iterations+1)][n]
+ * holds J before every step and after the last. */
+int kgmt_refine(const kgmt_config* cfg, const kgmt_adjoint_params* ap, uint32_t n, uint32_t max_steps,
+                const int32_t* d_steps, const float* d_x0, const float* d_goal, float* d_u,
+                const float* d_obstacles, uint32_t n_obs, float* d_cost_history, void* stream);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-from .abi import KgmtConfig, KgmtProfile, KgmtResult
+from .abi import KgmtAdjointParams, KgmtConfig, KgmtProfile, KgmtResult
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libkgmt.so")
 
@@ -62,6 +62,9 @@ SIGNATURES = {
     "kgmt_score": (C.c_int, [_cfgp, _P, _P, _P, _P, _P]),
     "kgmt_compact": (C.c_int, [_P, _u32, _P, _P, _P]),
     "kgmt_repeat": (C.c_int, [_P, _P, _u32, _P, _u32, _P, _P]),
+    "kgmt_rk4_adjoint": (C.c_int, [_cfgp, C.POINTER(KgmtAdjointParams), _u32, _u32, _P, _P, _P, _P, _P, _u32, _P, _P,
+                                   _P]),
+    "kgmt_refine": (C.c_int, [_cfgp, C.POINTER(KgmtAdjointParams), _u32, _u32, _P, _P, _P, _P, _P, _u32, _P, _P]),
 }
 
 _lib = None

@@ -130,6 +130,23 @@ class KgmtResult(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class KgmtAdjointParams(C.Structure):
+    _fields_ = [
+        ("w_effort", C.c_float),
+        ("w_obstacle", C.c_float),
+        ("w_goal", C.c_float),
+        ("margin", C.c_float),
+        ("tau", C.c_float),
+        ("u_ref", C.c_float * 4),
+        ("lr", C.c_float),
+        ("beta1", C.c_float),
+        ("beta2", C.c_float),
+        ("adam_eps", C.c_float),
+        ("iterations", C.c_uint32),
+        ("clamp", C.c_int32),
+    ]
+
+
 KGMT_PROFILE_KERNELS = 4
 PROFILE_KERNEL_NAMES = ("propagate", "score", "append", "expand")
 
@@ -153,3 +170,4 @@ class KgmtProfile(C.Structure):
 
 assert C.sizeof(KgmtConfig) == 160, C.sizeof(KgmtConfig)
 assert C.sizeof(KgmtResult) == 40, C.sizeof(KgmtResult)
+assert C.sizeof(KgmtAdjointParams) == 60, C.sizeof(KgmtAdjointParams)

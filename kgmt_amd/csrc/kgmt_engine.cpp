@@ -1016,4 +1016,61 @@ int kgmt_repeat(const uint32_t* d_ids, const uint32_t* d_counts, uint32_t n, uin
     return KGMT_OK;
 }
 
+// ---- trajectory refinement --------------------------------------------------
+static int adjoint_common(bool refine, const kgmt_config* cfg, const kgmt_adjoint_params* ap, uint32_t n,
+                          uint32_t max_steps, const int32_t* d_steps, const float* d_x0, const float* d_goal, float* d_u,
+                          const float* d_obstacles, uint32_t n_obs, float* d_cost, float* d_grad, void* stream)
+{
+    if(!cfg || !ap) return fail(KGMT_E_ARG, "null argument");
+    if(cfg->model != KGMT_MODEL_QUADCOPTER) return fail(KGMT_E_UNSUPPORTED, "refinement is built for the quadcopter");
+    if(n && (!d_steps || !d_x0 || !d_goal || !d_u || !d_cost || (!refine && !d_grad)))
+        return fail(KGMT_E_ARG, "null buffer");
+    if(n_obs > kMaxObs) return fail(KGMT_E_CAPACITY, "more than 1500 obstacles");
+    if(n_obs && !d_obstacles) return fail(KGMT_E_ARG, "obstacles pointer is null");
+    if(!(ap->tau > 0.0f)) return fail(KGMT_E_ARG, "tau must be positive");
+    if(refine && ap->iterations > 100000) return fail(KGMT_E_ARG, "too many iterations");
+    if(n == 0) return KGMT_OK;
+    StageDesc sd((hipStream_t)stream);
+    int rc = sd.make(cfg);
+    if(rc) return rc;
+    const size_t ck = ((size_t)max_steps + 1) * 12 * n, mv = (size_t)max_steps * 4 * n;
+    float *ckpt = nullptr, *mom = nullptr, *vel = nullptr;
+    float4* boxes = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&ckpt, ck * sizeof(float), sd.s));
+    HIP_TRY(hipMallocAsync((void**)&boxes, sizeof(float4) * 2 * std::max<uint32_t>(n_obs, 1), sd.s));
+    if(refine)
+        {
+            HIP_TRY(hipMallocAsync((void**)&mom, std::max<size_t>(mv, 1) * sizeof(float), sd.s));
+            HIP_TRY(hipMallocAsync((void**)&vel, std::max<size_t>(mv, 1) * sizeof(float), sd.s));
+            HIP_TRY(hipMemsetAsync(mom, 0, mv * sizeof(float), sd.s));
+            HIP_TRY(hipMemsetAsync(vel, 0, mv * sizeof(float), sd.s));
+        }
+    hipError_t e = launch_pack_boxes(d_obstacles, n_obs, boxes, sd.s);
+    if(e == hipSuccess)
+        e = launch_adjoint(refine, sd.d, *ap, n, max_steps, d_steps, d_x0, d_goal, d_u, d_grad, ckpt, mom, vel, boxes,
+                           n_obs, d_cost, sd.s);
+    (void)hipFreeAsync(ckpt, sd.s);
+    (void)hipFreeAsync(boxes, sd.s);
+    if(mom) (void)hipFreeAsync(mom, sd.s);
+    if(vel) (void)hipFreeAsync(vel, sd.s);
+    if(e != hipSuccess) return fail(KGMT_E_HIP, std::string("adjoint: ") + hipGetErrorString(e));
+    return KGMT_OK;
+}
+
+int kgmt_rk4_adjoint(const kgmt_config* cfg, const kgmt_adjoint_params* ap, uint32_t n, uint32_t max_steps,
+                     const int32_t* d_steps, const float* d_x0, const float* d_goal, const float* d_u,
+                     const float* d_obstacles, uint32_t n_obs, float* d_cost, float* d_grad, void* stream)
+{
+    return adjoint_common(false, cfg, ap, n, max_steps, d_steps, d_x0, d_goal, const_cast<float*>(d_u), d_obstacles,
+                          n_obs, d_cost, d_grad, stream);
+}
+
+int kgmt_refine(const kgmt_config* cfg, const kgmt_adjoint_params* ap, uint32_t n, uint32_t max_steps,
+                const int32_t* d_steps, const float* d_x0, const float* d_goal, float* d_u,
+                const float* d_obstacles, uint32_t n_obs, float* d_cost_history, void* stream)
+{
+    return adjoint_common(true, cfg, ap, n, max_steps, d_steps, d_x0, d_goal, d_u, d_obstacles, n_obs,
+                          d_cost_history, nullptr, stream);
+}
+
 }  // extern "C"

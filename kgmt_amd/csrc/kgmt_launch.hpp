@@ -50,4 +50,10 @@ hipError_t launch_stage_compact(const uint8_t* flags, uint32_t n, uint32_t* out,
 hipError_t launch_stage_repeat(const uint32_t* ids, const uint32_t* counts, uint32_t n, uint32_t* out,
                                uint32_t outCap, uint32_t* total, uint32_t* tmp, hipStream_t s);
 
+// trajectory refinement (kgmt_adjoint.hip)
+hipError_t launch_pack_boxes(const float* rows, uint32_t n, float4* out, hipStream_t s);
+hipError_t launch_adjoint(bool refine, const QueryDesc* qd, const kgmt_adjoint_params& ap, uint32_t n, uint32_t maxT,
+                          const int32_t* steps, const float* x0, const float* goal, float* u, float* grad, float* ckpt,
+                          float* mom, float* vel, const float4* obs, uint32_t nObs, float* cost, hipStream_t s);
+
 }  // namespace kgmt

@@ -166,6 +166,56 @@ def make_batch(n_queries: int = 512, seed: int = 0, max_iter: int = 300):
     return out
 
 
+@dataclass
+class RefineBatch:
+    cfg: KgmtConfig            # quadcopter parameters, dt, control bounds
+    params: object             # KgmtAdjointParams
+    steps: np.ndarray          # (n,) int32 trajectory lengths
+    x0: np.ndarray             # (12, n) initial states
+    goal: np.ndarray           # (3, n) goals
+    u: np.ndarray              # (max_steps, 4, n) initial control sequences
+    obstacles: np.ndarray      # (m, 6) boxes shared by all trajectories
+
+
+def refine_params(cfg: KgmtConfig, **kw):
+    """Cost weights of the refinement workload: effort measured from hover
+    thrust, softplus obstacle penalty with margin 0.02, squared goal distance;
+    Adam with lr 1e-2 for 50 steps."""
+    from .abi import KgmtAdjointParams
+    p = KgmtAdjointParams()
+    p.w_effort, p.w_obstacle, p.w_goal = 1e-3, 1.0, 100.0
+    p.margin, p.tau = 0.02, 0.01
+    hover = -cfg.gravity * cfg.quad_mass
+    for i, v in enumerate([hover, 0.0, 0.0, 0.0]):
+        p.u_ref[i] = v
+    p.lr, p.beta1, p.beta2, p.adam_eps = 1e-2, 0.9, 0.999, 1e-8
+    p.iterations, p.clamp = 50, 1
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def make_refine_batch(n: int = 4096, seed: int = 0, steps_lo: int = 20, steps_hi: int = 80) -> RefineBatch:
+    """Trajectory refinement workload: n quadcopter control sequences of
+    length U{steps_lo..steps_hi} at dt 0.01 from rest in the unit cube,
+    controls hover thrust + noise, goals 0.05-0.15 away, 50 shared boxes."""
+    cfg = config("quad12d")
+    rng = _rng(seed, 5)
+    steps = rng.integers(steps_lo, steps_hi + 1, n).astype(np.int32)
+    T = int(steps.max())
+    x0 = np.zeros((12, n), np.float32)
+    x0[:3] = rng.uniform(0.25, 0.75, (3, n))
+    d = rng.normal(size=(3, n))
+    d /= np.linalg.norm(d, axis=0, keepdims=True)
+    goal = (x0[:3] + d * rng.uniform(0.05, 0.15, n)).astype(np.float32)
+    hover = -cfg.gravity * cfg.quad_mass
+    u = np.zeros((T, 4, n), np.float32)
+    u[:, 0] = np.clip(hover + rng.normal(0, 1.0, (T, n)), cfg.ctrl_min[0], cfg.ctrl_max[0])
+    u[:, 1:] = np.clip(rng.normal(0, 0.005, (T, 3, n)), cfg.ctrl_min[1], cfg.ctrl_max[1])
+    obs = random_boxes(rng, 50, 0.02, 0.08)
+    return RefineBatch(cfg, refine_params(cfg), steps, x0, goal, u, obs)
+
+
 def pack_batch(queries):
     """Arrays for kgmt_batch_plan: configs, starts/goals (n, 17), obstacle
     rows, per-query offsets and counts."""

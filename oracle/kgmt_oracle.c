@@ -1007,3 +1007,233 @@ void orc_region_n(const kgmt_config* cfg, const float* pos /* n x 3 */, uint32_t
             r2[i] = orc_subregion(cfg, pos + 3 * (size_t)i, r1[i]);
         }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Trajectory cost and adjoint (FP64). Not part of the reference planner:    */
+/* the refinement extension differentiates the reference's quadcopter RK4    */
+/* rollout (statePropagator.cu:246-370). VJP of ode() derived by hand:       */
+/* position rates R(phi,theta,psi) V, Euler-angle rates, -w x V + R^T g +    */
+/* drag + thrust, Euler's equations with drag and torques.                   */
+/* ------------------------------------------------------------------------ */
+void orc_quad_vjp_f64(const kgmt_config* cfg, const double* s, const double* c, const double* a, double* gs,
+                      double* gc)
+{
+    double phi = s[3], th = s[4], psi = s[5];
+    double u = s[6], v = s[7], w = s[8], p = s[9], q = s[10], r = s[11];
+    double sf = sin(phi), cf = cos(phi), st = sin(th), ct = cos(th), sp = sin(psi), cp = cos(psi);
+    double tt = st / ct;
+    double g = cfg->gravity, mi = 1.0 / cfg->quad_mass;
+    double ix = cfg->quad_ix, iy = cfg->quad_iy, iz = cfg->quad_iz;
+    double ax = (iy - iz) / ix, ay = (iz - ix) / iy, az = (ix - iy) / iz;
+    double bx = 1.0 / ix, by = 1.0 / iy, bz = 1.0 / iz;
+    (void)c;
+    for(int i = 0; i < 12; ++i) gs[i] = 0.0;
+    /* rotation rows */
+    double R00 = ct * cp, R01 = sf * st * cp - cf * sp, R02 = cf * st * cp + sf * sp;
+    double R10 = ct * sp, R11 = sf * st * sp + cf * cp, R12 = cf * st * sp - sf * cp;
+    double R20 = -st, R21 = sf * ct, R22 = cf * ct;
+    double a0 = a[0], a1 = a[1], a2 = a[2];
+    gs[6] += R00 * a0 + R10 * a1 + R20 * a2;
+    gs[7] += R01 * a0 + R11 * a1 + R21 * a2;
+    gs[8] += R02 * a0 + R12 * a1 + R22 * a2;
+    /* d/dphi, d/dtheta, d/dpsi of R V */
+    double d0f = (cf * st * cp + sf * sp) * v + (-sf * st * cp + cf * sp) * w;
+    double d1f = (cf * st * sp - sf * cp) * v + (-sf * st * sp - cf * cp) * w;
+    double d2f = cf * ct * v - sf * ct * w;
+    double d0t = -st * cp * u + sf * ct * cp * v + cf * ct * cp * w;
+    double d1t = -st * sp * u + sf * ct * sp * v + cf * ct * sp * w;
+    double d2t = -ct * u - sf * st * v - cf * st * w;
+    double f0 = R00 * u + R01 * v + R02 * w, f1 = R10 * u + R11 * v + R12 * w;
+    gs[3] += d0f * a0 + d1f * a1 + d2f * a2;
+    gs[4] += d0t * a0 + d1t * a1 + d2t * a2;
+    gs[5] += -f1 * a0 + f0 * a1;
+    /* Euler-angle rates */
+    double E = q * sf + r * cf, F = q * cf - r * sf;
+    double a3 = a[3], a4 = a[4], a5 = a[5];
+    gs[3] += F * tt * a3 - E * a4 + F / ct * a5;
+    gs[4] += E / (ct * ct) * a3 + E * st / (ct * ct) * a5;
+    gs[9] += a3;
+    gs[10] += sf * tt * a3 + cf * a4 + sf / ct * a5;
+    gs[11] += cf * tt * a3 - sf * a4 + cf / ct * a5;
+    /* translational dynamics */
+    double a6 = a[6], a7 = a[7], a8 = a[8];
+    double nv = sqrt(u * u + v * v + w * w), N = -cfg->quad_nu * nv;
+    gs[7] += r * a6 - p * a8;
+    gs[8] += -q * a6 + p * a7;
+    gs[6] += -r * a7 + q * a8;
+    gs[9] += w * a7 - v * a8;
+    gs[10] += -w * a6 + u * a8;
+    gs[11] += v * a6 - u * a7;
+    gs[4] += -g * ct * a6 - g * st * sf * a7 - g * st * cf * a8;
+    gs[3] += g * ct * cf * a7 - g * ct * sf * a8;
+    double dotV = u * a6 + v * a7 + w * a8;
+    gs[6] += mi * N * a6;
+    gs[7] += mi * N * a7;
+    gs[8] += mi * N * a8;
+    if(nv > 0.0)
+        {
+            double k = mi * (-cfg->quad_nu) / nv * dotV;
+            gs[6] += k * u, gs[7] += k * v, gs[8] += k * w;
+        }
+    /* rotational dynamics */
+    double a9 = a[9], a10 = a[10], a11 = a[11];
+    double nw = sqrt(p * p + q * q + r * r), M = -cfg->quad_mu * nw;
+    gs[10] += ax * r * a9;
+    gs[11] += ax * q * a9;
+    gs[9] += ay * r * a10;
+    gs[11] += ay * p * a10;
+    gs[9] += az * q * a11;
+    gs[10] += az * p * a11;
+    double b9 = bx * a9, b10 = by * a10, b11 = bz * a11;
+    gs[9] += M * b9, gs[10] += M * b10, gs[11] += M * b11;
+    if(nw > 0.0)
+        {
+            double k = (-cfg->quad_mu) / nw * (p * b9 + q * b10 + r * b11);
+            gs[9] += k * p, gs[10] += k * q, gs[11] += k * r;
+        }
+    gc[0] = mi * a8;
+    gc[1] = bx * a9;
+    gc[2] = by * a10;
+    gc[3] = bz * a11;
+}
+
+static void rk4_f64(const kgmt_config* cfg, const double* x, const double* u, double* out)
+{
+    double h = (double)cfg->dt;
+    double k1[12], k2[12], k3[12], k4[12], st[12];
+    orc_quad_ode_f64(cfg, x, u, k1);
+    for(int j = 0; j < 12; ++j) st[j] = x[j] + 0.5 * h * k1[j];
+    orc_quad_ode_f64(cfg, st, u, k2);
+    for(int j = 0; j < 12; ++j) st[j] = x[j] + 0.5 * h * k2[j];
+    orc_quad_ode_f64(cfg, st, u, k3);
+    for(int j = 0; j < 12; ++j) st[j] = x[j] + h * k3[j];
+    orc_quad_ode_f64(cfg, st, u, k4);
+    for(int j = 0; j < 12; ++j) out[j] = x[j] + h / 6.0 * (k1[j] + 2.0 * k2[j] + 2.0 * k3[j] + k4[j]);
+}
+
+/* x-bar += J_Phi^T lam, u-bar += J_Phi,u^T lam for one RK4 step from x */
+static void rk4_vjp_f64(const kgmt_config* cfg, const double* x, const double* u, const double* lam, double* xbar,
+                        double* ubar)
+{
+    double h = (double)cfg->dt;
+    double k1[12], k2[12], k3[12], s2[12], s3[12], s4[12];
+    orc_quad_ode_f64(cfg, x, u, k1);
+    for(int j = 0; j < 12; ++j) s2[j] = x[j] + 0.5 * h * k1[j];
+    orc_quad_ode_f64(cfg, s2, u, k2);
+    for(int j = 0; j < 12; ++j) s3[j] = x[j] + 0.5 * h * k2[j];
+    orc_quad_ode_f64(cfg, s3, u, k3);
+    for(int j = 0; j < 12; ++j) s4[j] = x[j] + h * k3[j];
+    double kb1[12], kb2[12], kb3[12], kb4[12], gs[12], gc[4];
+    for(int j = 0; j < 12; ++j)
+        {
+            xbar[j] += lam[j];
+            kb1[j] = h / 6.0 * lam[j], kb2[j] = h / 3.0 * lam[j], kb3[j] = h / 3.0 * lam[j], kb4[j] = h / 6.0 * lam[j];
+        }
+    orc_quad_vjp_f64(cfg, s4, u, kb4, gs, gc);
+    for(int j = 0; j < 12; ++j) xbar[j] += gs[j], kb3[j] += h * gs[j];
+    for(int i = 0; i < 4; ++i) ubar[i] += gc[i];
+    orc_quad_vjp_f64(cfg, s3, u, kb3, gs, gc);
+    for(int j = 0; j < 12; ++j) xbar[j] += gs[j], kb2[j] += 0.5 * h * gs[j];
+    for(int i = 0; i < 4; ++i) ubar[i] += gc[i];
+    orc_quad_vjp_f64(cfg, s2, u, kb2, gs, gc);
+    for(int j = 0; j < 12; ++j) xbar[j] += gs[j], kb1[j] += 0.5 * h * gs[j];
+    for(int i = 0; i < 4; ++i) ubar[i] += gc[i];
+    orc_quad_vjp_f64(cfg, x, u, kb1, gs, gc);
+    for(int j = 0; j < 12; ++j) xbar[j] += gs[j];
+    for(int i = 0; i < 4; ++i) ubar[i] += gc[i];
+}
+
+/* tau * softplus((margin - sd(p, box)) / tau) summed over boxes, and its
+ * gradient with respect to p (added into gp when not NULL) */
+static double penalty_f64(const kgmt_adjoint_params* ap, const double* p, const double* obs, uint32_t n_obs, double scale,
+                          double* gp)
+{
+    double total = 0.0;
+    for(uint32_t o = 0; o < n_obs; ++o)
+        {
+            const double* b = obs + 6 * (size_t)o;
+            double qv[3], sgn[3];
+            for(int d = 0; d < 3; ++d)
+                {
+                    double lo = b[d] - p[d], hi = p[d] - b[3 + d];
+                    if(lo > hi)
+                        qv[d] = lo, sgn[d] = -1.0;
+                    else
+                        qv[d] = hi, sgn[d] = 1.0;
+                }
+            double ox = qv[0] > 0 ? qv[0] : 0, oy = qv[1] > 0 ? qv[1] : 0, oz = qv[2] > 0 ? qv[2] : 0;
+            double outside = sqrt(ox * ox + oy * oy + oz * oz);
+            int am         = qv[0] >= qv[1] ? (qv[0] >= qv[2] ? 0 : 2) : (qv[1] >= qv[2] ? 1 : 2);
+            double inside  = qv[am] < 0 ? qv[am] : 0;
+            double sd      = outside + inside;
+            double z       = ((double)ap->margin - sd) / (double)ap->tau;
+            double sp      = z > 30.0 ? z : log1p(exp(z));
+            total += (double)ap->tau * sp;
+            if(gp)
+                {
+                    double sig  = 1.0 / (1.0 + exp(-z));
+                    double dsd  = -sig * scale; /* d(tau softplus)/d sd = -sigmoid(z) */
+                    if(outside > 0)
+                        {
+                            gp[0] += dsd * (ox / outside) * sgn[0];
+                            gp[1] += dsd * (oy / outside) * sgn[1];
+                            gp[2] += dsd * (oz / outside) * sgn[2];
+                        }
+                    else
+                        gp[am] += dsd * sgn[am];
+                }
+        }
+    return total;
+}
+
+double orc_traj_cost_f64(const kgmt_config* cfg, const kgmt_adjoint_params* ap, int32_t T, const double* x0,
+                         const double* u, const double* goal, const double* obs, uint32_t n_obs, double* xT)
+{
+    double x[12], y[12], J = 0.0;
+    memcpy(x, x0, sizeof(x));
+    for(int32_t t = 0; t < T; ++t)
+        {
+            const double* ut = u + 4 * (size_t)t;
+            for(int i = 0; i < 4; ++i)
+                {
+                    double d = ut[i] - (double)ap->u_ref[i];
+                    J += (double)ap->w_effort * d * d;
+                }
+            rk4_f64(cfg, x, ut, y);
+            memcpy(x, y, sizeof(x));
+            J += (double)ap->w_obstacle * penalty_f64(ap, x, obs, n_obs, 1.0, NULL);
+        }
+    for(int d = 0; d < 3; ++d)
+        {
+            double e = x[d] - goal[d];
+            J += (double)ap->w_goal * e * e;
+        }
+    if(xT) memcpy(xT, x, sizeof(x));
+    return J;
+}
+
+double orc_traj_grad_f64(const kgmt_config* cfg, const kgmt_adjoint_params* ap, int32_t T, const double* x0,
+                         const double* u, const double* goal, const double* obs, uint32_t n_obs, double* grad)
+{
+    double* xs = (double*)malloc(sizeof(double) * 12 * ((size_t)T + 1));
+    if(!xs) return NAN;
+    memcpy(xs, x0, sizeof(double) * 12);
+    for(int32_t t = 0; t < T; ++t) rk4_f64(cfg, xs + 12 * (size_t)t, u + 4 * (size_t)t, xs + 12 * ((size_t)t + 1));
+    double J = orc_traj_cost_f64(cfg, ap, T, x0, u, goal, obs, n_obs, NULL);
+    double lam[12] = {0};
+    const double* xT = xs + 12 * (size_t)T;
+    for(int d = 0; d < 3; ++d) lam[d] = 2.0 * (double)ap->w_goal * (xT[d] - goal[d]);
+    if(T >= 1) penalty_f64(ap, xT, obs, n_obs, (double)ap->w_obstacle, lam);
+    for(int32_t t = T - 1; t >= 0; --t)
+        {
+            const double* xt = xs + 12 * (size_t)t;
+            const double* ut = u + 4 * (size_t)t;
+            double xbar[12] = {0}, ubar[4] = {0};
+            rk4_vjp_f64(cfg, xt, ut, lam, xbar, ubar);
+            for(int i = 0; i < 4; ++i) grad[4 * (size_t)t + i] = ubar[i] + 2.0 * (double)ap->w_effort * (ut[i] - (double)ap->u_ref[i]);
+            if(t >= 1) penalty_f64(ap, xt, obs, n_obs, (double)ap->w_obstacle, xbar);
+            memcpy(lam, xbar, sizeof(lam));
+        }
+    free(xs);
+    return J;
+}

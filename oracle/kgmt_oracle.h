@@ -77,6 +77,16 @@ int orc_plan_many(const kgmt_config* cfgs, uint32_t n_q, const float* starts, co
                   const float* obs, const uint32_t* obs_off, const uint32_t* obs_cnt, int threads,
                   orc_stats* st_out, double* ms_out);
 
+/* trajectory cost and its gradient (FP64): the quadcopter ODE's
+ * vector-Jacobian product, the cost of one control sequence and its
+ * gradient by the discrete adjoint of the RK4 rollout. u: T x 4 row-major. */
+void orc_quad_vjp_f64(const kgmt_config* cfg, const double* s, const double* c, const double* a, double* gs,
+                      double* gc);
+double orc_traj_cost_f64(const kgmt_config* cfg, const kgmt_adjoint_params* ap, int32_t T, const double* x0,
+                         const double* u, const double* goal, const double* obs, uint32_t n_obs, double* xT);
+double orc_traj_grad_f64(const kgmt_config* cfg, const kgmt_adjoint_params* ap, int32_t T, const double* x0,
+                         const double* u, const double* goal, const double* obs, uint32_t n_obs, double* grad);
+
 /* batch conveniences (SoA layouts of the stage-level C-ABI) */
 void orc_sincosf_n(const float* x, uint32_t n, float* s, float* c);
 void orc_philox_n(const uint32_t* ctr, const uint32_t* key, uint32_t n, uint32_t* out);

@@ -11,7 +11,7 @@ import subprocess
 
 import numpy as np
 
-from kgmt_amd.abi import KGMT_MAX_SAMPLE, KgmtConfig, dims
+from kgmt_amd.abi import KGMT_MAX_SAMPLE, KgmtAdjointParams, KgmtConfig, dims
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libkgmt_oracle.so")
@@ -59,6 +59,9 @@ _SIG = {
     "orc_propagate_children_f64": (C.c_int, [_cfgp, _P, _P, _P, _u32, _P, _u32, _P, _P]),
     "orc_segment_valid_n": (None, [_P, _P, _u32, _P, _u32, _P]),
     "orc_region_n": (None, [_cfgp, _P, _u32, _P, _P]),
+    "orc_quad_vjp_f64": (None, [_cfgp, _P, _P, _P, _P, _P]),
+    "orc_traj_cost_f64": (C.c_double, [_cfgp, C.POINTER(KgmtAdjointParams), C.c_int32, _P, _P, _P, _P, _u32, _P]),
+    "orc_traj_grad_f64": (C.c_double, [_cfgp, C.POINTER(KgmtAdjointParams), C.c_int32, _P, _P, _P, _P, _u32, _P]),
 }
 
 _lib = None
@@ -262,3 +265,32 @@ def plan_many(cfgs, starts, goals, obstacles, offsets, counts, threads: int = 0)
                              _p(ms))
     assert rc == 0, rc
     return [stats[i].as_dict() for i in range(n)], ms
+
+
+# ---- trajectory refinement (FP64) -----------------------------------------
+def quad_vjp_f64(cfg, s, c, a):
+    s, c, a = (_c(v, np.float64) for v in (s, c, a))
+    gs = np.zeros(12)
+    gc = np.zeros(4)
+    lib().orc_quad_vjp_f64(C.byref(cfg), _p(s), _p(c), _p(a), _p(gs), _p(gc))
+    return gs, gc
+
+
+def traj_cost_f64(cfg, params, x0, u, goal, obs):
+    """u: (T, 4) controls; returns (J, x_T)."""
+    x0, u, goal = _c(x0, np.float64), _c(u, np.float64), _c(goal, np.float64)
+    ob = _c(obs, np.float64).reshape(-1)
+    xT = np.zeros(12)
+    J = lib().orc_traj_cost_f64(C.byref(cfg), C.byref(params), len(u), _p(x0), _p(u), _p(goal),
+                                _p(ob) if ob.size else None, ob.size // 6, _p(xT))
+    return J, xT
+
+
+def traj_grad_f64(cfg, params, x0, u, goal, obs):
+    """Cost and its gradient (T, 4) by the discrete adjoint, in FP64."""
+    x0, u, goal = _c(x0, np.float64), _c(u, np.float64), _c(goal, np.float64)
+    ob = _c(obs, np.float64).reshape(-1)
+    g = np.zeros_like(u)
+    J = lib().orc_traj_grad_f64(C.byref(cfg), C.byref(params), len(u), _p(x0), _p(u), _p(goal),
+                                _p(ob) if ob.size else None, ob.size // 6, _p(g))
+    return J, g

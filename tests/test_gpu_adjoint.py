@@ -1,0 +1,141 @@
+"""GPU parity of the differentiable quadcopter rollout (kgmt_rk4_adjoint,
+kgmt_refine) with the oracle's FP64 cost and discrete adjoint, which the CPU
+tests pin against central finite differences (h = 1e-6)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from kgmt_amd import adjoint as A
+from kgmt_amd import workloads as W
+from kgmt_amd._lib import KgmtError
+
+pytestmark = pytest.mark.gpu
+
+# FP32 rollout vs FP64 adjoint: cost within 1e-4 relative; gradient elements
+# within 1e-4 of the trajectory's largest gradient component (and of 1.0)
+COST_RTOL = 1e-4
+GRAD_TOL = 1e-4
+
+
+def _dev(rb):
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+    return d(rb.u), d(rb.x0), d(rb.goal), d(rb.steps), d(rb.obstacles)
+
+
+def _near_boxes(orc, rb, k):
+    """Small boxes at the midpoints of the first k rollouts, so that the
+    obstacle penalty is active on those trajectories."""
+    extra = []
+    for i in range(k):
+        T = int(rb.steps[i])
+        _, xT = orc.traj_cost_f64(rb.cfg, rb.params, rb.x0[:, i], rb.u[:T, :, i], rb.goal[:, i], rb.obstacles)
+        mid = (rb.x0[:3, i] + xT[:3]) / 2
+        extra.append([*(mid - 0.01), *(mid + 0.01)])
+    return np.concatenate([rb.obstacles, np.array(extra, np.float32)]).astype(np.float32)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_cost_and_gradient_match_fp64_adjoint(seed, orc):
+    rb = W.make_refine_batch(96, seed=seed)
+    rb.obstacles = _near_boxes(orc, rb, 16)
+    u, x0, goal, steps, obs = _dev(rb)
+    print(f"adjoint parity seed={seed} n=96 T<={u.shape[0]} boxes={len(rb.obstacles)}", flush=True)
+    cost, grad = A.cost_and_grad(u, x0, goal, steps, obs, rb.cfg, rb.params)
+    torch.cuda.synchronize()
+    cost, grad = cost.cpu().numpy(), grad.cpu().numpy()
+    worst_c = worst_g = 0.0
+    for i in range(96):
+        T = int(rb.steps[i])
+        J, g = orc.traj_grad_f64(rb.cfg, rb.params, rb.x0[:, i], rb.u[:T, :, i], rb.goal[:, i], rb.obstacles)
+        ec = abs(cost[i] - J) / max(1.0, abs(J))
+        scale = max(1.0, np.abs(g).max())
+        eg = np.abs(grad[:T, :, i] - g).max() / scale
+        worst_c, worst_g = max(worst_c, ec), max(worst_g, eg)
+        assert ec <= COST_RTOL, (i, cost[i], J)
+        assert eg <= GRAD_TOL, (i, eg)
+        assert np.all(grad[T:, :, i] == 0.0)
+    print(f"worst cost rel err {worst_c:.3e}, worst grad err {worst_g:.3e}", flush=True)
+
+
+def test_autograd_backward_scales_gradient(orc):
+    rb = W.make_refine_batch(64, seed=3)
+    u, x0, goal, steps, obs = _dev(rb)
+    c0, g0 = A.cost_and_grad(u, x0, goal, steps, obs, rb.cfg, rb.params)
+    uu = u.clone().requires_grad_(True)
+    cost = A.rollout_cost(uu, x0, goal, steps, obs, rb.cfg, rb.params)
+    w = torch.linspace(0.5, 2.0, 64, device="cuda:0")
+    (w * cost).sum().backward()
+    torch.cuda.synchronize()
+    assert torch.equal(cost.detach(), c0)
+    assert torch.equal(uu.grad, g0 * w.view(1, 1, -1))
+    # the gradient is a descent direction: a small step along -g lowers J
+    c1 = A.cost_and_grad(u - 1e-3 * g0 / g0.abs().amax(dim=(0, 1), keepdim=True).clamp_min(1e-12), x0, goal,
+                         steps, obs, rb.cfg, rb.params)[0]
+    assert bool((c1 <= c0).all())
+
+
+def test_refine_lowers_mean_cost_by_15_percent():
+    rb = W.make_refine_batch(4096, seed=0)
+    u, x0, goal, steps, obs = _dev(rb)
+    print(f"refine n=4096 iterations={rb.params.iterations}", flush=True)
+    hist = A.refine(u, x0, goal, steps, obs, rb.cfg, rb.params)
+    c_final = A.cost_and_grad(u, x0, goal, steps, obs, rb.cfg, rb.params)[0]
+    torch.cuda.synchronize()
+    h = hist.cpu().numpy()
+    assert h.shape == (rb.params.iterations + 1, 4096)
+    m0, m1 = float(h[0].mean()), float(h[-1].mean())
+    print(f"mean cost {m0:.4f} -> {m1:.4f} ({100 * (1 - m1 / m0):.1f}% lower)", flush=True)
+    assert m1 <= 0.85 * m0
+    # the history's last row is the cost of the refined controls
+    np.testing.assert_allclose(h[-1], c_final.cpu().numpy(), rtol=1e-6, atol=1e-7)
+    # controls stay inside the configured bounds
+    uc = u.cpu().numpy()
+    for c in range(4):
+        assert uc[:, c].min() >= rb.cfg.ctrl_min[c] and uc[:, c].max() <= rb.cfg.ctrl_max[c]
+
+
+def test_refine_follows_fp64_adam(orc):
+    """The first Adam steps of the GPU refinement against numpy Adam driven
+    by the oracle's FP64 gradient."""
+    rb = W.make_refine_batch(8, seed=5)
+    p = W.refine_params(rb.cfg, iterations=5)
+    u, x0, goal, steps, obs = _dev(rb)
+    hist = A.refine(u, x0, goal, steps, obs, rb.cfg, p).cpu().numpy()
+    uc = u.cpu().numpy()
+    for i in range(8):
+        T = int(rb.steps[i])
+        ui = rb.u[:T, :, i].astype(np.float64)
+        m = np.zeros_like(ui)
+        v = np.zeros_like(ui)
+        costs = []
+        for it in range(1, p.iterations + 1):
+            J, g = orc.traj_grad_f64(rb.cfg, p, rb.x0[:, i], ui, rb.goal[:, i], rb.obstacles)
+            costs.append(J)
+            m = p.beta1 * m + (1 - p.beta1) * g
+            v = p.beta2 * v + (1 - p.beta2) * g * g
+            ui = ui - p.lr * (m / (1 - p.beta1 ** it)) / (np.sqrt(v / (1 - p.beta2 ** it)) + p.adam_eps)
+            ui = np.clip(ui, np.array(rb.cfg.ctrl_min[:4]), np.array(rb.cfg.ctrl_max[:4]))
+        costs.append(orc.traj_cost_f64(rb.cfg, p, rb.x0[:, i], ui, rb.goal[:, i], rb.obstacles)[0])
+        np.testing.assert_allclose(hist[:, i], costs, rtol=1e-3)
+        # Adam moves every control by at most lr per step; tiny gradients may
+        # flip sign between FP32 and FP64, so compare to within that bound
+        assert np.abs(uc[:T, :, i] - ui).max() <= p.lr * p.iterations + 1e-6
+
+
+def test_argument_errors():
+    rb = W.make_refine_batch(4, seed=0)
+    u, x0, goal, steps, obs = _dev(rb)
+    with pytest.raises(ValueError):
+        A.rollout_cost(u, x0, goal, steps, obs, W.config("di6d"), rb.params)
+    with pytest.raises(ValueError):
+        A.cost_and_grad(u[:, :3], x0, goal, steps, obs, rb.cfg, rb.params)
+    many = torch.zeros((1501, 6), device="cuda:0")
+    with pytest.raises(KgmtError) as e:
+        A.cost_and_grad(u, x0, goal, steps, many, rb.cfg, rb.params)
+    assert e.value.code == -4
+    # n = 0 is a no-op
+    e0 = torch.zeros((u.shape[0], 4, 0), device="cuda:0")
+    c, g = A.cost_and_grad(e0, x0[:, :0], goal[:, :0], steps[:0], obs, rb.cfg, rb.params)
+    assert c.numel() == 0 and g.numel() == 0
