@@ -47,7 +47,7 @@ def bytes_per_child(model: int) -> int:
     return 4 + 4 * S + 4 * D + 4 + 4 + 1
 
 
-def run_plain(name, steps, warmup, profile=True):
+def run_plain(name, steps, warmup, profile=True, launch=None):
     import torch
     from kgmt_amd import workloads as W
     from kgmt_amd.planner import KGMT
@@ -55,6 +55,8 @@ def run_plain(name, steps, warmup, profile=True):
     qs = [W.make_query(name, s) for s in range(warmup + steps)]
     d_obs = [torch.from_numpy(q.obstacles.reshape(-1)).cuda() for q in qs]
     g = KGMT(qs[0].cfg)
+    if launch:
+        g.set_launch(**launch)
     for q, d in zip(qs[:warmup], d_obs[:warmup]):
         g.set_seed(q.cfg.seed)
         g.plan(q.start, q.goal, int(d.data_ptr()), len(q.obstacles))
@@ -85,15 +87,23 @@ def run_cpu(name, n_queries, threads):
     return stats, ms, dt
 
 
-def run_batch(n, seed=0):
+def run_batch(n, seed=0, tune=True):
     import numpy as np
     import torch
+    from kgmt_amd import autotune
     from kgmt_amd import workloads as W
     from kgmt_amd.planner import KGMT, BatchKGMT
 
     qs = W.make_batch(n, seed=seed)
     cfgs, starts, goals, obs, offs, cnts = W.pack_batch(qs)
+    tuned = None
+    if tune:
+        # launch shape chosen on a calibration batch of the same mix (other seed)
+        cq = W.make_batch(n, seed=seed + 1000)
+        tuned = autotune.tune_batch(*W.pack_batch(cq), use_graph=True, reps=2, cache=None)
     b = BatchKGMT(cfgs)
+    if tuned:
+        b.set_launch(**tuned["launch"])
     b.plan(starts, goals, obs, offs, cnts, use_graph=True)  # warm-up: graph capture
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -130,7 +140,55 @@ def run_batch(n, seed=0):
         "success_rate_plain": succ_p,
         "identical_results": same,
         "hip_graph": True,
+        "launch": tuned["launch"] if tuned else "default",
+        "autotune": {"candidates": len(tuned["trials"]), "calibration": "same mix, seed + 1000",
+                     "best_s": round(tuned["seconds"], 5),
+                     "default_s": round(tuned["trials"][0]["seconds"], 5)} if tuned else None,
     }
+
+
+def run_refine(n=4096, seed=0, cpu_sample=0):
+    """Adam refinement of n quadcopter control sequences (T ~ U{20..80},
+    dt 0.01, 50 steps, lr 1e-2) through kgmt_refine; the CPU figure is the
+    oracle's FP64 cost+adjoint on cpu_sample trajectories, scaled to the
+    50 steps, one thread."""
+    import numpy as np
+    import torch
+    from kgmt_amd import adjoint as A
+    from kgmt_amd import workloads as W
+
+    rb = W.make_refine_batch(n, seed=seed)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+    u0, x0, goal, steps, obs = d(rb.u), d(rb.x0), d(rb.goal), d(rb.steps), d(rb.obstacles)
+    u = u0.clone()
+    A.refine(u, x0, goal, steps, obs, rb.cfg, rb.params)  # warm-up
+    u = u0.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = A.refine(u, x0, goal, steps, obs, rb.cfg, rb.params)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    h = hist.cpu().numpy()
+    out = {
+        "trajectories": n,
+        "adam_steps": int(rb.params.iterations),
+        "ms": round(dt * 1e3, 3),
+        "trajectories_per_s": round(n / dt, 1),
+        "mean_cost_before": round(float(h[0].mean()), 5),
+        "mean_cost_after": round(float(h[-1].mean()), 5),
+        "mean_cost_reduction": round(1.0 - float(h[-1].mean()) / float(h[0].mean()), 4),
+        "dtype": "f32",
+    }
+    if cpu_sample:
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        for i in range(cpu_sample):
+            T = int(rb.steps[i])
+            O.traj_grad_f64(rb.cfg, rb.params, rb.x0[:, i], rb.u[:T, :, i], rb.goal[:, i], rb.obstacles)
+        per = (time.perf_counter() - t0) / cpu_sample * rb.params.iterations
+        out["cpu_baseline"] = {"value": round(1.0 / per, 2), "unit": "trajectories/s", "cores": 1, "kind": "port",
+                               "sample": f"FP64 oracle cost+adjoint of {cpu_sample} trajectories, x50 steps"}
+    return out
 
 
 def main(argv=None):
@@ -141,6 +199,8 @@ def main(argv=None):
     ap.add_argument("--workload", default="dubins5d", choices=sorted(WORKLOAD_DESC))
     ap.add_argument("--batch", type=int, default=512, help="mixed batch size (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes (no autotuning)")
+    ap.add_argument("--refine", type=int, default=4096, help="trajectories of the adjoint refinement (0: skip)")
     ap.add_argument("--cpu-queries", type=int, default=0, help="CPU sample size (default: one per thread)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 256 MiB three times (scales the FETCH_SIZE/WRITE_SIZE counters)")
@@ -168,7 +228,13 @@ def main(argv=None):
         print(json.dumps({"error": "no GPU visible"}), flush=True)
         return 1
 
-    cfg, res, dt, prof = run_plain(a.workload, a.steps, a.warmup)
+    plain_launch = None
+    if not a.no_tune:
+        from kgmt_amd import autotune
+        from kgmt_amd import workloads as W
+        tq = [W.make_query(a.workload, s) for s in range(1000, 1004)]
+        plain_launch = autotune.tune_plain(tq, reps=1, cache=None)["launch"]
+    cfg, res, dt, prof = run_plain(a.workload, a.steps, a.warmup, launch=plain_launch)
     tts = [r["time_ms"] for r in res]
     succ = [r for r in res if r["success"]]
     qps = len(res) / dt
@@ -208,7 +274,8 @@ def main(argv=None):
                       f"median {statistics.median(ms):.0f} ms/query single-thread",
         }
 
-    batch = run_batch(a.batch) if a.batch else None
+    batch = run_batch(a.batch, tune=not a.no_tune) if a.batch else None
+    refine = run_refine(a.refine, cpu_sample=0 if a.no_cpu else 64) if a.refine else None
 
     line = {
         "metric": "planning queries/sec + median time-to-solution, batched vs plain, 1xMI355X",
@@ -229,6 +296,7 @@ def main(argv=None):
             "boxes": 100 if a.workload == "dubins5d" else None,
             "max_iter": cfg.max_iter,
             "parallelism": "single GPU",
+            "launch": plain_launch or "default",
         },
         "median_tts_ms": round(statistics.median(tts), 4),
         "success_rate": len(succ) / len(res),
@@ -237,6 +305,7 @@ def main(argv=None):
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batch,
+        "refine": refine,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -40,6 +40,7 @@ SIGNATURES = {
     "kgmt_destroy": (C.c_int, [_P]),
     "kgmt_set_seed": (C.c_int, [_P, C.c_uint64]),
     "kgmt_set_graph": (C.c_int, [_P, _i32]),
+    "kgmt_set_launch": (C.c_int, [_P, _u32, _u32, _u32]),
     "kgmt_set_stream": (C.c_int, [_P, _P]),
     "kgmt_plan": (C.c_int, [_P, _P, _P, _P, _u32, _i32, _resp]),
     "kgmt_get_tree": (C.c_int, [_P, _P, _P, _P, _u32, C.POINTER(_u32)]),

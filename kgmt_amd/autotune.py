@@ -1,0 +1,164 @@
+"""Launch-shape autotuner of the planner's persistent kernels.
+
+The tunables are the propagate tile (children per tile = threads per
+workgroup: 128, 256 or 512), the number of persistent propagate workgroups
+and the number of append/expand workgroups (multiples of the device's compute
+units). None of them changes a result: every candidate's outcomes are checked
+against the first candidate's, query by query, and a mismatch is an error.
+
+A search times every candidate on a calibration workload (plans run as the
+caller will run them: batched with or without a hipGraph, or plain), keeps
+the fastest by the median of `reps` timed runs, and records it in a JSON
+cache keyed by device and workload signature, so later processes reuse it.
+
+    best = tune_batch(cfgs, starts, goals, obstacles, offsets, counts)
+    batch.set_launch(**best["launch"])
+"""
+from __future__ import annotations
+
+import hashlib
+import itertools
+import json
+import os
+import statistics
+import time
+from typing import Callable, Iterable
+
+TILES = (128, 256, 512)
+PROP_PER_CU = (2, 4, 8)
+CHUNK_PER_CU = (1, 2, 4)
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "kgmt_amd", "launch.json")
+
+
+def candidates(cus: int, tiles: Iterable[int] = TILES, prop_per_cu: Iterable[int] = PROP_PER_CU,
+               chunk_per_cu: Iterable[int] = CHUNK_PER_CU) -> list[dict]:
+    """Every launch shape of the search, the engine's default (256-child
+    tiles, 4 workgroups per CU for both kernels) first."""
+    out = []
+    for t, p, c in itertools.product(tiles, prop_per_cu, chunk_per_cu):
+        out.append({"propagate_block": int(t), "propagate_grid": int(p * cus), "chunk_grid": int(c * cus)})
+    default = {"propagate_block": 256, "propagate_grid": 4 * cus, "chunk_grid": 4 * cus}
+    out.sort(key=lambda d: d != default)
+    return out
+
+
+def search(cands: list[dict], run: Callable[[dict], tuple[float, object]], reps: int = 3) -> dict:
+    """Times `run(shape) -> (seconds, outcome)` `reps` times per shape after
+    one untimed run, checks that every outcome equals the first shape's and
+    returns {"launch": best shape, "seconds": its median, "trials": [...]}."""
+    if not cands:
+        raise ValueError("no candidates")
+    ref = None
+    trials = []
+    for shape in cands:
+        _, outcome = run(shape)  # warm-up (graph capture, code load)
+        times = []
+        for _ in range(reps):
+            dt, outcome = run(shape)
+            times.append(dt)
+        if ref is None:
+            ref = outcome
+        elif outcome != ref:
+            raise RuntimeError(f"launch shape {shape} changed the planning outcome")
+        trials.append({"launch": shape, "seconds": statistics.median(times)})
+    best = min(trials, key=lambda r: r["seconds"])
+    return {"launch": best["launch"], "seconds": best["seconds"], "trials": trials}
+
+
+def signature(cfgs, counts) -> str:
+    """Workload key: models, capacities, step ranges and obstacle counts."""
+    h = hashlib.sha1()
+    for c, n in zip(cfgs, counts):
+        h.update(f"{c.model},{c.capacity},{c.steps_min},{c.steps_max},{c.max_iter},{int(n)};".encode())
+    return h.hexdigest()[:16]
+
+
+def _device():
+    import torch
+    p = torch.cuda.get_device_properties(0)
+    return f"{p.name}|{p.multi_processor_count}", p.multi_processor_count
+
+
+def load_cache(path: str = DEFAULT_CACHE) -> dict:
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def store_cache(key: str, entry: dict, path: str = DEFAULT_CACHE) -> None:
+    d = load_cache(path)
+    d[key] = entry
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(d, f, indent=1)
+    os.replace(tmp, path)
+
+
+def tune_batch(cfgs, starts, goals, obstacles, offsets, counts, use_graph: bool = True, reps: int = 3,
+               cache: str | None = DEFAULT_CACHE, cands: list[dict] | None = None) -> dict:
+    """Best launch shape of the batched planner on this workload."""
+    import torch
+    from .planner import BatchKGMT
+
+    dev, cus = _device()
+    key = f"batch|{dev}|{signature(cfgs, counts)}|graph={int(use_graph)}"
+    if cache:
+        hit = load_cache(cache).get(key)
+        if hit:
+            return dict(hit, cached=True)
+    b = BatchKGMT(cfgs)
+
+    def run(shape):
+        b.set_launch(**shape)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res, _ = b.plan(starts, goals, obstacles, offsets, counts, use_graph=use_graph)
+        dt = time.perf_counter() - t0
+        return dt, [(r.tree_size, r.iterations, r.goal_index, r.success) for r in res]
+
+    try:
+        out = search(cands or candidates(cus), run, reps)
+    finally:
+        b.close()
+    out["key"] = key
+    if cache:
+        store_cache(key, {k: v for k, v in out.items() if k != "trials"}, cache)
+    return dict(out, cached=False)
+
+
+def tune_plain(queries, reps: int = 3, cache: str | None = DEFAULT_CACHE, cands: list[dict] | None = None,
+               use_graph: bool = False) -> dict:
+    """Best launch shape of the single-query planner over `queries`
+    (workloads.Query objects sharing one configuration), run one by one."""
+    import torch
+    from .planner import KGMT
+
+    dev, cus = _device()
+    key = f"plain|{dev}|{signature([q.cfg for q in queries], [len(q.obstacles) for q in queries])}"
+    if cache:
+        hit = load_cache(cache).get(key)
+        if hit:
+            return dict(hit, cached=True)
+    g = KGMT(queries[0].cfg)
+    g.set_graph(use_graph)
+    d_obs = [torch.from_numpy(q.obstacles.reshape(-1)).cuda() for q in queries]
+
+    def run(shape):
+        g.set_launch(**shape)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        outs = []
+        for q, d in zip(queries, d_obs):
+            g.set_seed(q.cfg.seed)
+            r = g.plan(q.start, q.goal, int(d.data_ptr()), len(q.obstacles))
+            outs.append((r.tree_size, r.iterations, r.goal_index))
+        return time.perf_counter() - t0, outs
+
+    out = search(cands or candidates(cus), run, reps)
+    out["key"] = key
+    if cache:
+        store_cache(key, {k: v for k, v in out.items() if k != "trials"}, cache)
+    return dict(out, cached=False)

@@ -21,7 +21,7 @@ namespace kgmt
 {
 
 constexpr uint32_t kChunk   = 1024;  // elements per bookkeeping chunk (one workgroup job)
-constexpr uint32_t kTile    = 256;   // children per propagate tile
+constexpr uint32_t kTile    = 256;   // default children per propagate tile (128, 256 or 512)
 constexpr uint32_t kBlock   = 256;   // threads of chunk / tile workgroups
 constexpr uint32_t kMaxObs  = 1500;  // obstacles a query may carry
 constexpr uint32_t kGroup   = 8;     // boxes per bounding group of a scene
@@ -113,6 +113,7 @@ struct WorkLists
     uint32_t* counter;     // arrivals of query finishers (last-block pattern)
     uint32_t* allDone;     // host-mapped word: 1 once every query is done
     uint32_t nq;
+    uint32_t tile;         // children per propagate tile (= its workgroup size)
 
     __device__ __forceinline__ const uint32_t* tiles(uint32_t parity) const { return tileStart + parity * (nq + 1); }
     __device__ __forceinline__ const uint32_t* jobs(uint32_t parity) const { return jobStart + parity * (nq + 1); }

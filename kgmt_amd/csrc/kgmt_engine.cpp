@@ -310,7 +310,8 @@ struct Engine
            (rc = dev_alloc(W.jobStart, 2 * (size_t)(n + 1), owned)) || (rc = dev_alloc(W.active, 2, owned)) ||
            (rc = dev_alloc(W.counter, 1, owned)))
             return rc;
-        W.nq = n;
+        W.nq   = n;
+        W.tile = L.propBlock;
         HIP_TRY(hipMemset(W.counter, 0, sizeof(uint32_t)));
         HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         ownStream = true;
@@ -445,7 +446,7 @@ struct Engine
         __atomic_store_n(pinned, 0u, __ATOMIC_RELEASE);
         HIP_TRY(launch_init(dq, ds, P, W, stream));
         if(useGraph && (!graphExec || graphObs != maxPairs || graphLaunch.propGrid != L.propGrid ||
-                        graphLaunch.chunkGrid != L.chunkGrid))
+                        graphLaunch.chunkGrid != L.chunkGrid || graphLaunch.propBlock != L.propBlock))
             {
                 int rc = build_graph();
                 if(rc) return rc;
@@ -670,6 +671,23 @@ int kgmt_set_seed(kgmt_ctx* ctx, uint64_t seed)
     return KGMT_OK;
 }
 
+static int set_launch(Engine& e, uint32_t propagate_block, uint32_t propagate_grid, uint32_t chunk_grid)
+{
+    if(propagate_block && propagate_block != 128 && propagate_block != 256 && propagate_block != 512)
+        return fail(KGMT_E_UNSUPPORTED, "propagate block must be 128, 256 or 512");
+    if(propagate_grid > (1u << 20) || chunk_grid > (1u << 20)) return fail(KGMT_E_ARG, "grid too large");
+    if(propagate_block) e.L.propBlock = propagate_block, e.W.tile = propagate_block;
+    if(propagate_grid) e.L.propGrid = propagate_grid;
+    if(chunk_grid) e.L.chunkGrid = chunk_grid;
+    return KGMT_OK;
+}
+
+int kgmt_set_launch(kgmt_ctx* ctx, uint32_t propagate_block, uint32_t propagate_grid, uint32_t chunk_grid)
+{
+    if(!ctx) return fail(KGMT_E_ARG, "null context");
+    return set_launch(ctx->eng, propagate_block, propagate_grid, chunk_grid);
+}
+
 int kgmt_set_graph(kgmt_ctx* ctx, int32_t on)
 {
     if(!ctx) return fail(KGMT_E_ARG, "null context");
@@ -836,10 +854,7 @@ int kgmt_batch_set_stream(kgmt_batch* b, void* s)
 int kgmt_batch_set_launch(kgmt_batch* b, uint32_t propagate_block, uint32_t propagate_grid, uint32_t chunk_grid)
 {
     if(!b) return fail(KGMT_E_ARG, "null batch");
-    if(propagate_block && propagate_block != kTile) return fail(KGMT_E_UNSUPPORTED, "propagate block must be 256");
-    if(propagate_grid) b->eng.L.propGrid = propagate_grid;
-    if(chunk_grid) b->eng.L.chunkGrid = chunk_grid;
-    return KGMT_OK;
+    return set_launch(b->eng, propagate_block, propagate_grid, chunk_grid);
 }
 
 int kgmt_batch_plan(kgmt_batch* b, const float* starts, const float* goals, const float* obstacles,

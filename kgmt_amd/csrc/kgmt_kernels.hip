@@ -144,7 +144,7 @@ __device__ void build_worklists(const QueryState* sts, const WorkLists& W, uint3
                                                                   __HIP_MEMORY_SCOPE_AGENT);
                             const uint32_t tr = __hip_atomic_load(&sts[q].treeSize, __ATOMIC_RELAXED,
                                                                   __HIP_MEMORY_SCOPE_AGENT);
-                            tiles             = (nc + kTile - 1) / kTile;
+                            tiles             = (nc + W.tile - 1) / W.tile;
                             jobs              = (nc + kChunk - 1) / kChunk + (tr + kChunk - 1) / kChunk;
                             alive             = 1;
                         }
@@ -243,26 +243,28 @@ __global__ __launch_bounds__(kBlock) void k_init(const QueryDesc* __restrict__ q
 // ---------------------------------------------------------------------------
 // propagate: persistent over 256-child tiles
 // ---------------------------------------------------------------------------
-// LDS of one propagate tile: the draws of its 256 children and the order in
+// LDS of one propagate tile: the draws of its NT children and the order in
 // which the lanes take them.
+template <int NT>
 struct TileLds
 {
-    uint32_t hist[kTile];      // children per step bucket, then bucket offsets
-    uint16_t slot[kTile];      // lane -> child of the tile
-    float ctrl[KGMT_MAX_CONTROL][kTile];
-    int32_t steps[kTile];
-    float ufront[kTile];
-    uint32_t sbuf[kTile / 64 + 1];
+    uint32_t hist[NT];      // children per step bucket, then bucket offsets
+    uint16_t slot[NT];      // lane -> child of the tile
+    float ctrl[KGMT_MAX_CONTROL][NT];
+    int32_t steps[NT];
+    float ufront[NT];
+    uint32_t sbuf[NT / 64 + 1];
 };
 
-// One tile of 256 children u = base .. base + 255 of a query. The children
+// One tile of NT children u = base .. base + NT - 1 of a query (NT divides
+// kChunk, so a tile never straddles two chunks). The children
 // are drawn in tile order, then handed to the lanes longest-first (counting
 // sort on the step count) so that each wave's lanes integrate about the same
 // number of steps. Each child still writes only its own slot and the
 // counters are integer atomics, so the outputs do not depend on the order.
-template <int MODEL>
+template <int MODEL, int NT>
 __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState& st, uint32_t base,
-                                           const Pools& P, const float4* sObs, TileLds& L)
+                                           const Pools& P, const float4* sObs, TileLds<NT>& L)
 {
     constexpr int S = ModelDims<MODEL>::S, C = ModelDims<MODEL>::C, D = S + C + 1;
     const uint32_t nC  = st.nChildren;
@@ -275,19 +277,19 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
         float uc[C];
         int32_t steps = 0;
         float uf      = 0.0f;
-        uint32_t bucket = kTile - 1;  // past the end: last
+        uint32_t bucket = NT - 1;  // past the end: last
         if(un < nC)
             {
                 draw_child<C>(qd.cfg, qd.cfg.query_id, st.itr, un, uc, steps, uf);
                 const int32_t k = steps - qd.cfg.steps_min;
-                bucket          = (uint32_t)(kTile - 2) - (uint32_t)(k < (int32_t)kTile - 2 ? k : (int32_t)kTile - 2);
+                bucket          = (uint32_t)(NT - 2) - (uint32_t)(k < (int32_t)NT - 2 ? k : (int32_t)NT - 2);
             }
         L.hist[t] = 0;
         __syncthreads();
         const uint32_t rank = atomicAdd(&L.hist[bucket], 1u);
         __syncthreads();
         uint32_t tot;
-        const uint32_t off = block_excl_scan<kTile>(L.hist[t], L.sbuf, tot);
+        const uint32_t off = block_excl_scan<NT>(L.hist[t], L.sbuf, tot);
         L.hist[t]          = off;
         __syncthreads();
         L.slot[L.hist[bucket] + rank] = (uint16_t)t;
@@ -344,13 +346,14 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
     __syncthreads();  // L is reused by the next tile
 }
 
-template <int MODEL>
-__global__ __launch_bounds__(kTile) void k_propagate(const QueryDesc* __restrict__ qds,
-                                                     const QueryState* __restrict__ sts, Pools P, WorkLists W,
-                                                     uint32_t parity)
+template <int MODEL, int NT>
+__global__ __launch_bounds__(NT) void k_propagate(const QueryDesc* __restrict__ qds,
+                                                  const QueryState* __restrict__ sts, Pools P, WorkLists W,
+                                                  uint32_t parity)
 {
+    static_assert(NT % 64 == 0 && kChunk % NT == 0, "tile size");
     extern __shared__ float4 sObs[];
-    __shared__ TileLds L;
+    __shared__ TileLds<NT> L;
     const uint32_t nq        = W.nq;
     const uint32_t* tileList = W.tiles(parity);
     const uint32_t nTiles    = tileList[nq];
@@ -367,23 +370,23 @@ __global__ __launch_bounds__(kTile) void k_propagate(const QueryDesc* __restrict
                     __syncthreads();
                     staged = q;
                 }
-            const uint32_t base = (t - tileList[q]) * kTile;
+            const uint32_t base = (t - tileList[q]) * NT;
             if constexpr(MODEL >= 0)
-                child_tile<MODEL>(qd, sts[q], base, P, sObs, L);
+                child_tile<MODEL, NT>(qd, sts[q], base, P, sObs, L);
             else
                 {
                     switch(qd.cfg.model)
                         {
                             case KGMT_MODEL_DOUBLE_INTEGRATOR:
-                                child_tile<KGMT_MODEL_DOUBLE_INTEGRATOR>(qd, sts[q], base, P, sObs, L);
+                                child_tile<KGMT_MODEL_DOUBLE_INTEGRATOR, NT>(qd, sts[q], base, P, sObs, L);
                                 break;
                             case KGMT_MODEL_DUBINS_AIRPLANE:
-                                child_tile<KGMT_MODEL_DUBINS_AIRPLANE>(qd, sts[q], base, P, sObs, L);
+                                child_tile<KGMT_MODEL_DUBINS_AIRPLANE, NT>(qd, sts[q], base, P, sObs, L);
                                 break;
                             case KGMT_MODEL_DUBINS_5D:
-                                child_tile<KGMT_MODEL_DUBINS_5D>(qd, sts[q], base, P, sObs, L);
+                                child_tile<KGMT_MODEL_DUBINS_5D, NT>(qd, sts[q], base, P, sObs, L);
                                 break;
-                            default: child_tile<KGMT_MODEL_QUADCOPTER>(qd, sts[q], base, P, sObs, L); break;
+                            default: child_tile<KGMT_MODEL_QUADCOPTER, NT>(qd, sts[q], base, P, sObs, L); break;
                         }
                 }
         }
@@ -861,28 +864,45 @@ hipError_t launch_init(const QueryDesc* qds, QueryState* sts, const Pools& P, co
     return hipGetLastError();
 }
 
-hipError_t launch_propagate(int model, const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
-                            uint32_t parity, uint32_t maxPairs, const Launch& L, hipStream_t s)
+template <int NT>
+static hipError_t launch_propagate_nt(int model, const QueryDesc* qds, QueryState* sts, const Pools& P,
+                                      const WorkLists& W, uint32_t parity, uint32_t maxPairs, const Launch& L,
+                                      hipStream_t s)
 {
     const size_t lds = (size_t)maxPairs * 2 * sizeof(float4);
-    const dim3 g(L.propGrid), b(kTile);
+    const dim3 g(L.propGrid), b(NT);
     switch(model)
         {
             case KGMT_MODEL_DOUBLE_INTEGRATOR:
-                hipLaunchKernelGGL(k_propagate<KGMT_MODEL_DOUBLE_INTEGRATOR>, g, b, lds, s, qds, sts, P, W, parity);
+                hipLaunchKernelGGL((k_propagate<KGMT_MODEL_DOUBLE_INTEGRATOR, NT>), g, b, lds, s, qds, sts, P, W,
+                                   parity);
                 break;
             case KGMT_MODEL_DUBINS_AIRPLANE:
-                hipLaunchKernelGGL(k_propagate<KGMT_MODEL_DUBINS_AIRPLANE>, g, b, lds, s, qds, sts, P, W, parity);
+                hipLaunchKernelGGL((k_propagate<KGMT_MODEL_DUBINS_AIRPLANE, NT>), g, b, lds, s, qds, sts, P, W,
+                                   parity);
                 break;
             case KGMT_MODEL_DUBINS_5D:
-                hipLaunchKernelGGL(k_propagate<KGMT_MODEL_DUBINS_5D>, g, b, lds, s, qds, sts, P, W, parity);
+                hipLaunchKernelGGL((k_propagate<KGMT_MODEL_DUBINS_5D, NT>), g, b, lds, s, qds, sts, P, W, parity);
                 break;
             case KGMT_MODEL_QUADCOPTER:
-                hipLaunchKernelGGL(k_propagate<KGMT_MODEL_QUADCOPTER>, g, b, lds, s, qds, sts, P, W, parity);
+                hipLaunchKernelGGL((k_propagate<KGMT_MODEL_QUADCOPTER, NT>), g, b, lds, s, qds, sts, P, W, parity);
                 break;
-            default: hipLaunchKernelGGL(k_propagate<-1>, g, b, lds, s, qds, sts, P, W, parity); break;
+            default: hipLaunchKernelGGL((k_propagate<-1, NT>), g, b, lds, s, qds, sts, P, W, parity); break;
         }
     return hipGetLastError();
+}
+
+hipError_t launch_propagate(int model, const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
+                            uint32_t parity, uint32_t maxPairs, const Launch& L, hipStream_t s)
+{
+    if(W.tile != L.propBlock) return hipErrorInvalidValue;
+    switch(L.propBlock)
+        {
+            case 128: return launch_propagate_nt<128>(model, qds, sts, P, W, parity, maxPairs, L, s);
+            case 256: return launch_propagate_nt<256>(model, qds, sts, P, W, parity, maxPairs, L, s);
+            case 512: return launch_propagate_nt<512>(model, qds, sts, P, W, parity, maxPairs, L, s);
+            default: return hipErrorInvalidValue;
+        }
 }
 
 hipError_t launch_score(const QueryDesc* qds, QueryState* sts, const Pools& P, uint32_t nq, hipStream_t s)

@@ -12,7 +12,7 @@ namespace kgmt
 struct Launch
 {
     uint32_t propGrid  = 1024;  // persistent workgroups of the propagate kernel
-    uint32_t propBlock = 256;   // threads per propagate workgroup (= kTile)
+    uint32_t propBlock = kTile; // threads per propagate workgroup = children per tile (128, 256, 512)
     uint32_t chunkGrid = 1024;  // persistent workgroups of append / expand
 };
 

@@ -1,0 +1,61 @@
+"""Writes tests/golden/plan_stats_<workload>.json: the CPU oracle's planning
+outcome for every seed of the statistical parity sets (50 seeds of the 6D
+double integrator, 200 of the 5D Dubins airplane, 200 of the 12D
+quadcopter), one record per seed:
+
+  [seed, iterations, tree_size, goal_index, path_rows, path_cost bits,
+   path_length bits, expanded]
+
+(float32 values as their uint32 bit patterns). The GPU planner is compared
+against these records seed by seed and as distributions.
+
+    python tests/golden/make_plan_stats.py [workload ...] [--threads N]
+"""
+import argparse
+import json
+import os
+import struct
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+SEEDS = {"di6d": 50, "dubins5d": 200, "quad12d": 200}
+
+
+def f32_bits(x: float) -> int:
+    return struct.unpack("<I", struct.pack("<f", x))[0]
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("workloads", nargs="*", default=sorted(SEEDS))
+    ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
+    a = ap.parse_args(argv)
+    from kgmt_amd import workloads as W
+    from oracle import oracle as O
+
+    for name in a.workloads:
+        n = SEEDS[name]
+        qs = [W.make_query(name, s) for s in range(n)]
+        cfgs, starts, goals, obs, offs, cnts = W.pack_batch(qs)
+        t0 = time.perf_counter()
+        stats, ms = O.plan_many(cfgs, starts, goals, obs, offs, cnts, threads=a.threads)
+        dt = time.perf_counter() - t0
+        rows = [[s, st["iterations"], st["tree_size"], st["goal_index"], st["path_rows"], f32_bits(st["path_cost"]),
+                 f32_bits(st["path_length"]), st["expanded"]] for s, st in enumerate(stats)]
+        out = {"workload": name, "seeds": n, "max_iter": qs[0].cfg.max_iter, "capacity": qs[0].cfg.capacity,
+               "fields": ["seed", "iterations", "tree_size", "goal_index", "path_rows", "path_cost_bits",
+                          "path_length_bits", "expanded"],
+               "records": rows}
+        with open(os.path.join(HERE, f"plan_stats_{name}.json"), "w") as f:
+            json.dump(out, f, separators=(",", ":"))
+            f.write("\n")
+        ok = sum(1 for r in rows if r[3] >= 0)
+        print(f"{name}: {n} seeds, {ok} solved, {dt:.1f} s on {a.threads} threads", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from kgmt_amd import workloads as W
+from kgmt_amd._lib import KgmtError
 from kgmt_amd.abi import KGMT_DI_EXACT
 from kgmt_amd.planner import KGMT, BatchKGMT
 
@@ -124,3 +125,40 @@ def test_batch_matches_oracle(use_graph, orc):
     # replaying the same batch gives the same answers
     results2, _ = b.plan(starts, goals, obs, offs, cnts, use_graph=use_graph)
     assert [r.tree_size for r in results2] == [r.tree_size for r in results]
+
+
+@pytest.mark.parametrize("shape", [dict(propagate_block=128, propagate_grid=64, chunk_grid=32),
+                                   dict(propagate_block=512, propagate_grid=2048, chunk_grid=1024),
+                                   dict(propagate_block=256, propagate_grid=1, chunk_grid=1)])
+def test_launch_shape_does_not_change_results(shape, orc):
+    """Tiles of 128/256/512 children and any persistent grid give the
+    oracle's trees, plain and batched, with and without the graph."""
+    q = W.make_query("dubins5d", 4, capacity=1 << 14, max_iter=30)
+    ref = orc.plan(q.cfg, q.start, q.goal, q.obstacles)
+    g = KGMT(q.cfg)
+    g.set_launch(**shape)
+    for graph in (False, True):
+        g.set_graph(graph)
+        res = g.plan(q.start, q.goal, q.obstacles, len(q.obstacles))
+        _same_plan(g, res, ref)
+    qs = W.make_batch(8, seed=11, max_iter=20)
+    for qq in qs:
+        qq.cfg = qq.cfg.replace(capacity=min(qq.cfg.capacity, 1 << 13))
+    cfgs, starts, goals, obs, offs, cnts = W.pack_batch(qs)
+    b = BatchKGMT(cfgs)
+    b.set_launch(**shape)
+    results, _ = b.plan(starts, goals, obs, offs, cnts, use_graph=True)
+    for i, qq in enumerate(qs):
+        st = orc.plan(qq.cfg, qq.start, qq.goal, qq.obstacles)["stats"]
+        assert (results[i].iterations, results[i].tree_size) == (st["iterations"], st["tree_size"]), i
+
+
+def test_set_launch_rejects_bad_shapes():
+    q = W.make_query("di6d", 0, capacity=1 << 12)
+    g = KGMT(q.cfg)
+    for bad in (dict(propagate_block=64), dict(propagate_block=1024), dict(propagate_grid=1 << 21)):
+        with pytest.raises(KgmtError):
+            g.set_launch(**bad)
+    g.set_launch(propagate_block=512)
+    res = g.plan(q.start, q.goal, q.obstacles, len(q.obstacles))
+    assert res.tree_size >= 1
