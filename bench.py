@@ -200,6 +200,8 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=512, help="mixed batch size (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes (no autotuning)")
+    ap.add_argument("--launch", default="",
+                    help="plain planner launch shape BLOCK,LANES,GRID,CHUNKGRID (skips tuning)")
     ap.add_argument("--refine", type=int, default=4096, help="trajectories of the adjoint refinement (0: skip)")
     ap.add_argument("--cpu-queries", type=int, default=0, help="CPU sample size (default: one per thread)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -229,11 +231,14 @@ def main(argv=None):
         return 1
 
     plain_launch = None
-    if not a.no_tune:
+    if a.launch:
+        blk, lanes, grd, chk = (int(x) for x in a.launch.split(","))
+        plain_launch = {"propagate_block": blk, "lanes_per_child": lanes, "propagate_grid": grd, "chunk_grid": chk}
+    elif not a.no_tune:
         from kgmt_amd import autotune
         from kgmt_amd import workloads as W
-        tq = [W.make_query(a.workload, s) for s in range(1000, 1004)]
-        plain_launch = autotune.tune_plain(tq, reps=1, cache=None)["launch"]
+        tq = [W.make_query(a.workload, s) for s in range(1000, 1008)]
+        plain_launch = autotune.tune_plain(tq, reps=2, cache=None)["launch"]
     cfg, res, dt, prof = run_plain(a.workload, a.steps, a.warmup, launch=plain_launch)
     tts = [r["time_ms"] for r in res]
     succ = [r for r in res if r["success"]]

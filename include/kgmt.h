@@ -179,12 +179,14 @@ int kgmt_set_seed(kgmt_ctx* ctx, uint64_t seed);
 /* Replay iterations from a hipGraph captured at the first plan (on != 0) or
  * launch them one by one (0, the default; needed for kgmt_set_profiling). */
 int kgmt_set_graph(kgmt_ctx* ctx, int32_t on);
-/* Launch shape of the planner's persistent kernels: children per propagate
- * tile (= its workgroup size: 128, 256 or 512) and the workgroup counts of the
- * propagate and append/expand kernels (0 keeps the current value). Results do
- * not depend on it. No counterpart in the reference (fixed launch shapes at
- * KGMT.cu:183-211). */
-int kgmt_set_launch(kgmt_ctx* ctx, uint32_t propagate_block, uint32_t propagate_grid, uint32_t chunk_grid);
+/* Launch shape of the planner's persistent kernels: threads per propagate
+ * workgroup (128, 256 or 512), lanes per child (1; or 2 or 4 with 256
+ * threads: the lanes split the child's box tests), and the workgroup counts
+ * of the propagate and append/expand kernels (0 keeps the current value).
+ * Results do not depend on it. No counterpart in the reference (fixed launch
+ * shapes at KGMT.cu:183-211). */
+int kgmt_set_launch(kgmt_ctx* ctx, uint32_t propagate_block, uint32_t lanes_per_child, uint32_t propagate_grid,
+                    uint32_t chunk_grid);
 /* Use an existing hipStream_t for every launch of this context (NULL: own stream). */
 int kgmt_set_stream(kgmt_ctx* ctx, void* hip_stream);
 /* start, goal: host arrays of D floats (only state[0..S) of start and
@@ -225,8 +227,8 @@ int kgmt_batch_get_path(kgmt_batch* b, uint32_t q, float* rows, uint32_t cap_row
 int kgmt_batch_get_tree(kgmt_batch* b, uint32_t q, float* samples, int32_t* parents, float* costs,
                         uint32_t cap, uint32_t* n);
 /* Launch shape of the persistent kernels, as kgmt_set_launch. */
-int kgmt_batch_set_launch(kgmt_batch* b, uint32_t propagate_block, uint32_t propagate_grid,
-                          uint32_t chunk_grid);
+int kgmt_batch_set_launch(kgmt_batch* b, uint32_t propagate_block, uint32_t lanes_per_child,
+                          uint32_t propagate_grid, uint32_t chunk_grid);
 int kgmt_batch_set_profiling(kgmt_batch* b, int32_t on);
 int kgmt_batch_get_profile(kgmt_batch* b, kgmt_profile* out);
 

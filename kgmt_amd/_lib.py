@@ -40,7 +40,7 @@ SIGNATURES = {
     "kgmt_destroy": (C.c_int, [_P]),
     "kgmt_set_seed": (C.c_int, [_P, C.c_uint64]),
     "kgmt_set_graph": (C.c_int, [_P, _i32]),
-    "kgmt_set_launch": (C.c_int, [_P, _u32, _u32, _u32]),
+    "kgmt_set_launch": (C.c_int, [_P, _u32, _u32, _u32, _u32]),
     "kgmt_set_stream": (C.c_int, [_P, _P]),
     "kgmt_plan": (C.c_int, [_P, _P, _P, _P, _u32, _i32, _resp]),
     "kgmt_get_tree": (C.c_int, [_P, _P, _P, _P, _u32, C.POINTER(_u32)]),
@@ -56,7 +56,7 @@ SIGNATURES = {
     "kgmt_batch_plan": (C.c_int, [_P, _P, _P, _P, _P, _P, _i32, _resp, _P]),
     "kgmt_batch_get_path": (C.c_int, [_P, _u32, _P, _u32, C.POINTER(_u32)]),
     "kgmt_batch_get_tree": (C.c_int, [_P, _u32, _P, _P, _P, _u32, C.POINTER(_u32)]),
-    "kgmt_batch_set_launch": (C.c_int, [_P, _u32, _u32, _u32]),
+    "kgmt_batch_set_launch": (C.c_int, [_P, _u32, _u32, _u32, _u32]),
     "kgmt_draw_controls": (C.c_int, [_cfgp, _u32, _u32, _u32, _u32, _P, _P, _P, _P]),
     "kgmt_propagate_collide": (C.c_int, [_cfgp, _P, _P, _u32, _P, _P, _u32, _P, _u32, _P, _P, _P, _P, _P]),
     "kgmt_region_update": (C.c_int, [_cfgp, _P, _P, _P, _P, _u32, _P, _P, _P, _P, _P, _P, _P]),

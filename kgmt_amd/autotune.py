@@ -1,15 +1,19 @@
 """Launch-shape autotuner of the planner's persistent kernels.
 
-The tunables are the propagate tile (children per tile = threads per
-workgroup: 128, 256 or 512), the number of persistent propagate workgroups
-and the number of append/expand workgroups (multiples of the device's compute
-units). None of them changes a result: every candidate's outcomes are checked
-against the first candidate's, query by query, and a mismatch is an error.
+The tunables are the propagate workgroup shape (threads per workgroup 128,
+256 or 512 with one lane per child, or 256 threads with 2 or 4 lanes per
+child splitting each child's box tests: tiles of 64 to 512 children), the
+number of persistent propagate workgroups and the number of append/expand
+workgroups (multiples of the device's compute units). None of them changes a
+result: every candidate's outcomes are checked against the first candidate's,
+query by query, and a mismatch is an error.
 
-A search times every candidate on a calibration workload (plans run as the
-caller will run them: batched with or without a hipGraph, or plain), keeps
-the fastest by the median of `reps` timed runs, and records it in a JSON
-cache keyed by device and workload signature, so later processes reuse it.
+A search times candidates on a calibration workload (plans run as the caller
+will run them: batched with or without a hipGraph, or plain) in two passes:
+every propagate shape x propagate grid at the default append/expand grid,
+then the best of those x every append/expand grid. It keeps the fastest by
+the median of `reps` timed runs and records it in a JSON cache keyed by
+device and workload signature, so later processes reuse it.
 
     best = tune_batch(cfgs, starts, goals, obstacles, offsets, counts)
     batch.set_launch(**best["launch"])
@@ -24,22 +28,45 @@ import statistics
 import time
 from typing import Callable, Iterable
 
-TILES = (128, 256, 512)
+SHAPES = ((256, 1), (128, 1), (512, 1), (256, 2), (256, 4))  # (threads, lanes per child)
 PROP_PER_CU = (2, 4, 8)
 CHUNK_PER_CU = (1, 2, 4)
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "kgmt_amd", "launch.json")
 
 
-def candidates(cus: int, tiles: Iterable[int] = TILES, prop_per_cu: Iterable[int] = PROP_PER_CU,
-               chunk_per_cu: Iterable[int] = CHUNK_PER_CU) -> list[dict]:
-    """Every launch shape of the search, the engine's default (256-child
-    tiles, 4 workgroups per CU for both kernels) first."""
+def candidates(cus: int, shapes: Iterable = SHAPES, prop_per_cu: Iterable[int] = PROP_PER_CU,
+               chunk_grid: int | None = None) -> list[dict]:
+    """First-pass launch shapes: every propagate shape x propagate grid, with
+    the append/expand grid fixed (default 4 workgroups per CU); the engine's
+    default (256 threads, 1 lane, 4 per CU) first."""
+    cg = int(chunk_grid if chunk_grid is not None else 4 * cus)
     out = []
-    for t, p, c in itertools.product(tiles, prop_per_cu, chunk_per_cu):
-        out.append({"propagate_block": int(t), "propagate_grid": int(p * cus), "chunk_grid": int(c * cus)})
-    default = {"propagate_block": 256, "propagate_grid": 4 * cus, "chunk_grid": 4 * cus}
+    for (t, k), p in itertools.product(shapes, prop_per_cu):
+        out.append({"propagate_block": int(t), "lanes_per_child": int(k), "propagate_grid": int(p * cus),
+                    "chunk_grid": cg})
+    default = {"propagate_block": 256, "lanes_per_child": 1, "propagate_grid": 4 * cus, "chunk_grid": cg}
     out.sort(key=lambda d: d != default)
     return out
+
+
+def chunk_candidates(best: dict, cus: int, chunk_per_cu: Iterable[int] = CHUNK_PER_CU) -> list[dict]:
+    """Second pass: the first pass's best with every append/expand grid (the
+    best itself first, so the outcome reference stays the same)."""
+    out = [dict(best)]
+    for c in chunk_per_cu:
+        d = dict(best, chunk_grid=int(c * cus))
+        if d not in out:
+            out.append(d)
+    return out
+
+
+def two_pass(cus: int, run, reps: int, cands: list[dict] | None = None) -> dict:
+    first = search(cands or candidates(cus), run, reps)
+    if cands:
+        return first
+    second = search(chunk_candidates(first["launch"], cus), run, reps)
+    best = second if second["seconds"] < first["seconds"] else first
+    return {"launch": best["launch"], "seconds": best["seconds"], "trials": first["trials"] + second["trials"][1:]}
 
 
 def search(cands: list[dict], run: Callable[[dict], tuple[float, object]], reps: int = 3) -> dict:
@@ -120,7 +147,7 @@ def tune_batch(cfgs, starts, goals, obstacles, offsets, counts, use_graph: bool 
         return dt, [(r.tree_size, r.iterations, r.goal_index, r.success) for r in res]
 
     try:
-        out = search(cands or candidates(cus), run, reps)
+        out = two_pass(cus, run, reps, cands)
     finally:
         b.close()
     out["key"] = key
@@ -157,7 +184,7 @@ def tune_plain(queries, reps: int = 3, cache: str | None = DEFAULT_CACHE, cands:
             outs.append((r.tree_size, r.iterations, r.goal_index))
         return time.perf_counter() - t0, outs
 
-    out = search(cands or candidates(cus), run, reps)
+    out = two_pass(cus, run, reps, cands)
     out["key"] = key
     if cache:
         store_cache(key, {k: v for k, v in out.items() if k != "trials"}, cache)

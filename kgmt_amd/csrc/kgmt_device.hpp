@@ -228,9 +228,27 @@ __device__ __forceinline__ void sincos_k(float x, float& s, float& c)
 // -------------------------------------------------------------------------
 struct Scene
 {
-    const float4* grp;  // nGrp (lo, hi) pairs, then kGroup * nGrp boxes
+    const float4* grp;  // nGrp (lo, hi) pairs, then kGroup * nGrp boxes (LDS)
     int nGrp;
+    int sub = 0;        // lane within the child's group of K lanes
 };
+
+// OR of a flag over each group of K consecutive lanes (K = 1, 2, 4), by DPP
+// quad permutations; every lane of a group must be active.
+template <int K>
+__device__ __forceinline__ bool group_or(bool v)
+{
+    static_assert(K == 1 || K == 2 || K == 4, "lanes per child");
+    if constexpr(K == 1)
+        return v;
+    else
+        {
+            int x = v ? 1 : 0;
+            x |= __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]: lane ^ 1
+            if constexpr(K == 4) x |= __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);  // [2,3,0,1]: lane ^ 2
+            return x != 0;
+        }
+}
 
 __device__ __forceinline__ bool separated(float mnx, float mny, float mnz, float mxx, float mxy, float mxz, float4 lo,
                                           float4 hi)
@@ -238,6 +256,10 @@ __device__ __forceinline__ bool separated(float mnx, float mny, float mnz, float
     return (mxx <= lo.x) | (hi.x <= mnx) | (mxy <= lo.y) | (hi.y <= mny) | (mxz <= lo.z) | (hi.z <= mnz);
 }
 
+// Segment vs every box of the scene (collisionCheck.cu:49-62 per step). With
+// K lanes per child, lane `sc.sub` of the group tests groups sub, sub + K, ...
+// and the verdicts are ORed over the group, so every lane returns the same.
+template <int K = 1>
 __device__ __forceinline__ bool segment_free(float ax, float ay, float az, float bx, float by, float bz,
                                              const Scene& sc)
 {
@@ -245,26 +267,41 @@ __device__ __forceinline__ bool segment_free(float ax, float ay, float az, float
     const float mny = ay > by ? by : ay, mxy = ay > by ? ay : by;
     const float mnz = az > bz ? bz : az, mxz = az > bz ? az : bz;
     const float4* box = sc.grp + 2 * sc.nGrp;
-    bool hit          = false;
-    for(int g0 = 0; g0 < sc.nGrp; g0 += (int)kGroupBatch)
+    bool hit            = false;
+    for(int g0 = 0; g0 < sc.nGrp; g0 += (int)kGroupBatch * K)
         {
             bool near[kGroupBatch];
+            int gi[kGroupBatch];
 #pragma unroll
             for(int k = 0; k < (int)kGroupBatch; ++k)
-                near[k] = !separated(mnx, mny, mnz, mxx, mxy, mxz, sc.grp[2 * (g0 + k)], sc.grp[2 * (g0 + k) + 1]);
+                {
+                    if constexpr(K == 1)
+                        {
+                            gi[k]   = g0 + k;
+                            near[k] = !separated(mnx, mny, mnz, mxx, mxy, mxz, sc.grp[2 * gi[k]], sc.grp[2 * gi[k] + 1]);
+                        }
+                    else
+                        {
+                            // past the last group: re-test the last one (an OR
+                            // does not change when a box is tested twice)
+                            const int g = g0 + k * K + sc.sub;
+                            gi[k]       = g < sc.nGrp ? g : sc.nGrp - 1;
+                            near[k]     = !separated(mnx, mny, mnz, mxx, mxy, mxz, sc.grp[2 * gi[k]], sc.grp[2 * gi[k] + 1]);
+                        }
+                }
 #pragma unroll
             for(int k = 0; k < (int)kGroupBatch; ++k)
                 {
                     if(__any(near[k] && !hit))
                         {
-                            const float4* b = box + 2 * (int)kGroup * (g0 + k);
+                            const float4* b = box + 2 * (int)kGroup * gi[k];
 #pragma unroll 4
                             for(int m = 0; m < (int)kGroup; ++m)
                                 hit |= !separated(mnx, mny, mnz, mxx, mxy, mxz, b[2 * m], b[2 * m + 1]);
                         }
                 }
         }
-    return !hit;
+    return !group_or<K>(hit);
 }
 
 __device__ __forceinline__ bool ws_out(const kgmt_config& c, float x, float y, float z)
@@ -300,6 +337,7 @@ struct ModelDims<KGMT_MODEL_DUBINS_5D>
 };
 
 // statePropagator.cu:76-142
+template <int K = 1>
 __device__ __forceinline__ bool prop_di(const QueryDesc& qd, float* s, const float* u, int steps,
                                         const Scene& sc)
 {
@@ -340,7 +378,7 @@ __device__ __forceinline__ bool prop_di(const QueryDesc& qd, float* s, const flo
                     valid = false;
                     break;
                 }
-            if(!segment_free(px, py, pz, x, y, z, sc))
+            if(!segment_free<K>(px, py, pz, x, y, z, sc))
                 {
                     valid = false;
                     break;
@@ -351,7 +389,7 @@ __device__ __forceinline__ bool prop_di(const QueryDesc& qd, float* s, const flo
 }
 
 // statePropagator.cu:147-241; FIVE_D: speed is control u[2], no acceleration.
-template <bool FIVE_D>
+template <bool FIVE_D, int K = 1>
 __device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const float* u, int steps,
                                             const Scene& sc)
 {
@@ -403,7 +441,7 @@ __device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const
                     valid = false;
                     break;
                 }
-            if(!segment_free(px, py, pz, x, y, z, sc))
+            if(!segment_free<K>(px, py, pz, x, y, z, sc))
                 {
                     valid = false;
                     break;
@@ -443,6 +481,7 @@ __device__ __forceinline__ void quad_ode(const QueryDesc& qd, const float* st, c
 }
 
 // statePropagator.cu:246-323
+template <int K = 1>
 __device__ __forceinline__ bool prop_quad(const QueryDesc& qd, float* s, const float* uc, int steps,
                                           const Scene& sc)
 {
@@ -476,7 +515,7 @@ __device__ __forceinline__ bool prop_quad(const QueryDesc& qd, float* s, const f
                     valid = false;
                     break;
                 }
-            if(!segment_free(px, py, pz, s[0], s[1], s[2], sc))
+            if(!segment_free<K>(px, py, pz, s[0], s[1], s[2], sc))
                 {
                     valid = false;
                     break;
@@ -485,18 +524,18 @@ __device__ __forceinline__ bool prop_quad(const QueryDesc& qd, float* s, const f
     return valid;
 }
 
-template <int MODEL>
+template <int MODEL, int K = 1>
 __device__ __forceinline__ bool propagate(const QueryDesc& qd, float* s, const float* u, int steps,
                                           const Scene& sc)
 {
     if constexpr(MODEL == KGMT_MODEL_DOUBLE_INTEGRATOR)
-        return prop_di(qd, s, u, steps, sc);
+        return prop_di<K>(qd, s, u, steps, sc);
     else if constexpr(MODEL == KGMT_MODEL_DUBINS_AIRPLANE)
-        return prop_dubins<false>(qd, s, u, steps, sc);
+        return prop_dubins<false, K>(qd, s, u, steps, sc);
     else if constexpr(MODEL == KGMT_MODEL_DUBINS_5D)
-        return prop_dubins<true>(qd, s, u, steps, sc);
+        return prop_dubins<true, K>(qd, s, u, steps, sc);
     else
-        return prop_quad(qd, s, u, steps, sc);
+        return prop_quad<K>(qd, s, u, steps, sc);
 }
 
 // -------------------------------------------------------------------------

@@ -311,7 +311,7 @@ struct Engine
            (rc = dev_alloc(W.counter, 1, owned)))
             return rc;
         W.nq   = n;
-        W.tile = L.propBlock;
+        W.tile = L.propBlock / L.lanes;
         HIP_TRY(hipMemset(W.counter, 0, sizeof(uint32_t)));
         HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         ownStream = true;
@@ -446,7 +446,8 @@ struct Engine
         __atomic_store_n(pinned, 0u, __ATOMIC_RELEASE);
         HIP_TRY(launch_init(dq, ds, P, W, stream));
         if(useGraph && (!graphExec || graphObs != maxPairs || graphLaunch.propGrid != L.propGrid ||
-                        graphLaunch.chunkGrid != L.chunkGrid || graphLaunch.propBlock != L.propBlock))
+                        graphLaunch.chunkGrid != L.chunkGrid || graphLaunch.propBlock != L.propBlock ||
+                        graphLaunch.lanes != L.lanes))
             {
                 int rc = build_graph();
                 if(rc) return rc;
@@ -671,21 +672,26 @@ int kgmt_set_seed(kgmt_ctx* ctx, uint64_t seed)
     return KGMT_OK;
 }
 
-static int set_launch(Engine& e, uint32_t propagate_block, uint32_t propagate_grid, uint32_t chunk_grid)
+static int set_launch(Engine& e, uint32_t propagate_block, uint32_t lanes_per_child, uint32_t propagate_grid,
+                      uint32_t chunk_grid)
 {
-    if(propagate_block && propagate_block != 128 && propagate_block != 256 && propagate_block != 512)
-        return fail(KGMT_E_UNSUPPORTED, "propagate block must be 128, 256 or 512");
+    const uint32_t blk = propagate_block ? propagate_block : e.L.propBlock;
+    const uint32_t ln  = lanes_per_child ? lanes_per_child : e.L.lanes;
+    if(!launch_supported(blk, ln))
+        return fail(KGMT_E_UNSUPPORTED, "propagate shape must be 128/256/512 threads with 1 lane per child, "
+                                        "or 256 threads with 2 or 4 lanes per child");
     if(propagate_grid > (1u << 20) || chunk_grid > (1u << 20)) return fail(KGMT_E_ARG, "grid too large");
-    if(propagate_block) e.L.propBlock = propagate_block, e.W.tile = propagate_block;
+    e.L.propBlock = blk, e.L.lanes = ln, e.W.tile = blk / ln;
     if(propagate_grid) e.L.propGrid = propagate_grid;
     if(chunk_grid) e.L.chunkGrid = chunk_grid;
     return KGMT_OK;
 }
 
-int kgmt_set_launch(kgmt_ctx* ctx, uint32_t propagate_block, uint32_t propagate_grid, uint32_t chunk_grid)
+int kgmt_set_launch(kgmt_ctx* ctx, uint32_t propagate_block, uint32_t lanes_per_child, uint32_t propagate_grid,
+                    uint32_t chunk_grid)
 {
     if(!ctx) return fail(KGMT_E_ARG, "null context");
-    return set_launch(ctx->eng, propagate_block, propagate_grid, chunk_grid);
+    return set_launch(ctx->eng, propagate_block, lanes_per_child, propagate_grid, chunk_grid);
 }
 
 int kgmt_set_graph(kgmt_ctx* ctx, int32_t on)
@@ -851,10 +857,11 @@ int kgmt_batch_set_stream(kgmt_batch* b, void* s)
     return b->eng.set_stream(s);
 }
 
-int kgmt_batch_set_launch(kgmt_batch* b, uint32_t propagate_block, uint32_t propagate_grid, uint32_t chunk_grid)
+int kgmt_batch_set_launch(kgmt_batch* b, uint32_t propagate_block, uint32_t lanes_per_child, uint32_t propagate_grid,
+                          uint32_t chunk_grid)
 {
     if(!b) return fail(KGMT_E_ARG, "null batch");
-    return set_launch(b->eng, propagate_block, propagate_grid, chunk_grid);
+    return set_launch(b->eng, propagate_block, lanes_per_child, propagate_grid, chunk_grid);
 }
 
 int kgmt_batch_plan(kgmt_batch* b, const float* starts, const float* goals, const float* obstacles,

@@ -256,17 +256,21 @@ struct TileLds
     uint32_t sbuf[NT / 64 + 1];
 };
 
-// One tile of NT children u = base .. base + NT - 1 of a query (NT divides
-// kChunk, so a tile never straddles two chunks). The children
-// are drawn in tile order, then handed to the lanes longest-first (counting
-// sort on the step count) so that each wave's lanes integrate about the same
-// number of steps. Each child still writes only its own slot and the
-// counters are integer atomics, so the outputs do not depend on the order.
-template <int MODEL, int NT>
+// One tile of TC = NT / K children u = base .. base + TC - 1 of a query (TC
+// divides kChunk, so a tile never straddles two chunks), K lanes per child.
+// The children are drawn in tile order, then handed to the lane groups
+// longest-first (counting sort on the step count) so that each wave's groups
+// integrate about the same number of steps. The K lanes of a group integrate
+// the same edge (same operations, same bits) and split its box tests; the
+// group's first lane writes the child. Each child still writes only its own
+// slot and the counters are integer atomics, so the outputs do not depend on
+// the order or on K.
+template <int MODEL, int NT, int K>
 __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState& st, uint32_t base,
                                            const Pools& P, const float4* sObs, TileLds<NT>& L)
 {
     constexpr int S = ModelDims<MODEL>::S, C = ModelDims<MODEL>::C, D = S + C + 1;
+    constexpr uint32_t TC = NT / K;
     const uint32_t nC  = st.nChildren;
     const uint32_t cap = qd.cap;
     const size_t nodeO = qd.nodeOff;
@@ -278,7 +282,7 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
         int32_t steps = 0;
         float uf      = 0.0f;
         uint32_t bucket = NT - 1;  // past the end: last
-        if(un < nC)
+        if(t < TC && un < nC)
             {
                 draw_child<C>(qd.cfg, qd.cfg.query_id, st.itr, un, uc, steps, uf);
                 const int32_t k = steps - qd.cfg.steps_min;
@@ -286,23 +290,28 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
             }
         L.hist[t] = 0;
         __syncthreads();
-        const uint32_t rank = atomicAdd(&L.hist[bucket], 1u);
+        uint32_t rank = 0;
+        if(t < TC) rank = atomicAdd(&L.hist[bucket], 1u);
         __syncthreads();
         uint32_t tot;
         const uint32_t off = block_excl_scan<NT>(L.hist[t], L.sbuf, tot);
         L.hist[t]          = off;
         __syncthreads();
-        L.slot[L.hist[bucket] + rank] = (uint16_t)t;
+        if(t < TC)
+            {
+                L.slot[L.hist[bucket] + rank] = (uint16_t)t;
 #pragma unroll
-        for(int k = 0; k < C; ++k) L.ctrl[k][t] = uc[k];
-        L.steps[t]  = steps;
-        L.ufront[t] = uf;
+                for(int k = 0; k < C; ++k) L.ctrl[k][t] = uc[k];
+                L.steps[t]  = steps;
+                L.ufront[t] = uf;
+            }
         __syncthreads();
     }
-    const uint32_t i   = L.slot[t];
-    const uint32_t u   = base + i;
-    const bool has     = u < nC;
-    int r1             = 0;
+    const uint32_t i    = L.slot[t / K];
+    const uint32_t u    = base + i;
+    const bool has      = u < nC;
+    const bool lead     = K == 1 || t % K == 0;
+    int r1              = 0;
     bool ok = false, fn = false;
     if(has)
         {
@@ -317,41 +326,45 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
             for(int k = 0; k < C; ++k) uc[k] = L.ctrl[k][i];
             const int32_t steps = L.steps[i];
             const float uf      = L.ufront[i];
-            const Scene sc{sObs, (int)qd.nGrp};
-            ok       = propagate<MODEL>(qd, s, uc, steps, sc);
-            float* U = P.childS + qd.sampleOff;
-#pragma unroll
-            for(int d = 0; d < S; ++d) U[(size_t)d * cap + u] = s[d];
-#pragma unroll
-            for(int k = 0; k < C; ++k) U[(size_t)(S + k) * cap + u] = uc[k];
-            U[(size_t)(D - 1) * cap + u] = qd.cfg.dt * (float)steps;
-            P.parentU[nodeO + u]          = (int32_t)p;
-            if(u % per == 0) P.frontier[nodeO + p] = 0;
-            int ix, iy, iz;
-            r1                   = region_of(qd, s[0], s[1], s[2], ix, iy, iz);
-            P.r1child[nodeO + u] = r1;
-            if(ok)
+            const Scene sc{sObs, (int)qd.nGrp, (int)(t % K)};
+            ok = propagate<MODEL, K>(qd, s, uc, steps, sc);
+            if(lead)
                 {
-                    const int r2    = subregion_of(qd, s[0], s[1], s[2], r1, ix, iy, iz);
-                    const bool seen = test_bit(P.actPrev + qd.subOff, (uint32_t)r2);
-                    fn              = (uf < P.score[qd.regOff + r1]) || !seen;
-                    if(!seen) atomicOr(&P.actNext[qd.subOff + ((uint32_t)r2 >> 5)], 1u << ((uint32_t)r2 & 31u));
+                    float* U = P.childS + qd.sampleOff;
+#pragma unroll
+                    for(int d = 0; d < S; ++d) U[(size_t)d * cap + u] = s[d];
+#pragma unroll
+                    for(int k = 0; k < C; ++k) U[(size_t)(S + k) * cap + u] = uc[k];
+                    U[(size_t)(D - 1) * cap + u] = qd.cfg.dt * (float)steps;
+                    P.parentU[nodeO + u]          = (int32_t)p;
+                    if(u % per == 0) P.frontier[nodeO + p] = 0;
+                    int ix, iy, iz;
+                    r1                   = region_of(qd, s[0], s[1], s[2], ix, iy, iz);
+                    P.r1child[nodeO + u] = r1;
+                    if(ok)
+                        {
+                            const int r2    = subregion_of(qd, s[0], s[1], s[2], r1, ix, iy, iz);
+                            const bool seen = test_bit(P.actPrev + qd.subOff, (uint32_t)r2);
+                            fn              = (uf < P.score[qd.regOff + r1]) || !seen;
+                            if(!seen)
+                                atomicOr(&P.actNext[qd.subOff + ((uint32_t)r2 >> 5)], 1u << ((uint32_t)r2 & 31u));
+                        }
+                    P.fnext[nodeO + u] = fn ? 1 : 0;
                 }
-            P.fnext[nodeO + u] = fn ? 1 : 0;
         }
-    wave_count(P.total + qd.regOff, P.validc + qd.regOff, r1, has, ok);
+    wave_count(P.total + qd.regOff, P.validc + qd.regOff, r1, has && lead, ok);
     // the tile lies inside one 1024-element chunk
     const uint64_t fm = __ballot(fn);
     if(lane_id() == 0 && fm) atomicAdd(&P.partA[qd.chunkOff + base / kChunk], (uint32_t)__popcll(fm));
     __syncthreads();  // L is reused by the next tile
 }
 
-template <int MODEL, int NT>
+template <int MODEL, int NT, int K>
 __global__ __launch_bounds__(NT) void k_propagate(const QueryDesc* __restrict__ qds,
                                                   const QueryState* __restrict__ sts, Pools P, WorkLists W,
                                                   uint32_t parity)
 {
-    static_assert(NT % 64 == 0 && kChunk % NT == 0, "tile size");
+    static_assert(NT % 64 == 0 && kChunk % (NT / K) == 0, "tile size");
     extern __shared__ float4 sObs[];
     __shared__ TileLds<NT> L;
     const uint32_t nq        = W.nq;
@@ -370,23 +383,23 @@ __global__ __launch_bounds__(NT) void k_propagate(const QueryDesc* __restrict__ 
                     __syncthreads();
                     staged = q;
                 }
-            const uint32_t base = (t - tileList[q]) * NT;
+            const uint32_t base = (t - tileList[q]) * (NT / K);
             if constexpr(MODEL >= 0)
-                child_tile<MODEL, NT>(qd, sts[q], base, P, sObs, L);
+                child_tile<MODEL, NT, K>(qd, sts[q], base, P, sObs, L);
             else
                 {
                     switch(qd.cfg.model)
                         {
                             case KGMT_MODEL_DOUBLE_INTEGRATOR:
-                                child_tile<KGMT_MODEL_DOUBLE_INTEGRATOR, NT>(qd, sts[q], base, P, sObs, L);
+                                child_tile<KGMT_MODEL_DOUBLE_INTEGRATOR, NT, K>(qd, sts[q], base, P, sObs, L);
                                 break;
                             case KGMT_MODEL_DUBINS_AIRPLANE:
-                                child_tile<KGMT_MODEL_DUBINS_AIRPLANE, NT>(qd, sts[q], base, P, sObs, L);
+                                child_tile<KGMT_MODEL_DUBINS_AIRPLANE, NT, K>(qd, sts[q], base, P, sObs, L);
                                 break;
                             case KGMT_MODEL_DUBINS_5D:
-                                child_tile<KGMT_MODEL_DUBINS_5D, NT>(qd, sts[q], base, P, sObs, L);
+                                child_tile<KGMT_MODEL_DUBINS_5D, NT, K>(qd, sts[q], base, P, sObs, L);
                                 break;
-                            default: child_tile<KGMT_MODEL_QUADCOPTER, NT>(qd, sts[q], base, P, sObs, L); break;
+                            default: child_tile<KGMT_MODEL_QUADCOPTER, NT, K>(qd, sts[q], base, P, sObs, L); break;
                         }
                 }
         }
@@ -864,7 +877,7 @@ hipError_t launch_init(const QueryDesc* qds, QueryState* sts, const Pools& P, co
     return hipGetLastError();
 }
 
-template <int NT>
+template <int NT, int K>
 static hipError_t launch_propagate_nt(int model, const QueryDesc* qds, QueryState* sts, const Pools& P,
                                       const WorkLists& W, uint32_t parity, uint32_t maxPairs, const Launch& L,
                                       hipStream_t s)
@@ -874,34 +887,41 @@ static hipError_t launch_propagate_nt(int model, const QueryDesc* qds, QueryStat
     switch(model)
         {
             case KGMT_MODEL_DOUBLE_INTEGRATOR:
-                hipLaunchKernelGGL((k_propagate<KGMT_MODEL_DOUBLE_INTEGRATOR, NT>), g, b, lds, s, qds, sts, P, W,
+                hipLaunchKernelGGL((k_propagate<KGMT_MODEL_DOUBLE_INTEGRATOR, NT, K>), g, b, lds, s, qds, sts, P, W,
                                    parity);
                 break;
             case KGMT_MODEL_DUBINS_AIRPLANE:
-                hipLaunchKernelGGL((k_propagate<KGMT_MODEL_DUBINS_AIRPLANE, NT>), g, b, lds, s, qds, sts, P, W,
+                hipLaunchKernelGGL((k_propagate<KGMT_MODEL_DUBINS_AIRPLANE, NT, K>), g, b, lds, s, qds, sts, P, W,
                                    parity);
                 break;
             case KGMT_MODEL_DUBINS_5D:
-                hipLaunchKernelGGL((k_propagate<KGMT_MODEL_DUBINS_5D, NT>), g, b, lds, s, qds, sts, P, W, parity);
+                hipLaunchKernelGGL((k_propagate<KGMT_MODEL_DUBINS_5D, NT, K>), g, b, lds, s, qds, sts, P, W, parity);
                 break;
             case KGMT_MODEL_QUADCOPTER:
-                hipLaunchKernelGGL((k_propagate<KGMT_MODEL_QUADCOPTER, NT>), g, b, lds, s, qds, sts, P, W, parity);
+                hipLaunchKernelGGL((k_propagate<KGMT_MODEL_QUADCOPTER, NT, K>), g, b, lds, s, qds, sts, P, W, parity);
                 break;
-            default: hipLaunchKernelGGL((k_propagate<-1, NT>), g, b, lds, s, qds, sts, P, W, parity); break;
+            default: hipLaunchKernelGGL((k_propagate<-1, NT, K>), g, b, lds, s, qds, sts, P, W, parity); break;
         }
     return hipGetLastError();
+}
+
+bool launch_supported(uint32_t block, uint32_t lanes)
+{
+    return (lanes == 1 && (block == 128 || block == 256 || block == 512)) ||
+           (block == 256 && (lanes == 2 || lanes == 4));
 }
 
 hipError_t launch_propagate(int model, const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
                             uint32_t parity, uint32_t maxPairs, const Launch& L, hipStream_t s)
 {
-    if(W.tile != L.propBlock) return hipErrorInvalidValue;
+    if(!launch_supported(L.propBlock, L.lanes) || W.tile != L.propBlock / L.lanes) return hipErrorInvalidValue;
+    if(L.lanes == 2) return launch_propagate_nt<256, 2>(model, qds, sts, P, W, parity, maxPairs, L, s);
+    if(L.lanes == 4) return launch_propagate_nt<256, 4>(model, qds, sts, P, W, parity, maxPairs, L, s);
     switch(L.propBlock)
         {
-            case 128: return launch_propagate_nt<128>(model, qds, sts, P, W, parity, maxPairs, L, s);
-            case 256: return launch_propagate_nt<256>(model, qds, sts, P, W, parity, maxPairs, L, s);
-            case 512: return launch_propagate_nt<512>(model, qds, sts, P, W, parity, maxPairs, L, s);
-            default: return hipErrorInvalidValue;
+            case 128: return launch_propagate_nt<128, 1>(model, qds, sts, P, W, parity, maxPairs, L, s);
+            case 256: return launch_propagate_nt<256, 1>(model, qds, sts, P, W, parity, maxPairs, L, s);
+            default: return launch_propagate_nt<512, 1>(model, qds, sts, P, W, parity, maxPairs, L, s);
         }
 }
 

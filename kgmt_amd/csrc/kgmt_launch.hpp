@@ -12,7 +12,8 @@ namespace kgmt
 struct Launch
 {
     uint32_t propGrid  = 1024;  // persistent workgroups of the propagate kernel
-    uint32_t propBlock = kTile; // threads per propagate workgroup = children per tile (128, 256, 512)
+    uint32_t propBlock = kTile; // threads per propagate workgroup (128, 256, 512)
+    uint32_t lanes     = 1;     // lanes per child (1; 2 or 4 with 256 threads); children per tile = propBlock / lanes
     uint32_t chunkGrid = 1024;  // persistent workgroups of append / expand
 };
 
@@ -20,6 +21,9 @@ struct Launch
 // grouped by kGroup, written as nGrp group bounds followed by the boxes, each
 // a (lo, hi) float4 pair. Returns nGrp. rows: n x 6 floats.
 uint32_t build_scene(const float* rows, uint32_t n, float4* out);
+
+// propagate launch shapes that are built
+bool launch_supported(uint32_t block, uint32_t lanes);
 
 // planner iteration kernels; `model` = KGMT_MODEL_* when every query shares
 // it, -1 otherwise; `parity` = (iteration - 1) & 1 selects the work lists

@@ -109,10 +109,11 @@ class KGMT(Planner):
         """Replay iterations from a captured hipGraph (ignored while profiling)."""
         _lib.check(_lib.lib().kgmt_set_graph(self._h, 1 if on else 0))
 
-    def set_launch(self, propagate_block=0, propagate_grid=0, chunk_grid=0):
+    def set_launch(self, propagate_block=0, propagate_grid=0, chunk_grid=0, lanes_per_child=0):
         """Launch shape of the persistent kernels (0 keeps the current value);
         results do not depend on it."""
-        _lib.check(_lib.lib().kgmt_set_launch(self._h, int(propagate_block), int(propagate_grid), int(chunk_grid)))
+        _lib.check(_lib.lib().kgmt_set_launch(self._h, int(propagate_block), int(lanes_per_child),
+                                              int(propagate_grid), int(chunk_grid)))
 
     # KGMT::plan, src/planners/KGMT.cu:41-105
     def plan(self, h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount) -> KgmtResult:
@@ -203,10 +204,10 @@ class BatchKGMT:
         _lib.check(_lib.lib().kgmt_batch_get_profile(self._h, C.byref(p)))
         return p.as_dict()
 
-    def set_launch(self, propagate_block=0, propagate_grid=0, chunk_grid=0):
+    def set_launch(self, propagate_block=0, propagate_grid=0, chunk_grid=0, lanes_per_child=0):
         """As KGMT.set_launch."""
-        _lib.check(_lib.lib().kgmt_batch_set_launch(self._h, int(propagate_block), int(propagate_grid),
-                                                    int(chunk_grid)))
+        _lib.check(_lib.lib().kgmt_batch_set_launch(self._h, int(propagate_block), int(lanes_per_child),
+                                                    int(propagate_grid), int(chunk_grid)))
 
     def plan(self, starts, goals, obstacles, offsets, counts, use_graph=True):
         starts = _f32(starts).reshape(self.n, KGMT_MAX_SAMPLE)

@@ -8,14 +8,20 @@ from kgmt_amd import workloads as W
 
 def test_candidates_cover_the_grid_default_first():
     c = T.candidates(256)
-    assert len(c) == len(T.TILES) * len(T.PROP_PER_CU) * len(T.CHUNK_PER_CU)
-    assert c[0] == {"propagate_block": 256, "propagate_grid": 1024, "chunk_grid": 1024}
+    assert len(c) == len(T.SHAPES) * len(T.PROP_PER_CU)
+    assert c[0] == {"propagate_block": 256, "lanes_per_child": 1, "propagate_grid": 1024, "chunk_grid": 1024}
     assert len({tuple(sorted(d.items())) for d in c}) == len(c)
     assert {d["propagate_block"] for d in c} == {128, 256, 512}
+    assert {d["lanes_per_child"] for d in c} == {1, 2, 4}
+    # lanes > 1 only with 256 threads (the shapes the library builds)
+    assert all(d["propagate_block"] == 256 for d in c if d["lanes_per_child"] > 1)
+    second = T.chunk_candidates(c[3], 256)
+    assert second[0] == c[3] and len(second) == len(T.CHUNK_PER_CU)
+    assert {d["chunk_grid"] for d in second} == {256, 512, 1024}
 
 
 def test_search_picks_lowest_median_and_checks_outcomes():
-    cands = [{"propagate_block": b, "propagate_grid": 1, "chunk_grid": 1} for b in (128, 256, 512)]
+    cands = [{"propagate_block": b, "lanes_per_child": 1, "propagate_grid": 1, "chunk_grid": 1} for b in (128, 256, 512)]
     cost = {128: [3.0, 1.0, 9.0, 2.0], 256: [1.0, 1.5, 1.5, 1.5], 512: [5.0, 0.5, 4.0, 4.0]}
     calls = {}
 

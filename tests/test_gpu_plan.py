@@ -129,10 +129,14 @@ def test_batch_matches_oracle(use_graph, orc):
 
 @pytest.mark.parametrize("shape", [dict(propagate_block=128, propagate_grid=64, chunk_grid=32),
                                    dict(propagate_block=512, propagate_grid=2048, chunk_grid=1024),
-                                   dict(propagate_block=256, propagate_grid=1, chunk_grid=1)])
+                                   dict(propagate_block=256, propagate_grid=1, chunk_grid=1),
+                                   dict(propagate_block=256, lanes_per_child=2, propagate_grid=512, chunk_grid=256),
+                                   dict(propagate_block=256, lanes_per_child=4, propagate_grid=2048, chunk_grid=1024),
+                                   dict(propagate_block=256, lanes_per_child=4, propagate_grid=3, chunk_grid=5)])
 def test_launch_shape_does_not_change_results(shape, orc):
-    """Tiles of 128/256/512 children and any persistent grid give the
-    oracle's trees, plain and batched, with and without the graph."""
+    """Workgroups of 128/256/512 threads, 1, 2 or 4 lanes per child (tiles
+    of 64 to 512 children) and any persistent grid give the oracle's trees,
+    plain and batched, with and without the graph."""
     q = W.make_query("dubins5d", 4, capacity=1 << 14, max_iter=30)
     ref = orc.plan(q.cfg, q.start, q.goal, q.obstacles)
     g = KGMT(q.cfg)
@@ -156,7 +160,9 @@ def test_launch_shape_does_not_change_results(shape, orc):
 def test_set_launch_rejects_bad_shapes():
     q = W.make_query("di6d", 0, capacity=1 << 12)
     g = KGMT(q.cfg)
-    for bad in (dict(propagate_block=64), dict(propagate_block=1024), dict(propagate_grid=1 << 21)):
+    for bad in (dict(propagate_block=64), dict(propagate_block=1024), dict(propagate_grid=1 << 21),
+                dict(propagate_block=128, lanes_per_child=2), dict(lanes_per_child=3),
+                dict(propagate_block=512, lanes_per_child=4)):
         with pytest.raises(KgmtError):
             g.set_launch(**bad)
     g.set_launch(propagate_block=512)
