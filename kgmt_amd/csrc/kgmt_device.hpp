@@ -304,6 +304,53 @@ __device__ __forceinline__ bool segment_free(float ax, float ay, float az, float
     return !group_or<K>(hit);
 }
 
+// Segment-check policies of the dynamics below: chk(px, py, pz, x, y, z)
+// returns whether the step's segment is free.
+template <int K>
+struct FullCheck  // every box of the scene (K lanes split the groups)
+{
+    Scene sc;
+    __device__ __forceinline__ bool operator()(float px, float py, float pz, float x, float y, float z) const
+    {
+        return segment_free<K>(px, py, pz, x, y, z, sc);
+    }
+};
+
+struct DryRun  // no boxes; records the box spanned by every checked segment
+{
+    float lo[3], hi[3];
+    bool finite = true;
+    __device__ __forceinline__ void start(float x, float y, float z)
+    {
+        lo[0] = hi[0] = x, lo[1] = hi[1] = y, lo[2] = hi[2] = z;
+        finite = (x == x) && (y == y) && (z == z);
+    }
+    __device__ __forceinline__ bool operator()(float, float, float, float x, float y, float z)
+    {
+        finite = finite && (x == x) && (y == y) && (z == z);
+        lo[0] = x < lo[0] ? x : lo[0], hi[0] = x > hi[0] ? x : hi[0];
+        lo[1] = y < lo[1] ? y : lo[1], hi[1] = y > hi[1] ? y : hi[1];
+        lo[2] = z < lo[2] ? z : lo[2], hi[2] = z > hi[2] ? z : hi[2];
+        return true;
+    }
+};
+
+struct CandCheck  // a wave's candidate boxes, (lo, hi) float4 pairs in LDS
+{
+    const float4* c;
+    int n;
+    __device__ __forceinline__ bool operator()(float ax, float ay, float az, float bx, float by, float bz) const
+    {
+        const float mnx = ax > bx ? bx : ax, mxx = ax > bx ? ax : bx;
+        const float mny = ay > by ? by : ay, mxy = ay > by ? ay : by;
+        const float mnz = az > bz ? bz : az, mxz = az > bz ? az : bz;
+        bool hit = false;
+#pragma unroll 4
+        for(int m = 0; m < n; ++m) hit |= !separated(mnx, mny, mnz, mxx, mxy, mxz, c[2 * m], c[2 * m + 1]);
+        return !hit;
+    }
+};
+
 __device__ __forceinline__ bool ws_out(const kgmt_config& c, float x, float y, float z)
 {
     return x < c.ws_min || x > c.ws_max || y < c.ws_min || y > c.ws_max || z < c.ws_min || z > c.ws_max;
@@ -337,9 +384,8 @@ struct ModelDims<KGMT_MODEL_DUBINS_5D>
 };
 
 // statePropagator.cu:76-142
-template <int K = 1>
-__device__ __forceinline__ bool prop_di(const QueryDesc& qd, float* s, const float* u, int steps,
-                                        const Scene& sc)
+template <class CHK>
+__device__ __forceinline__ bool prop_di(const QueryDesc& qd, float* s, const float* u, int steps, CHK& chk)
 {
     const kgmt_config& c = qd.cfg;
     const float h        = c.dt;
@@ -378,7 +424,7 @@ __device__ __forceinline__ bool prop_di(const QueryDesc& qd, float* s, const flo
                     valid = false;
                     break;
                 }
-            if(!segment_free<K>(px, py, pz, x, y, z, sc))
+            if(!chk(px, py, pz, x, y, z))
                 {
                     valid = false;
                     break;
@@ -389,9 +435,8 @@ __device__ __forceinline__ bool prop_di(const QueryDesc& qd, float* s, const flo
 }
 
 // statePropagator.cu:147-241; FIVE_D: speed is control u[2], no acceleration.
-template <bool FIVE_D, int K = 1>
-__device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const float* u, int steps,
-                                            const Scene& sc)
+template <bool FIVE_D, class CHK>
+__device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const float* u, int steps, CHK& chk)
 {
     const kgmt_config& c = qd.cfg;
     const float h        = c.dt;
@@ -441,7 +486,7 @@ __device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const
                     valid = false;
                     break;
                 }
-            if(!segment_free<K>(px, py, pz, x, y, z, sc))
+            if(!chk(px, py, pz, x, y, z))
                 {
                     valid = false;
                     break;
@@ -481,9 +526,8 @@ __device__ __forceinline__ void quad_ode(const QueryDesc& qd, const float* st, c
 }
 
 // statePropagator.cu:246-323
-template <int K = 1>
-__device__ __forceinline__ bool prop_quad(const QueryDesc& qd, float* s, const float* uc, int steps,
-                                          const Scene& sc)
+template <class CHK>
+__device__ __forceinline__ bool prop_quad(const QueryDesc& qd, float* s, const float* uc, int steps, CHK& chk)
 {
     const kgmt_config& c = qd.cfg;
     const float h = c.dt, hh = 0.5f * h, h6 = h / 6.0f;
@@ -515,7 +559,7 @@ __device__ __forceinline__ bool prop_quad(const QueryDesc& qd, float* s, const f
                     valid = false;
                     break;
                 }
-            if(!segment_free<K>(px, py, pz, s[0], s[1], s[2], sc))
+            if(!chk(px, py, pz, s[0], s[1], s[2]))
                 {
                     valid = false;
                     break;
@@ -524,18 +568,88 @@ __device__ __forceinline__ bool prop_quad(const QueryDesc& qd, float* s, const f
     return valid;
 }
 
+template <int MODEL, class CHK>
+__device__ __forceinline__ bool propagate_with(const QueryDesc& qd, float* s, const float* u, int steps, CHK& chk)
+{
+    if constexpr(MODEL == KGMT_MODEL_DOUBLE_INTEGRATOR)
+        return prop_di(qd, s, u, steps, chk);
+    else if constexpr(MODEL == KGMT_MODEL_DUBINS_AIRPLANE)
+        return prop_dubins<false>(qd, s, u, steps, chk);
+    else if constexpr(MODEL == KGMT_MODEL_DUBINS_5D)
+        return prop_dubins<true>(qd, s, u, steps, chk);
+    else
+        return prop_quad(qd, s, u, steps, chk);
+}
+
+// Every box of the scene tested at every step (statePropagator.cu order:
+// integrate, bound checks, then the AABB test of the step's segment).
 template <int MODEL, int K = 1>
 __device__ __forceinline__ bool propagate(const QueryDesc& qd, float* s, const float* u, int steps,
                                           const Scene& sc)
 {
-    if constexpr(MODEL == KGMT_MODEL_DOUBLE_INTEGRATOR)
-        return prop_di<K>(qd, s, u, steps, sc);
-    else if constexpr(MODEL == KGMT_MODEL_DUBINS_AIRPLANE)
-        return prop_dubins<false, K>(qd, s, u, steps, sc);
-    else if constexpr(MODEL == KGMT_MODEL_DUBINS_5D)
-        return prop_dubins<true, K>(qd, s, u, steps, sc);
+    FullCheck<K> chk{sc};
+    return propagate_with<MODEL>(qd, s, u, steps, chk);
+}
+
+constexpr int kCand = 64;  // candidate boxes per wave of the two-phase check
+
+// Two-phase edge check with the same verdict and final state as propagate():
+// 1. integrate without boxes, recording the box T spanned by every segment the
+//    real run would test (the run stops at the same bound check);
+// 2. the wave's candidates: boxes overlapping some lane's T (a box that meets
+//    a segment overlaps the segment's box, which lies in T; group bounds
+//    contain their boxes), gathered into this wave's LDS list `cand`;
+// 3. integrate again (same operations, same bits) testing only candidates.
+// When a lane's positions are not finite, or the candidates exceed kCand, the
+// wave tests every box instead. Wave-uniform decisions only; call with the
+// lanes that propagate a child active.
+template <int MODEL>
+__device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* s, const float* u, int steps,
+                                                    const Scene& sc, float4* cand)
+{
+    constexpr int S = ModelDims<MODEL>::S;
+    DryRun dry;
+    dry.start(s[0], s[1], s[2]);
+    {
+        float s1[S];
+#pragma unroll
+        for(int d = 0; d < S; ++d) s1[d] = s[d];
+        (void)propagate_with<MODEL>(qd, s1, u, steps, dry);
+    }
+    int n = 0;
+    if(!__any(!dry.finite))
+        {
+            const float4* box = sc.grp + 2 * sc.nGrp;
+            for(int g = 0; g < sc.nGrp && n <= kCand; ++g)
+                {
+                    const bool ng = !separated(dry.lo[0], dry.lo[1], dry.lo[2], dry.hi[0], dry.hi[1], dry.hi[2],
+                                               sc.grp[2 * g], sc.grp[2 * g + 1]);
+                    if(!__any(ng)) continue;
+                    for(int m = 0; m < (int)kGroup; ++m)
+                        {
+                            const float4 lo = box[2 * (kGroup * g + m)], hi = box[2 * (kGroup * g + m) + 1];
+                            const bool c = ng && !separated(dry.lo[0], dry.lo[1], dry.lo[2], dry.hi[0], dry.hi[1],
+                                                            dry.hi[2], lo, hi);
+                            if(__any(c))
+                                {
+                                    if(n < kCand) cand[2 * n] = lo, cand[2 * n + 1] = hi;  // same value in every lane
+                                    ++n;
+                                }
+                        }
+                }
+        }
     else
-        return prop_quad<K>(qd, s, u, steps, sc);
+        n = kCand + 1;
+    if(n > kCand)
+        {
+            FullCheck<1> chk{sc};
+            return propagate_with<MODEL>(qd, s, u, steps, chk);
+        }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    CandCheck chk{cand, n};
+    return propagate_with<MODEL>(qd, s, u, steps, chk);
 }
 
 // -------------------------------------------------------------------------

@@ -254,6 +254,7 @@ struct TileLds
     int32_t steps[NT];
     float ufront[NT];
     uint32_t sbuf[NT / 64 + 1];
+    float4 cand[NT / 64][2 * kCand];  // per-wave candidate boxes (two-phase check)
 };
 
 // One tile of TC = NT / K children u = base .. base + TC - 1 of a query (TC
@@ -327,7 +328,10 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
             const int32_t steps = L.steps[i];
             const float uf      = L.ufront[i];
             const Scene sc{sObs, (int)qd.nGrp, (int)(t % K)};
-            ok = propagate<MODEL, K>(qd, s, uc, steps, sc);
+            if constexpr(K == 1 && MODEL != KGMT_MODEL_QUADCOPTER)
+                ok = propagate_two_phase<MODEL>(qd, s, uc, steps, sc, L.cand[t / 64]);
+            else
+                ok = propagate<MODEL, K>(qd, s, uc, steps, sc);
             if(lead)
                 {
                     float* U = P.childS + qd.sampleOff;
