@@ -133,8 +133,9 @@ typedef struct kgmt_result {
 
 /* Per-kernel device time accumulated while profiling is on (HIP events
  * around every launch of the plain, non-graph iteration loop). Kernel order:
- * 0 propagate, 1 score, 2 append, 3 expand (which also closes the iteration). */
-#define KGMT_PROFILE_KERNELS 4
+ * 0 propagate, 1 score, 2 append, 3 expand, 4 finish (closes the iteration of
+ * every query and builds the next iteration's work lists). */
+#define KGMT_PROFILE_KERNELS 5
 typedef struct kgmt_profile {
     uint64_t launches[KGMT_PROFILE_KERNELS];
     double   ms[KGMT_PROFILE_KERNELS];

@@ -147,8 +147,8 @@ class KgmtAdjointParams(C.Structure):
     ]
 
 
-KGMT_PROFILE_KERNELS = 4
-PROFILE_KERNEL_NAMES = ("propagate", "score", "append", "expand")
+KGMT_PROFILE_KERNELS = 5
+PROFILE_KERNEL_NAMES = ("propagate", "score", "append", "expand", "finish")
 
 
 class KgmtProfile(C.Structure):

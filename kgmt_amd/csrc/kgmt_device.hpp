@@ -21,6 +21,7 @@ namespace kgmt
 {
 
 constexpr uint32_t kChunk   = 1024;  // elements per bookkeeping chunk (one workgroup job)
+constexpr uint32_t kNodeAlign = 16;  // per-query node ranges start 16-aligned (vector loads of 4 flags)
 constexpr uint32_t kTile    = 256;   // default children per propagate tile (128, 256 or 512)
 constexpr uint32_t kBlock   = 256;   // threads of chunk / tile workgroups
 constexpr uint32_t kMaxObs  = 1500;  // obstacles a query may carry

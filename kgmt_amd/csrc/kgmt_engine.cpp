@@ -76,7 +76,8 @@ int check_config(const kgmt_config* c)
     if(c->children_per_entry < 1 || c->children_per_entry > 1024)
         return fail(KGMT_E_ARG, "children_per_entry must be in [1, 1024]");
     if(c->root_repeat < 1) return fail(KGMT_E_ARG, "root_repeat must be >= 1");
-    if(c->repeat_sparse > 63 || c->repeat_dense > 63) return fail(KGMT_E_ARG, "repeat counts must be <= 63");
+    if(c->repeat_sparse < 1 || c->repeat_sparse > 63 || c->repeat_dense < 1 || c->repeat_dense > 63)
+        return fail(KGMT_E_ARG, "repeat counts must be 1..63");
     if(c->model == KGMT_MODEL_QUADCOPTER &&
        (!(c->quad_mass > 0.0f) || !(c->quad_ix > 0.0f) || !(c->quad_iy > 0.0f) || !(c->quad_iz > 0.0f)))
         return fail(KGMT_E_ARG, "quad mass and inertia must be positive");
@@ -285,7 +286,7 @@ struct Engine
                 qd.subOff    = subs;
                 qd.pathOff   = (uint32_t)paths;
                 samples += (uint64_t)qd.D * qd.cap;
-                nodes += qd.cap;
+                nodes += ((uint64_t)qd.cap + kNodeAlign - 1) / kNodeAlign * kNodeAlign;
                 chunks += qd.nChunks + 1;
                 regs += (uint32_t)qd.R1;
                 subs += (uint32_t)qd.R2words;
@@ -381,6 +382,8 @@ struct Engine
         if(marks) HIP_TRY(hipEventRecord(marks[3], stream));
         HIP_TRY(launch_expand(dq, ds, P, W, parity, L, stream));
         if(marks) HIP_TRY(hipEventRecord(marks[4], stream));
+        HIP_TRY(launch_finish(dq, ds, W, parity, stream));
+        if(marks) HIP_TRY(hipEventRecord(marks[5], stream));
         return KGMT_OK;
     }
 
@@ -487,6 +490,7 @@ struct Engine
                             }
                     }
             }
+        HIP_TRY(launch_paths(dq, ds, P, nq, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         itersRun = std::min(it, maxIter);
         HIP_TRY(hipMemcpy(hs.data(), ds, sizeof(QueryState) * nq, hipMemcpyDeviceToHost));

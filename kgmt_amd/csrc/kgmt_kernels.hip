@@ -443,43 +443,82 @@ __global__ __launch_bounds__(kScoreThreads) void k_score(const QueryDesc* __rest
 // append: child chunks (compaction + tree append + goal) and old-node chunks
 // (re-activation)
 // ---------------------------------------------------------------------------
+// Each thread of a chunk job handles kPer consecutive slots, so one block
+// scan covers the chunk and the flag bytes come in one 4-byte load (node
+// ranges are kNodeAlign-aligned and padded, so the load stays inside the
+// query's range even past its capacity).
+constexpr uint32_t kPer = kChunk / kBlock;
+static_assert(kPer == 4, "four slots per thread");
+
 __device__ void append_children(const QueryDesc& qd, QueryState& st, uint32_t c, const Pools& P, uint32_t* sbuf)
 {
     const uint32_t cap = qd.cap, nC = st.nChildren, tree = st.treeSize, next = st.next;
-    const size_t nodeO = qd.nodeOff;
-    const float* U     = P.childS + qd.sampleOff;
-    float* T           = P.treeS + qd.sampleOff;
-    const uint32_t offA = P.partA[qd.chunkOff + c];
-    uint32_t run = 0, repSum = 0;
-    for(uint32_t k = 0; k < kChunk / kBlock; ++k)
+    const size_t nodeO     = qd.nodeOff;
+    const float* __restrict__ U = P.childS + qd.sampleOff;
+    float* __restrict__ T       = P.treeS + qd.sampleOff;
+    const int D            = qd.D;
+    const uint32_t u0      = c * kChunk + kPer * threadIdx.x;
+    uint32_t fl            = 0;
+    if(u0 < nC)
         {
-            const uint32_t u = c * kChunk + k * kBlock + threadIdx.x;
-            const bool in    = u < nC;
-            const bool f     = in && P.fnext[nodeO + u] != 0;
-            uint32_t tot;
-            const uint32_t ex   = block_excl_scan<kBlock>(f ? 1u : 0u, sbuf, tot);
-            const uint32_t rank = offA + run + ex;
-            uint32_t rep        = 0;
-            if(f && rank < next)
-                {
-                    const uint32_t t = tree + rank;
-                    for(int d = 0; d < qd.D; ++d) T[(size_t)d * cap + t] = U[(size_t)d * cap + u];
-                    P.parent[nodeO + t]   = P.parentU[nodeO + u];
-                    const float dg        = goal_dist(qd, U[u], U[(size_t)cap + u], U[2 * (size_t)cap + u]);
-                    P.cost[nodeO + t]     = dg;
-                    P.frontier[nodeO + t] = 1;
-                    const int r1          = P.r1child[nodeO + u];
-                    P.r1node[nodeO + t]   = r1;
-                    rep = P.validc[qd.regOff + r1] < qd.cfg.sparse_valid_below ? qd.cfg.repeat_sparse
-                                                                                : qd.cfg.repeat_dense;
-                    if(dg < qd.cfg.goal_thresh) atomicMin(&st.goalIdx, (int32_t)t);
-                }
-            if(in) P.repU[nodeO + u] = rep;
-            repSum += rep;
-            run += tot;
+            const uint32_t w = *reinterpret_cast<const uint32_t*>(P.fnext + nodeO + u0);
+#pragma unroll
+            for(uint32_t k = 0; k < kPer; ++k)
+                if(u0 + k < nC && ((w >> (8 * k)) & 0xffu)) fl |= 1u << k;
         }
     uint32_t tot;
-    block_excl_scan<kBlock>(repSum, sbuf, tot);
+    uint32_t rank = P.partA[qd.chunkOff + c] + block_excl_scan<kBlock>((uint32_t)__popc(fl), sbuf, tot);
+    // appended slots: u0 + k with flag k set and rank < next
+    uint32_t app = 0, t0 = 0;
+#pragma unroll
+    for(uint32_t k = 0; k < kPer; ++k)
+        if((fl >> k) & 1u)
+            {
+                if(rank < next)
+                    {
+                        if(!app) t0 = tree + rank;
+                        app |= 1u << k;
+                    }
+                ++rank;
+            }
+    uint32_t rep[kPer] = {0u, 0u, 0u, 0u};
+    if(app)
+        {
+            // all loads of the (at most four) children first, then the stores
+            float v[kPer][KGMT_MAX_SAMPLE];
+            int32_t par[kPer], r1[kPer];
+#pragma unroll
+            for(uint32_t k = 0; k < kPer; ++k)
+                if((app >> k) & 1u)
+                    {
+                        const uint32_t u = u0 + k;
+#pragma unroll
+                        for(int d = 0; d < KGMT_MAX_SAMPLE; ++d)
+                            if(d < D) v[k][d] = U[(size_t)d * cap + u];
+                        par[k] = P.parentU[nodeO + u];
+                        r1[k]  = P.r1child[nodeO + u];
+                    }
+            uint32_t t = t0;
+#pragma unroll
+            for(uint32_t k = 0; k < kPer; ++k)
+                if((app >> k) & 1u)
+                    {
+#pragma unroll
+                        for(int d = 0; d < KGMT_MAX_SAMPLE; ++d)
+                            if(d < D) T[(size_t)d * cap + t] = v[k][d];
+                        P.parent[nodeO + t]   = par[k];
+                        const float dg        = goal_dist(qd, v[k][0], v[k][1], v[k][2]);
+                        P.cost[nodeO + t]     = dg;
+                        P.frontier[nodeO + t] = 1;
+                        P.r1node[nodeO + t]   = r1[k];
+                        rep[k] = P.validc[qd.regOff + r1[k]] < qd.cfg.sparse_valid_below ? qd.cfg.repeat_sparse
+                                                                                         : qd.cfg.repeat_dense;
+                        if(dg < qd.cfg.goal_thresh) atomicMin(&st.goalIdx, (int32_t)t);
+                        ++t;
+                    }
+        }
+    if(u0 < nC) *reinterpret_cast<uint4*>(P.repU + nodeO + u0) = make_uint4(rep[0], rep[1], rep[2], rep[3]);
+    block_excl_scan<kBlock>(rep[0] + rep[1] + rep[2] + rep[3], sbuf, tot);
     if(threadIdx.x == 0) P.partC[qd.chunkOff + c] = tot;
 }
 
@@ -489,25 +528,35 @@ __device__ void reactivate(const QueryDesc& qd, QueryState& st, uint32_t o, cons
     const uint32_t tree = st.treeSize, itr = st.itr;
     const float fA      = st.fAccept;
     const size_t nodeO  = qd.nodeOff;
+    const uint32_t i0   = o * kChunk + kPer * threadIdx.x;
     uint32_t cnt        = 0;
-    for(uint32_t k = 0; k < kChunk / kBlock; ++k)
+    if(i0 < tree)
         {
-            const uint32_t i = o * kChunk + k * kBlock + threadIdx.x;
-            if(i < tree)
+            const uint32_t fw = *reinterpret_cast<const uint32_t*>(P.frontier + nodeO + i0);
+            const int4 rr     = *reinterpret_cast<const int4*>(P.r1node + nodeO + i0);
+            const int32_t r[kPer] = {rr.x, rr.y, rr.z, rr.w};
+            float sc[kPer];
+#pragma unroll
+            for(uint32_t k = 0; k < kPer; ++k)
                 {
-                    bool fl = false;
-                    if(P.frontier[nodeO + i] == 0)
-                        {
-                            const float Ur = draw_react(qd.cfg, qd.cfg.query_id, itr, i);
-                            if(Ur <= P.score[qd.regOff + P.r1node[nodeO + i]] + fA)
-                                {
-                                    fl                    = true;
-                                    P.frontier[nodeO + i] = 1;
-                                }
-                        }
-                    P.react[nodeO + i] = fl ? 1 : 0;
-                    cnt += fl ? 1u : 0u;
+                    const bool cand = i0 + k < tree && ((fw >> (8 * k)) & 0xffu) == 0;
+                    sc[k]           = cand ? P.score[qd.regOff + r[k]] : 0.0f;
                 }
+            uint32_t rw = 0;
+#pragma unroll
+            for(uint32_t k = 0; k < kPer; ++k)
+                {
+                    const bool cand = i0 + k < tree && ((fw >> (8 * k)) & 0xffu) == 0;
+                    if(cand && draw_react(qd.cfg, qd.cfg.query_id, itr, i0 + k) <= sc[k] + fA)
+                        {
+                            // byte store: the new nodes' flags past the tree are
+                            // written concurrently by the child chunk jobs
+                            P.frontier[nodeO + i0 + k] = 1;
+                            rw |= 1u << (8 * k);
+                            ++cnt;
+                        }
+                }
+            *reinterpret_cast<uint32_t*>(P.react + nodeO + i0) = rw;  // bytes past the tree read as 0
         }
     uint32_t tot;
     block_excl_scan<kBlock>(cnt, sbuf, tot);
@@ -518,7 +567,6 @@ __global__ __launch_bounds__(kBlock) void k_append(const QueryDesc* __restrict__
                                                    Pools P, WorkLists W, uint32_t parity)
 {
     __shared__ uint32_t sbuf[kBlock / 64 + 1];
-    __shared__ uint32_t sflag;
     const uint32_t nq       = W.nq;
     const uint32_t* jobList = W.jobs(parity);
     const uint32_t nJobs    = jobList[nq];
@@ -529,24 +577,11 @@ __global__ __launch_bounds__(kBlock) void k_append(const QueryDesc* __restrict__
             if(st.done) continue;  // stale list entry of a finished query
             const QueryDesc& qd  = qds[q];
             const uint32_t local = j - jobList[q];
-            const uint32_t jobsQ = jobList[q + 1] - jobList[q];
             const uint32_t nCC   = (st.nChildren + kChunk - 1) / kChunk;
             if(local < nCC)
                 append_children(qd, st, local, P, sbuf);
             else
                 reactivate(qd, st, local - nCC, P, sbuf);
-            if(last_arrival(&st.jobsDone, jobsQ, &sflag))
-                {
-                    const uint32_t nOC    = (st.treeSize + kChunk - 1) / kChunk;
-                    const uint32_t nReact = block_scan_array<kBlock>(P.partB + qd.chunkOff, nOC, sbuf);
-                    const uint32_t sumRep = block_scan_array<kBlock>(P.partC + qd.chunkOff, nCC, sbuf);
-                    if(threadIdx.x == 0)
-                        {
-                            st.nReact   = nReact;
-                            st.R        = nReact + sumRep;
-                            st.jobsDone = 0;
-                        }
-                }
             __syncthreads();
         }
 }
@@ -555,45 +590,16 @@ __global__ __launch_bounds__(kBlock) void k_append(const QueryDesc* __restrict__
 // expand: next expansion list (re-activated ascending, then new nodes each
 // repeated repU times), truncated at capacity entries
 // ---------------------------------------------------------------------------
-// End of an iteration for one query (all threads of the workgroup; thread 0
-// does the bookkeeping): tree size, goal path, stop rules, next regime.
-__device__ void finish_query(const QueryDesc& qd, QueryState& st, const Pools& P)
+// End of an iteration for one query (one thread): tree size, stop rules,
+// next launch regime. The goal path is walked once the plan has ended
+// (k_paths).
+__device__ void finish_query(const QueryDesc& qd, QueryState& st)
 {
-    for(uint32_t c = threadIdx.x; c <= qd.nChunks; c += kBlock) P.partA[qd.chunkOff + c] = 0u;
-    if(threadIdx.x != 0) return;
     const uint32_t tree = st.treeSize + st.next;
     st.treeSize         = tree;
     st.expanded += st.nChildren;
-    st.jobsDone = 0;
     if(st.goalIdx != kNoGoal)
         {
-            // goal -> root rows, exact depth (Q13)
-            const uint32_t cap = qd.cap;
-            const int D        = qd.D;
-            const float* T     = P.treeS + qd.sampleOff;
-            float* out         = P.path + qd.pathOff;
-            int32_t n          = st.goalIdx;
-            uint32_t rows      = 0;
-            float pc = 0.0f, pl = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
-            const uint32_t maxRows = qd.cfg.max_iter + 2;
-            while(n != -1 && rows < maxRows)
-                {
-                    for(int d = 0; d < D; ++d) out[rows * D + d] = T[(size_t)d * cap + n];
-                    const float x = T[n], y = T[(size_t)cap + n], z = T[2 * (size_t)cap + n];
-                    const int32_t par = P.parent[qd.nodeOff + n];
-                    if(par != -1) pc = pc + T[(size_t)(D - 1) * cap + n];
-                    if(rows > 0)
-                        {
-                            const float dx = px - x, dy = py - y, dz = pz - z;
-                            pl             = pl + __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
-                        }
-                    px = x, py = y, pz = z;
-                    ++rows;
-                    n = par;
-                }
-            st.pathRows  = rows;
-            st.pathCost  = pc;
-            st.pathLen   = pl;
             st.success   = 1;
             st.done      = 1;
             st.finishItr = st.itr;
@@ -614,20 +620,30 @@ __device__ void finish_query(const QueryDesc& qd, QueryState& st, const Pools& P
         }
 }
 
+// sum of a[0..n) over the workgroup
+__device__ __forceinline__ uint32_t block_sum(const uint32_t* a, uint32_t n, uint32_t* sbuf)
+{
+    uint32_t v = 0;
+    for(uint32_t i = threadIdx.x; i < n; i += kBlock) v += a[i];
+    uint32_t tot;
+    block_excl_scan<kBlock>(v, sbuf, tot);
+    return tot;
+}
+
 // ---------------------------------------------------------------------------
 // expand: next expansion list (re-activated ascending, then new nodes each
-// repeated repU times), truncated at capacity entries; the last job of a query
-// closes its iteration, the last query to close rebuilds the work lists.
+// repeated repU times), truncated at capacity entries. Each job sums the
+// partial counts before its chunk itself (no communication between
+// workgroups inside the kernel); the first job of a query also records the
+// list length R.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
                                                    Pools P, WorkLists W, uint32_t parity)
 {
     __shared__ uint32_t sbuf[kBlock / 64 + 1];
-    __shared__ uint32_t sflag;
     const uint32_t nq       = W.nq;
     const uint32_t* jobList = W.jobs(parity);
     const uint32_t nJobs    = jobList[nq];
-    const uint32_t active   = W.active[parity];
     for(uint32_t j = blockIdx.x; j < nJobs; j += gridDim.x)
         {
             const uint32_t q      = locate(jobList, nq, j);
@@ -635,58 +651,131 @@ __global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__
             if(st.done) continue;  // stale list entry of a finished query
             const QueryDesc& qd   = qds[q];
             const uint32_t local  = j - jobList[q];
-            const uint32_t jobsQ  = jobList[q + 1] - jobList[q];
             const uint32_t nC     = st.nChildren;
             const uint32_t nCC    = (nC + kChunk - 1) / kChunk;
             const uint32_t cap    = qd.cap, tree = st.treeSize;
+            const uint32_t nOC    = (tree + kChunk - 1) / kChunk;
             const size_t nodeO    = qd.nodeOff;
             uint32_t* out         = P.repIdx + nodeO;
+            const uint32_t* pB    = P.partB + qd.chunkOff;
+            const uint32_t* pC    = P.partC + qd.chunkOff;
+            if(local == 0)
+                {
+                    const uint32_t R = block_sum(pB, nOC, sbuf) + block_sum(pC, nCC, sbuf);
+                    if(threadIdx.x == 0) st.R = R;
+                }
             if(local < nCC)
                 {
+                    // new nodes of chunk `local`, each repeated repU times
                     const uint32_t c    = local;
+                    const uint32_t u0   = c * kChunk + kPer * threadIdx.x;
                     const uint32_t offA = P.partA[qd.chunkOff + c];
-                    const uint32_t offC = P.partC[qd.chunkOff + c] + st.nReact;
-                    uint32_t runN = 0, runS = 0;
-                    for(uint32_t k = 0; k < kChunk / kBlock; ++k)
+                    const uint32_t offC = block_sum(pB, nOC, sbuf) + block_sum(pC, c, sbuf);
+                    uint32_t rep[kPer] = {0u, 0u, 0u, 0u};
+                    if(u0 < nC)
                         {
-                            const uint32_t u   = c * kChunk + k * kBlock + threadIdx.x;
-                            const uint32_t rep = u < nC ? P.repU[nodeO + u] : 0u;
-                            uint32_t tot;
-                            const uint32_t ex = block_excl_scan<kBlock>((rep ? (1u << 16) : 0u) | rep, sbuf, tot);
-                            if(rep)
-                                {
-                                    const uint32_t t   = tree + offA + runN + (ex >> 16);
-                                    const uint32_t pos = offC + runS + (ex & 0xffffu);
-                                    for(uint32_t m = 0; m < rep; ++m)
-                                        if(pos + m < cap) out[pos + m] = t;
-                                }
-                            runN += tot >> 16;
-                            runS += tot & 0xffffu;
+                            const uint4 r4 = *reinterpret_cast<const uint4*>(P.repU + nodeO + u0);
+                            rep[0] = r4.x, rep[1] = r4.y, rep[2] = r4.z, rep[3] = r4.w;
+#pragma unroll
+                            for(uint32_t k = 1; k < kPer; ++k)
+                                if(u0 + k >= nC) rep[k] = 0u;
                         }
+                    uint32_t nn = 0, ns = 0;
+#pragma unroll
+                    for(uint32_t k = 0; k < kPer; ++k) nn += rep[k] ? 1u : 0u, ns += rep[k];
+                    uint32_t tot;
+                    const uint32_t ex = block_excl_scan<kBlock>((nn << 16) | ns, sbuf, tot);
+                    uint32_t t        = tree + offA + (ex >> 16);
+                    uint32_t pos      = offC + (ex & 0xffffu);
+#pragma unroll
+                    for(uint32_t k = 0; k < kPer; ++k)
+                        if(rep[k])
+                            {
+                                for(uint32_t m = 0; m < rep[k]; ++m)
+                                    if(pos + m < cap) out[pos + m] = t;
+                                pos += rep[k];
+                                ++t;
+                            }
+                    // chunk counts start at zero for the next propagate
+                    if(threadIdx.x == 0) P.partA[qd.chunkOff + c] = 0u;
                 }
             else
                 {
-                    const uint32_t o    = local - nCC;
-                    const uint32_t offB = P.partB[qd.chunkOff + o];
-                    uint32_t run        = 0;
-                    for(uint32_t k = 0; k < kChunk / kBlock; ++k)
-                        {
-                            const uint32_t i = o * kChunk + k * kBlock + threadIdx.x;
-                            const bool fl    = i < tree && P.react[nodeO + i] != 0;
-                            uint32_t tot;
-                            const uint32_t ex = block_excl_scan<kBlock>(fl ? 1u : 0u, sbuf, tot);
-                            const uint32_t pos = offB + run + ex;
-                            if(fl && pos < cap) out[pos] = i;
-                            run += tot;
-                        }
-                }
-            if(last_arrival(&st.jobsDone, jobsQ, &sflag))
-                {
-                    finish_query(qd, st, P);
-                    if(last_arrival(W.counter, active, &sflag)) build_worklists<kBlock>(sts, W, parity ^ 1u, sbuf);
+                    // re-activated old nodes of chunk `local - nCC`, ascending
+                    const uint32_t o  = local - nCC;
+                    const uint32_t i0 = o * kChunk + kPer * threadIdx.x;
+                    const uint32_t rw = i0 < tree ? *reinterpret_cast<const uint32_t*>(P.react + nodeO + i0) : 0u;
+                    uint32_t fl       = 0;
+#pragma unroll
+                    for(uint32_t k = 0; k < kPer; ++k)
+                        if(i0 + k < tree && ((rw >> (8 * k)) & 0xffu)) fl |= 1u << k;
+                    uint32_t tot;
+                    uint32_t pos = block_sum(pB, o, sbuf) + block_excl_scan<kBlock>((uint32_t)__popc(fl), sbuf, tot);
+#pragma unroll
+                    for(uint32_t k = 0; k < kPer; ++k)
+                        if((fl >> k) & 1u)
+                            {
+                                if(pos < cap) out[pos] = i0 + k;
+                                ++pos;
+                            }
                 }
             __syncthreads();
         }
+}
+
+// ---------------------------------------------------------------------------
+// finish: one workgroup closes the iteration of every query and builds the
+// next iteration's work lists
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_finish(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
+                                                   WorkLists W, uint32_t parity)
+{
+    __shared__ uint32_t sbuf[kBlock / 64 + 1];
+    for(uint32_t q = threadIdx.x; q < W.nq; q += kBlock)
+        if(!sts[q].done) finish_query(qds[q], sts[q]);
+    __syncthreads();
+    build_worklists<kBlock>(sts, W, parity ^ 1u, sbuf);
+}
+
+// ---------------------------------------------------------------------------
+// paths: goal -> root rows of every solved query, exact depth (Q13); path
+// cost = sum of edge durations, length = sum of segment lengths between the
+// node positions
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_paths(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
+                                              Pools P, uint32_t nq)
+{
+    const uint32_t q = blockIdx.x * 64 + threadIdx.x;
+    if(q >= nq) return;
+    QueryState& st = sts[q];
+    if(!st.success || st.pathRows) return;
+    const QueryDesc& qd = qds[q];
+    const uint32_t cap  = qd.cap;
+    const int D         = qd.D;
+    const float* T      = P.treeS + qd.sampleOff;
+    float* out          = P.path + qd.pathOff;
+    int32_t n           = st.goalIdx;
+    uint32_t rows       = 0;
+    float pc = 0.0f, pl = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
+    const uint32_t maxRows = qd.cfg.max_iter + 2;
+    while(n != -1 && rows < maxRows)
+        {
+            for(int d = 0; d < D; ++d) out[rows * D + d] = T[(size_t)d * cap + n];
+            const float x = T[n], y = T[(size_t)cap + n], z = T[2 * (size_t)cap + n];
+            const int32_t par = P.parent[qd.nodeOff + n];
+            if(par != -1) pc = pc + T[(size_t)(D - 1) * cap + n];
+            if(rows > 0)
+                {
+                    const float dx = px - x, dy = py - y, dz = pz - z;
+                    pl             = pl + __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+                }
+            px = x, py = y, pz = z;
+            ++rows;
+            n = par;
+        }
+    st.pathRows = rows;
+    st.pathCost = pc;
+    st.pathLen  = pl;
 }
 
 // ---------------------------------------------------------------------------
@@ -939,6 +1028,19 @@ hipError_t launch_append(const QueryDesc* qds, QueryState* sts, const Pools& P, 
                          uint32_t parity, const Launch& L, hipStream_t s)
 {
     hipLaunchKernelGGL(k_append, dim3(L.chunkGrid), dim3(kBlock), 0, s, qds, sts, P, W, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_finish(const QueryDesc* qds, QueryState* sts, const WorkLists& W, uint32_t parity, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), 0, s, qds, sts, W, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_paths(const QueryDesc* qds, QueryState* sts, const Pools& P, uint32_t nq, hipStream_t s)
+{
+    if(nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_paths, dim3((nq + 63) / 64), dim3(64), 0, s, qds, sts, P, nq);
     return hipGetLastError();
 }
 

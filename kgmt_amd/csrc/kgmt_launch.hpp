@@ -35,6 +35,9 @@ hipError_t launch_append(const QueryDesc* qds, QueryState* sts, const Pools& P, 
                          uint32_t parity, const Launch& L, hipStream_t s);
 hipError_t launch_expand(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
                          uint32_t parity, const Launch& L, hipStream_t s);
+hipError_t launch_finish(const QueryDesc* qds, QueryState* sts, const WorkLists& W, uint32_t parity, hipStream_t s);
+// goal paths of the solved queries, once a plan has ended
+hipError_t launch_paths(const QueryDesc* qds, QueryState* sts, const Pools& P, uint32_t nq, hipStream_t s);
 
 // stage-level kernels (one query, caller buffers)
 hipError_t launch_stage_draw(const QueryDesc* qd, int C, uint32_t query, uint32_t itr, uint32_t u0, uint32_t n,

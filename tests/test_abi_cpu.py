@@ -59,6 +59,8 @@ def test_config_validation():
         dict(r1_len=0),
         dict(r1_len=21),
         dict(repeat_sparse=64),
+        dict(repeat_sparse=0),
+        dict(repeat_dense=0),
         dict(children_per_entry=0),
         dict(max_iter=0),
     ]
