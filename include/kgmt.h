@@ -242,6 +242,15 @@ int kgmt_batch_get_profile(kgmt_batch* b, kgmt_profile* out);
  * `query`: controls [C][n], steps [n], frontier uniform [n]. */
 int kgmt_draw_controls(const kgmt_config* cfg, uint32_t query, uint32_t itr, uint32_t u0, uint32_t n,
                        float* d_controls, int32_t* d_steps, float* d_ufront, void* stream);
+/* Obstacle scene as the propagate kernels read it, built from n rows of 6
+ * floats (xmin ymin zmin xmax ymax zmax, collisionCheck.cu:53-58): boxes in
+ * Morton order of their centres in groups of 8, n_groups group bounds first;
+ * each entry is a (lo, hi) pair of 4-float vectors, so the scene takes
+ * 8 * n_groups * 9 floats. Returns KGMT_E_CAPACITY (n_groups still set) when
+ * out_floats is too small. No counterpart in the reference (it tests every
+ * box in input order). */
+int kgmt_build_scene(const float* d_rows, uint32_t n, float* d_out, uint32_t out_floats, uint32_t* n_groups,
+                     void* stream);
 /* Propagate + collide: child i starts from parent_states[:, parent_idx[i]]
  * ([S][n_parents]), applies controls[:, i] for steps[i] steps. Writes child
  * samples [D][n] (state, control, duration), valid[n], r1[n], r2[n]. */

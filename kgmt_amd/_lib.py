@@ -63,6 +63,7 @@ SIGNATURES = {
     "kgmt_score": (C.c_int, [_cfgp, _P, _P, _P, _P, _P]),
     "kgmt_compact": (C.c_int, [_P, _u32, _P, _P, _P]),
     "kgmt_repeat": (C.c_int, [_P, _P, _u32, _P, _u32, _P, _P]),
+    "kgmt_build_scene": (C.c_int, [_P, _u32, _P, _u32, C.POINTER(_u32), _P]),
     "kgmt_rk4_adjoint": (C.c_int, [_cfgp, C.POINTER(KgmtAdjointParams), _u32, _u32, _P, _P, _P, _P, _P, _u32, _P, _P,
                                    _P]),
     "kgmt_refine": (C.c_int, [_cfgp, C.POINTER(KgmtAdjointParams), _u32, _u32, _P, _P, _P, _P, _P, _u32, _P, _P]),

@@ -51,7 +51,8 @@ struct QueryDesc
     float goal[3];
     float start[KGMT_MAX_SAMPLE];
     float qminv, qax, qbx, qay, qby, qaz, qbz;  // quad: 1/m, (Iy-Iz)/Ix, 1/Ix, ...
-    uint32_t pad[2];
+    uint32_t rowOff;                 // first obstacle row (6 floats) of the query in the rows the scene is built from
+    uint32_t pad;
 };
 
 struct QueryState

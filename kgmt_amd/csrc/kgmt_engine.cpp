@@ -109,25 +109,6 @@ void fill_desc(QueryDesc& qd, const kgmt_config& c)
     qd.qaz = (c.quad_ix - c.quad_iy) / c.quad_iz, qd.qbz = 1.0f / c.quad_iz;
 }
 
-// groups of a scene of n boxes: whole batches of kGroupBatch groups
-uint32_t scene_groups(uint32_t n)
-{
-    const uint32_t per = kGroup * kGroupBatch;
-    return (n + per - 1) / per * kGroupBatch;
-}
-
-uint32_t scene_pairs(uint32_t n)
-{
-    return scene_groups(n) * (1 + kGroup);
-}
-
-uint32_t morton10(uint32_t x, uint32_t y, uint32_t z)
-{
-    uint32_t code = 0;
-    for(int b = 9; b >= 0; --b) code = (code << 3) | (((x >> b) & 1u) << 2) | (((y >> b) & 1u) << 1) | ((z >> b) & 1u);
-    return code;
-}
-
 template <typename T>
 int dev_alloc(T*& p, size_t count, std::vector<void*>& owned)
 {
@@ -145,73 +126,46 @@ constexpr int kLag  = 2;  // launch units the host stays behind the GPU
 
 }  // namespace
 
-uint32_t kgmt::build_scene(const float* rows, uint32_t n, float4* out)
+// groups of a scene of n boxes: whole batches of kGroupBatch groups
+uint32_t kgmt::scene_groups(uint32_t n)
 {
-    if(n == 0) return 0;
-    const uint32_t nGrp = scene_groups(n);
-    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for(uint32_t i = 0; i < n; ++i)
-        for(int d = 0; d < 3; ++d)
-            {
-                const float c = 0.5f * (rows[6 * (size_t)i + d] + rows[6 * (size_t)i + 3 + d]);
-                if(std::isfinite(c)) lo[d] = std::min(lo[d], c), hi[d] = std::max(hi[d], c);
-            }
-    std::vector<std::pair<uint32_t, uint32_t>> key(n);
-    for(uint32_t i = 0; i < n; ++i)
-        {
-            uint32_t q[3];
-            for(int d = 0; d < 3; ++d)
-                {
-                    const float c = 0.5f * (rows[6 * (size_t)i + d] + rows[6 * (size_t)i + 3 + d]);
-                    float t       = hi[d] > lo[d] ? (c - lo[d]) / (hi[d] - lo[d]) : 0.0f;
-                    if(!(t >= 0.0f)) t = 0.0f;
-                    if(t > 1.0f) t = 1.0f;
-                    q[d] = (uint32_t)(t * 1023.0f);
-                }
-            key[i] = {morton10(q[0], q[1], q[2]), i};
-        }
-    std::sort(key.begin(), key.end());
-    // boxes in Morton order, then never-hit dummies up to nGrp * kGroup
-    float4* box = out + 2 * nGrp;
-    for(uint32_t k = 0; k < nGrp * kGroup; ++k)
-        {
-            if(k < n)
-                {
-                    const float* r = rows + 6 * (size_t)key[k].second;
-                    box[2 * k]     = make_float4(r[0], r[1], r[2], 0.0f);
-                    box[2 * k + 1] = make_float4(r[3], r[4], r[5], 0.0f);
-                }
-            else
-                {
-                    box[2 * k]     = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
-                    box[2 * k + 1] = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
-                }
-        }
-    for(uint32_t g = 0; g < nGrp; ++g)
-        {
-            float gl[3] = {INFINITY, INFINITY, INFINITY}, gh[3] = {-INFINITY, -INFINITY, -INFINITY};
-            bool nanAxis[3] = {false, false, false};
-            const uint32_t b1 = std::min(n, (g + 1) * kGroup);
-            for(uint32_t k = g * kGroup; k < b1; ++k)
-                {
-                    const float l[3] = {box[2 * k].x, box[2 * k].y, box[2 * k].z};
-                    const float h[3] = {box[2 * k + 1].x, box[2 * k + 1].y, box[2 * k + 1].z};
-                    for(int d = 0; d < 3; ++d)
-                        {
-                            if(std::isnan(l[d]) || std::isnan(h[d])) nanAxis[d] = true;
-                            gl[d] = std::min(gl[d], l[d]);
-                            gh[d] = std::max(gh[d], h[d]);
-                        }
-                }
-            // a NaN member makes its group unskippable on that axis; a group of
-            // dummies only keeps lo = +inf, hi = -inf and is never near
-            for(int d = 0; d < 3; ++d)
-                if(nanAxis[d]) gl[d] = gh[d] = NAN;
-            out[2 * g]     = make_float4(gl[0], gl[1], gl[2], 0.0f);
-            out[2 * g + 1] = make_float4(gh[0], gh[1], gh[2], 0.0f);
-        }
-    return nGrp;
+    const uint32_t per = kGroup * kGroupBatch;
+    return (n + per - 1) / per * kGroupBatch;
 }
+
+uint32_t kgmt::scene_pairs(uint32_t n)
+{
+    return scene_groups(n) * (1 + kGroup);
+}
+
+// Host array in pinned memory (asynchronous copies to and from the device).
+template <typename T>
+struct PinnedArray
+{
+    T* p     = nullptr;
+    size_t n = 0;
+    PinnedArray() = default;
+    PinnedArray(const PinnedArray&) = delete;
+    PinnedArray& operator=(const PinnedArray&) = delete;
+    ~PinnedArray()
+    {
+        if(p) (void)hipHostFree(p);
+    }
+    bool resize(size_t count)
+    {
+        if(p) (void)hipHostFree(p);
+        p = nullptr, n = 0;
+        if(hipHostMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault) != hipSuccess)
+            return false;
+        std::memset((void*)p, 0, std::max<size_t>(count, 1) * sizeof(T));
+        n = count;
+        return true;
+    }
+    T* data() { return p; }
+    const T* data() const { return p; }
+    T& operator[](size_t i) { return p[i]; }
+    const T& operator[](size_t i) const { return p[i]; }
+};
 
 // ---------------------------------------------------------------------------
 // Engine: device pools and the iteration loop for a table of queries
@@ -219,8 +173,8 @@ uint32_t kgmt::build_scene(const float* rows, uint32_t n, float4* out)
 struct Engine
 {
     uint32_t nq = 0;
-    std::vector<QueryDesc> hq;
-    std::vector<QueryState> hs;
+    PinnedArray<QueryDesc> hq;
+    PinnedArray<QueryState> hs;
     std::vector<void*> owned;
     QueryDesc* dq  = nullptr;
     QueryState* ds = nullptr;
@@ -228,6 +182,10 @@ struct Engine
     WorkLists W{};
     uint32_t obsCap = 0;     // (lo, hi) pairs the obstacle pool holds
     float4* obsPool = nullptr;
+    uint32_t rowsCap = 0;    // obstacle rows the staging buffer holds (host-provided obstacles)
+    float* rowsBuf   = nullptr;
+    const float* sceneRows = nullptr;  // device rows the scenes of the next run are built from
+    bool sceneBuild  = false;
     uint32_t maxPairs = 0;   // largest per-query scene (pairs) of the current plan
     int model       = -1;    // common model or -1
     uint32_t maxIter = 0;
@@ -268,8 +226,7 @@ struct Engine
         if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(KGMT_E_NODEVICE, "no HIP device visible");
         HIP_TRY(hipGetDevice(&device));
         nq = n;
-        hq.resize(n);
-        hs.resize(n);
+        if(!hq.resize(n) || !hs.resize(n)) return fail(KGMT_E_NOMEM, "pinned host allocation failed");
         uint64_t samples = 0, nodes = 0, paths = 0;
         uint32_t chunks = 0, regs = 0, subs = 0;
         model = cfgs[0].model;
@@ -370,7 +327,26 @@ struct Engine
         return KGMT_OK;
     }
 
-    // One iteration: 4 launches. marks: 5 timing events around them (profiling).
+    int ensure_rows(uint32_t rows)
+    {
+        if(rows <= rowsCap && rowsBuf) return KGMT_OK;
+        const uint32_t cap = std::max<uint32_t>(rows, 256);
+        float* p           = nullptr;
+        if(hipMalloc((void**)&p, (size_t)cap * 6 * sizeof(float)) != hipSuccess)
+            return fail(KGMT_E_NOMEM, "obstacle row buffer allocation failed");
+        if(rowsBuf)
+            {
+                auto it = std::find(owned.begin(), owned.end(), (void*)rowsBuf);
+                if(it != owned.end()) owned.erase(it);
+                HIP_TRY(hipStreamSynchronize(stream));
+                (void)hipFree(rowsBuf);
+            }
+        owned.push_back(p);
+        rowsBuf = p, rowsCap = cap;
+        return KGMT_OK;
+    }
+
+    // One iteration: 5 launches. marks: 5 timing events around them (profiling).
     int enqueue_iteration(uint32_t parity, hipEvent_t* marks = nullptr)
     {
         if(marks) HIP_TRY(hipEventRecord(marks[0], stream));
@@ -445,6 +421,7 @@ struct Engine
     {
         const auto t0 = std::chrono::steady_clock::now();
         HIP_TRY(hipMemcpyAsync(dq, hq.data(), sizeof(QueryDesc) * nq, hipMemcpyHostToDevice, stream));
+        if(sceneBuild) HIP_TRY(launch_build_scenes(dq, sceneRows, P.obs, nq, stream));
         HIP_TRY(hipMemsetAsync(W.counter, 0, sizeof(uint32_t), stream));
         __atomic_store_n(pinned, 0u, __ATOMIC_RELEASE);
         HIP_TRY(launch_init(dq, ds, P, W, stream));
@@ -491,9 +468,9 @@ struct Engine
                     }
             }
         HIP_TRY(launch_paths(dq, ds, P, nq, stream));
+        HIP_TRY(hipMemcpyAsync(hs.data(), ds, sizeof(QueryState) * nq, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         itersRun = std::min(it, maxIter);
-        HIP_TRY(hipMemcpy(hs.data(), ds, sizeof(QueryState) * nq, hipMemcpyDeviceToHost));
         if(prof) return collect_profile(it);
         return KGMT_OK;
     }
@@ -714,20 +691,25 @@ int kgmt_set_stream(kgmt_ctx* ctx, void* s)
 // Obstacle rows of every query -> scenes (grouped, Morton-sorted) in the pool.
 // rows may be device memory (the reference passes a device pointer); they are
 // read back once per plan (6 floats per box) to build the groups on the host.
+// The scene kernels read obstacle rows in place: reject a pointer that the
+// device cannot read (for example host memory passed as device memory).
+static int check_device_rows(const float* p)
+{
+    hipPointerAttribute_t attr{};
+    if(hipPointerGetAttributes(&attr, p) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            return fail(KGMT_E_ARG, "obstacle pointer is not device memory");
+        }
+    if(attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged && attr.type != hipMemoryTypeUnified)
+        return fail(KGMT_E_ARG, "obstacle pointer is not device memory");
+    return KGMT_OK;
+}
+
 static int upload_obstacles(Engine& e, const float* obstacles, const uint32_t* offsets, const uint32_t* counts,
                             uint32_t total, int onDevice)
 {
     if(total && !obstacles) return fail(KGMT_E_ARG, "obstacles pointer is null");
-    std::vector<float> hostRows;
-    const float* rows = obstacles;
-    if(onDevice && total)
-        {
-            hostRows.resize((size_t)total * 6);
-            HIP_TRY(hipMemcpyAsync(hostRows.data(), obstacles, hostRows.size() * sizeof(float), hipMemcpyDeviceToHost,
-                                   e.stream));
-            HIP_TRY(hipStreamSynchronize(e.stream));
-            rows = hostRows.data();
-        }
     uint32_t pairs = 0;
     for(uint32_t q = 0; q < e.nq; ++q)
         {
@@ -739,7 +721,25 @@ static int upload_obstacles(Engine& e, const float* obstacles, const uint32_t* o
         }
     int rc = e.ensure_obstacles(pairs);
     if(rc) return rc;
-    std::vector<float4> packed((size_t)std::max<uint32_t>(pairs, 1) * 2);
+    // scenes are built on the device (run()) from the caller's device rows or
+    // from a device copy of the host rows
+    e.sceneRows  = nullptr;
+    e.sceneBuild = total != 0;
+    if(total)
+        {
+            if(onDevice)
+                {
+                    if((rc = check_device_rows(obstacles))) return rc;
+                    e.sceneRows = obstacles;
+                }
+            else
+                {
+                    if((rc = e.ensure_rows(total))) return rc;
+                    HIP_TRY(hipMemcpyAsync(e.rowsBuf, obstacles, (size_t)total * 6 * sizeof(float),
+                                           hipMemcpyHostToDevice, e.stream));
+                    e.sceneRows = e.rowsBuf;
+                }
+        }
     uint32_t p = 0;
     e.maxPairs = 0;
     for(uint32_t q = 0; q < e.nq; ++q)
@@ -749,15 +749,10 @@ static int upload_obstacles(Engine& e, const float* obstacles, const uint32_t* o
             QueryDesc& qd      = e.hq[q];
             qd.obsOff          = p;
             qd.nObs            = c;
-            qd.nGrp            = build_scene(rows + 6 * (size_t)off, c, packed.data() + 2 * (size_t)p);
+            qd.nGrp            = scene_groups(c);
+            qd.rowOff          = off;
             p += scene_pairs(c);
             e.maxPairs = std::max(e.maxPairs, scene_pairs(c));
-        }
-    if(pairs)
-        {
-            HIP_TRY(hipMemcpyAsync(e.obsPool, packed.data(), (size_t)pairs * 2 * sizeof(float4), hipMemcpyHostToDevice,
-                                   e.stream));
-            HIP_TRY(hipStreamSynchronize(e.stream));
         }
     return KGMT_OK;
 }
@@ -957,6 +952,22 @@ int kgmt_draw_controls(const kgmt_config* cfg, uint32_t query, uint32_t itr, uin
     return KGMT_OK;
 }
 
+int kgmt_build_scene(const float* d_rows, uint32_t n, float* d_out, uint32_t out_floats, uint32_t* n_groups,
+                     void* stream)
+{
+    if(!n_groups) return fail(KGMT_E_ARG, "null argument");
+    if(n > kMaxObs) return fail(KGMT_E_CAPACITY, "more than 1500 obstacles");
+    const uint32_t nGrp = scene_groups(n);
+    *n_groups           = nGrp;
+    if(n == 0) return KGMT_OK;
+    if(!d_rows || !d_out) return fail(KGMT_E_ARG, "null buffer");
+    if((uint64_t)out_floats < (uint64_t)scene_pairs(n) * 8) return fail(KGMT_E_CAPACITY, "scene buffer too small");
+    int rc = check_device_rows(d_rows);
+    if(rc) return rc;
+    HIP_TRY(launch_build_scene(d_rows, n, nGrp, reinterpret_cast<float4*>(d_out), (hipStream_t)stream));
+    return KGMT_OK;
+}
+
 int kgmt_propagate_collide(const kgmt_config* cfg, const int32_t* d_parent_idx, const float* d_parent_states,
                            uint32_t n_parents, const float* d_controls, const int32_t* d_steps, uint32_t n_children,
                            const float* d_obstacles, uint32_t n_obs, float* d_child_samples, uint8_t* d_valid,
@@ -970,22 +981,16 @@ int kgmt_propagate_collide(const kgmt_config* cfg, const int32_t* d_parent_idx, 
     StageDesc sd((hipStream_t)stream);
     int rc = sd.make(cfg);
     if(rc) return rc;
-    std::vector<float> rows((size_t)n_obs * 6);
-    if(n_obs)
-        {
-            HIP_TRY(hipMemcpyAsync(rows.data(), d_obstacles, rows.size() * sizeof(float), hipMemcpyDeviceToHost, sd.s));
-            HIP_TRY(hipStreamSynchronize(sd.s));
-        }
-    std::vector<float4> scene((size_t)std::max<uint32_t>(scene_pairs(n_obs), 1) * 2);
-    const uint32_t nGrp = build_scene(rows.data(), n_obs, scene.data());
+    if(n_obs && (rc = check_device_rows(d_obstacles))) return rc;
+    const uint32_t nGrp = scene_groups(n_obs);
     float4* d_scene     = nullptr;
-    HIP_TRY(hipMallocAsync((void**)&d_scene, scene.size() * sizeof(float4), sd.s));
-    HIP_TRY(hipMemcpyAsync(d_scene, scene.data(), scene.size() * sizeof(float4), hipMemcpyHostToDevice, sd.s));
-    hipError_t e = launch_stage_propagate(cfg->model, sd.d, d_parent_idx, d_parent_states, n_parents, d_controls,
-                                          d_steps, n_children, d_scene, nGrp, n_obs, d_child_samples, d_valid, d_r1,
-                                          d_r2, sd.s);
+    HIP_TRY(hipMallocAsync((void**)&d_scene, (size_t)std::max<uint32_t>(scene_pairs(n_obs), 1) * 2 * sizeof(float4),
+                           sd.s));
+    hipError_t e = launch_build_scene(d_obstacles, n_obs, nGrp, d_scene, sd.s);
+    if(e == hipSuccess)
+        e = launch_stage_propagate(cfg->model, sd.d, d_parent_idx, d_parent_states, n_parents, d_controls, d_steps,
+                                   n_children, d_scene, nGrp, n_obs, d_child_samples, d_valid, d_r1, d_r2, sd.s);
     (void)hipFreeAsync(d_scene, sd.s);
-    if(e == hipSuccess) e = hipStreamSynchronize(sd.s);  // the host scene buffer must outlive the copy
     if(e != hipSuccess) return fail(KGMT_E_HIP, std::string("propagate: ") + hipGetErrorString(e));
     return KGMT_OK;
 }

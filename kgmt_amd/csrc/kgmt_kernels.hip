@@ -779,6 +779,136 @@ __global__ __launch_bounds__(64) void k_paths(const QueryDesc* __restrict__ qds,
 }
 
 // ---------------------------------------------------------------------------
+// scene build: one workgroup per scene. Boxes sorted along a Morton curve of
+// their centres (10 bits per axis over the bounds of the finite centres; ties
+// by input order), grouped by kGroup; each group's bounds are the min/max of
+// its boxes per axis, or NaN on an axis where a member has a NaN face (such a
+// group is never skipped); groups past the boxes hold lo = +inf, hi = -inf and
+// are never near. Output: nGrp group (lo, hi) pairs, then nGrp * kGroup boxes.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t morton10(uint32_t x, uint32_t y, uint32_t z)
+{
+    uint32_t code = 0;
+    for(int b = 9; b >= 0; --b) code = (code << 3) | (((x >> b) & 1u) << 2) | (((y >> b) & 1u) << 1) | ((z >> b) & 1u);
+    return code;
+}
+
+struct SceneLds
+{
+    float box[kMaxObs][6];
+    uint32_t key[kMaxObs];
+    float red[6][kBlock / 64];
+};
+
+__device__ void build_scene_wg(const float* __restrict__ rows, uint32_t n, uint32_t nGrp, float4* __restrict__ out,
+                               SceneLds& L)
+{
+    // bounds of the finite centres
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for(uint32_t i = threadIdx.x; i < n; i += kBlock)
+        for(int d = 0; d < 3; ++d)
+            {
+                const float c = 0.5f * (rows[6 * (size_t)i + d] + rows[6 * (size_t)i + 3 + d]);
+                if(__builtin_isfinite(c)) lo[d] = c < lo[d] ? c : lo[d], hi[d] = c > hi[d] ? c : hi[d];
+            }
+    for(int d = 0; d < 3; ++d)
+        {
+            for(int o = 32; o > 0; o >>= 1)
+                {
+                    const float a = __shfl_xor(lo[d], o, 64), b = __shfl_xor(hi[d], o, 64);
+                    lo[d] = a < lo[d] ? a : lo[d], hi[d] = b > hi[d] ? b : hi[d];
+                }
+            if(lane_id() == 0) L.red[d][threadIdx.x / 64] = lo[d], L.red[3 + d][threadIdx.x / 64] = hi[d];
+        }
+    __syncthreads();
+    for(int d = 0; d < 3; ++d)
+        {
+            lo[d] = INFINITY, hi[d] = -INFINITY;
+            for(int w = 0; w < kBlock / 64; ++w)
+                {
+                    const float a = L.red[d][w], b = L.red[3 + d][w];
+                    lo[d] = a < lo[d] ? a : lo[d], hi[d] = b > hi[d] ? b : hi[d];
+                }
+        }
+    // Morton keys
+    for(uint32_t i = threadIdx.x; i < n; i += kBlock)
+        {
+            uint32_t q[3];
+            for(int d = 0; d < 3; ++d)
+                {
+                    const float c = 0.5f * (rows[6 * (size_t)i + d] + rows[6 * (size_t)i + 3 + d]);
+                    float t       = hi[d] > lo[d] ? (c - lo[d]) / (hi[d] - lo[d]) : 0.0f;
+                    if(!(t >= 0.0f)) t = 0.0f;
+                    if(t > 1.0f) t = 1.0f;
+                    q[d] = (uint32_t)(t * 1023.0f);
+                }
+            L.key[i] = morton10(q[0], q[1], q[2]);
+        }
+    __syncthreads();
+    // rank = position in (key, index) order; box i goes to slot rank
+    for(uint32_t i = threadIdx.x; i < n; i += kBlock)
+        {
+            const uint32_t ki = L.key[i];
+            uint32_t rank     = 0;
+            for(uint32_t j = 0; j < n; ++j)
+                {
+                    const uint32_t kj = L.key[j];
+                    rank += (kj < ki || (kj == ki && j < i)) ? 1u : 0u;
+                }
+            for(int d = 0; d < 6; ++d) L.box[rank][d] = rows[6 * (size_t)i + d];
+        }
+    __syncthreads();
+    float4* box = out + 2 * nGrp;
+    for(uint32_t k = threadIdx.x; k < nGrp * kGroup; k += kBlock)
+        {
+            if(k < n)
+                {
+                    box[2 * k]     = make_float4(L.box[k][0], L.box[k][1], L.box[k][2], 0.0f);
+                    box[2 * k + 1] = make_float4(L.box[k][3], L.box[k][4], L.box[k][5], 0.0f);
+                }
+            else
+                {
+                    box[2 * k]     = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+                    box[2 * k + 1] = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+                }
+        }
+    for(uint32_t g = threadIdx.x; g < nGrp; g += kBlock)
+        {
+            float gl[3] = {INFINITY, INFINITY, INFINITY}, gh[3] = {-INFINITY, -INFINITY, -INFINITY};
+            bool nanAxis[3] = {false, false, false};
+            const uint32_t b1 = n < (g + 1) * kGroup ? n : (g + 1) * kGroup;
+            for(uint32_t k = g * kGroup; k < b1; ++k)
+                for(int d = 0; d < 3; ++d)
+                    {
+                        const float l = L.box[k][d], h = L.box[k][3 + d];
+                        if(l != l || h != h) nanAxis[d] = true;
+                        gl[d] = l < gl[d] ? l : gl[d];  // a NaN member leaves the bound as is
+                        gh[d] = gh[d] < h ? h : gh[d];
+                    }
+            for(int d = 0; d < 3; ++d)
+                if(nanAxis[d]) gl[d] = gh[d] = NAN;
+            out[2 * g]     = make_float4(gl[0], gl[1], gl[2], 0.0f);
+            out[2 * g + 1] = make_float4(gh[0], gh[1], gh[2], 0.0f);
+        }
+}
+
+__global__ __launch_bounds__(kBlock) void k_build_scenes(const QueryDesc* __restrict__ qds,
+                                                         const float* __restrict__ rows, float4* __restrict__ pool)
+{
+    __shared__ SceneLds L;
+    const QueryDesc& qd = qds[blockIdx.x];
+    if(qd.nObs == 0) return;
+    build_scene_wg(rows + 6 * (size_t)qd.rowOff, qd.nObs, qd.nGrp, pool + 2 * (size_t)qd.obsOff, L);
+}
+
+__global__ __launch_bounds__(kBlock) void k_build_scene(const float* __restrict__ rows, uint32_t n, uint32_t nGrp,
+                                                        float4* __restrict__ out)
+{
+    __shared__ SceneLds L;
+    build_scene_wg(rows, n, nGrp, out, L);
+}
+
+// ---------------------------------------------------------------------------
 // stage-level kernels (single query, caller buffers)
 // ---------------------------------------------------------------------------
 template <int C>
@@ -1028,6 +1158,20 @@ hipError_t launch_append(const QueryDesc* qds, QueryState* sts, const Pools& P, 
                          uint32_t parity, const Launch& L, hipStream_t s)
 {
     hipLaunchKernelGGL(k_append, dim3(L.chunkGrid), dim3(kBlock), 0, s, qds, sts, P, W, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_scenes(const QueryDesc* qds, const float* rows, float4* pool, uint32_t nq, hipStream_t s)
+{
+    if(nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_build_scenes, dim3(nq), dim3(kBlock), 0, s, qds, rows, pool);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_scene(const float* rows, uint32_t n, uint32_t nGrp, float4* out, hipStream_t s)
+{
+    if(n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_build_scene, dim3(1), dim3(kBlock), 0, s, rows, n, nGrp, out);
     return hipGetLastError();
 }
 

@@ -17,10 +17,15 @@ struct Launch
     uint32_t chunkGrid = 1024;  // persistent workgroups of append / expand
 };
 
-// Scene of one query: boxes sorted along a Morton curve of their centres,
-// grouped by kGroup, written as nGrp group bounds followed by the boxes, each
-// a (lo, hi) float4 pair. Returns nGrp. rows: n x 6 floats.
-uint32_t build_scene(const float* rows, uint32_t n, float4* out);
+// Scenes: boxes sorted along a Morton curve of their centres, grouped by
+// kGroup, written as nGrp group bounds followed by the boxes, each a (lo, hi)
+// float4 pair. Built on the device from rows of 6 floats: one scene per query
+// (rows + 6 * rowOff, nObs boxes, at pool pair obsOff), or one scene of n
+// boxes with nGrp = scene_groups(n) groups.
+uint32_t scene_groups(uint32_t n);
+uint32_t scene_pairs(uint32_t n);
+hipError_t launch_build_scenes(const QueryDesc* qds, const float* rows, float4* pool, uint32_t nq, hipStream_t s);
+hipError_t launch_build_scene(const float* rows, uint32_t n, uint32_t nGrp, float4* out, hipStream_t s);
 
 // propagate launch shapes that are built
 bool launch_supported(uint32_t block, uint32_t lanes);

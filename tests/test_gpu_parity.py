@@ -7,9 +7,12 @@ pre-generated buffers, then whole plans are compared node by node.
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 import pytest
+
+from kgmt_amd import _lib
 
 from kgmt_amd.abi import (KGMT_DI_EXACT, KGMT_MODEL_DOUBLE_INTEGRATOR, KGMT_MODEL_DUBINS_5D,
                           KGMT_MODEL_DUBINS_AIRPLANE, KGMT_MODEL_QUADCOPTER, dims)
@@ -19,6 +22,8 @@ pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
 from tests.gpu_util import bits, call, dev, empty, host, ptr, stream  # noqa: E402
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def dubins6d_config(**kw):
@@ -246,3 +251,83 @@ def test_too_many_obstacles_is_an_error():
     rc = _lib.lib().kgmt_propagate_collide(C.byref(cfg), ptr(buf), ptr(buf), 1, ptr(buf), ptr(buf), 1,
                                            ptr(d_obs), 1501, ptr(buf), ptr(buf), ptr(buf), ptr(buf), stream())
     assert rc == -4
+
+
+def _scene_numpy(rows):
+    """Scene layout restated in numpy (float32 arithmetic): Morton order of
+    the box centres over the bounds of the finite centres (10 bits per axis,
+    ties by input order), groups of 8, group bounds min/max per axis or NaN
+    where a member has a NaN face, dummies lo = +inf / hi = -inf."""
+    rows = np.asarray(rows, np.float32).reshape(-1, 6)
+    n = len(rows)
+    per = 8 * 4
+    ngrp = (n + per - 1) // per * 4
+    c = np.float32(0.5) * (rows[:, :3] + rows[:, 3:])
+    with np.errstate(invalid="ignore", over="ignore"):
+        fin = np.isfinite(c)
+        lo = np.array([c[fin[:, d], d].min() if fin[:, d].any() else np.inf for d in range(3)], np.float32)
+        hi = np.array([c[fin[:, d], d].max() if fin[:, d].any() else -np.inf for d in range(3)], np.float32)
+        t = np.zeros_like(c)
+        for d in range(3):
+            if hi[d] > lo[d]:
+                t[:, d] = (c[:, d] - lo[d]) / (hi[d] - lo[d])
+        t = np.where(t >= 0, t, np.float32(0))
+        t = np.minimum(t, np.float32(1))
+        q = (t * np.float32(1023)).astype(np.uint32)
+    key = np.zeros(n, np.uint64)
+    for b in range(9, -1, -1):
+        for d in range(3):
+            key = (key << np.uint64(1)) | ((q[:, d] >> b) & 1).astype(np.uint64)
+    order = np.lexsort((np.arange(n), key))
+    boxes = np.empty((ngrp * 8, 2, 4), np.float32)
+    boxes[:, 0, :3], boxes[:, 1, :3] = np.inf, -np.inf
+    boxes[:, :, 3] = 0
+    boxes[:n, 0, :3] = rows[order, :3]
+    boxes[:n, 1, :3] = rows[order, 3:]
+    grp = np.zeros((ngrp, 2, 4), np.float32)
+    for g in range(ngrp):
+        m = boxes[g * 8: min(n, (g + 1) * 8)]
+        for d in range(3):
+            l, h = m[:, 0, d], m[:, 1, d]
+            if len(m) == 0:
+                grp[g, 0, d], grp[g, 1, d] = np.inf, -np.inf
+            elif np.isnan(l).any() or np.isnan(h).any():
+                grp[g, 0, d] = grp[g, 1, d] = np.nan
+            else:
+                grp[g, 0, d], grp[g, 1, d] = l.min(), h.max()
+    return ngrp, np.concatenate([grp.reshape(-1), boxes.reshape(-1)])
+
+
+@pytest.mark.parametrize("kind", ["random_100", "random_1500", "nan_and_inverted", "ties", "one", "house"])
+def test_device_scene_layout(kind):
+    """kgmt_build_scene against the numpy restatement, bit for bit."""
+    rng = np.random.default_rng(5)
+    if kind.startswith("random"):
+        rows = W.random_boxes(rng, int(kind.split("_")[1]), 0.02, 0.1)
+    elif kind == "nan_and_inverted":
+        rows = W.random_boxes(rng, 40, 0.02, 0.1)
+        rows[3, 1] = np.nan
+        rows[17, 5] = np.nan
+        rows[9] = [0.6, 0.6, 0.6, 0.4, 0.4, 0.4]
+        rows[21, :3] = -np.inf
+    elif kind == "ties":
+        rows = np.tile(np.array([[0.1, 0.2, 0.3, 0.2, 0.3, 0.4]], np.float32), (37, 1))
+        rows[::5] += 0.25
+    elif kind == "one":
+        rows = np.array([[0.1, 0.2, 0.3, 0.2, 0.3, 0.4]], np.float32)
+    else:
+        rows = W.read_obstacles_csv(os.path.join(GOLDEN, "obstacles", "house.csv"))
+    rows = np.ascontiguousarray(rows, np.float32)
+    n = len(rows)
+    want_g, want = _scene_numpy(rows)
+    d_rows = dev(rows.reshape(-1))
+    out = empty(want.size, torch.float32)
+    ng = C.c_uint32()
+    call("kgmt_build_scene", ptr(d_rows), n, ptr(out), want.size, C.byref(ng), stream())
+    assert ng.value == want_g
+    got = host(out)
+    same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), np.flatnonzero(~same)[:10]
+    # a buffer one scene entry short is refused
+    fn = getattr(_lib.lib(), "kgmt_build_scene")
+    assert fn(ptr(d_rows), n, ptr(out), want.size - 8, C.byref(ng), stream()) == -4
