@@ -208,7 +208,8 @@ int kgmt_get_path(kgmt_ctx* ctx, float* rows, uint32_t cap_rows, uint32_t* n_row
 int kgmt_get_regions(kgmt_ctx* ctx, int32_t* total, int32_t* valid, float* score, uint32_t cap,
                      uint32_t* n_regions);
 
-/* Turn per-kernel event timing on/off (it also resets the accumulators). */
+/* Event timing of the plain iteration loop: 0 off, 1 propagate launches only
+ * (two events per iteration), 2 every kernel. Resets the accumulators. */
 int kgmt_set_profiling(kgmt_ctx* ctx, int32_t on);
 int kgmt_get_profile(kgmt_ctx* ctx, kgmt_profile* out);
 

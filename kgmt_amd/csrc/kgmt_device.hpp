@@ -601,7 +601,8 @@ constexpr int kCand = 64;  // candidate boxes per wave of the two-phase check
 // 2. the wave's candidates: boxes overlapping some lane's T (a box that meets
 //    a segment overlaps the segment's box, which lies in T; group bounds
 //    contain their boxes), gathered into this wave's LDS list `cand`;
-// 3. integrate again (same operations, same bits) testing only candidates.
+// 3. integrate again (same operations, same bits) testing only candidates;
+//    with no candidate at all the dry run's result stands.
 // When a lane's positions are not finite, or the candidates exceed kCand, the
 // wave tests every box instead. Wave-uniform decisions only; call with the
 // lanes that propagate a child active.
@@ -612,12 +613,10 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
     constexpr int S = ModelDims<MODEL>::S;
     DryRun dry;
     dry.start(s[0], s[1], s[2]);
-    {
-        float s1[S];
+    float s1[S];
 #pragma unroll
-        for(int d = 0; d < S; ++d) s1[d] = s[d];
-        (void)propagate_with<MODEL>(qd, s1, u, steps, dry);
-    }
+    for(int d = 0; d < S; ++d) s1[d] = s[d];
+    const bool dryOk = propagate_with<MODEL>(qd, s1, u, steps, dry);
     int n = 0;
     if(!__any(!dry.finite))
         {
@@ -646,6 +645,13 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
         {
             FullCheck<1> chk{sc};
             return propagate_with<MODEL>(qd, s, u, steps, chk);
+        }
+    if(n == 0)
+        {
+            // no box near any lane: the dry run is the run
+#pragma unroll
+            for(int d = 0; d < S; ++d) s[d] = s1[d];
+            return dryOk;
         }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();

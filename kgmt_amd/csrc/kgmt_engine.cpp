@@ -201,8 +201,9 @@ struct Engine
     std::vector<double> iterMs;
     uint32_t itersRun = 0;
     int device        = 0;
-    // profiling: 6 timing events per iteration (boundaries of the 5 kernels)
-    bool profiling = false;
+    // profiling: 1 = events around propagate only, 2 = at the boundaries of all
+    // 5 kernels of an iteration
+    int profiling = 0;
     std::vector<hipEvent_t> profEv;
     kgmt_profile prof{};
 
@@ -346,25 +347,27 @@ struct Engine
         return KGMT_OK;
     }
 
-    // One iteration: 5 launches. marks: 5 timing events around them (profiling).
+    // One iteration: 5 launches. marks: timing events around them (profiling).
     int enqueue_iteration(uint32_t parity, hipEvent_t* marks = nullptr)
     {
+        const bool all = marks && profiling > 1;
         if(marks) HIP_TRY(hipEventRecord(marks[0], stream));
         HIP_TRY(launch_propagate(model, dq, ds, P, W, parity, maxPairs, L, stream));
         if(marks) HIP_TRY(hipEventRecord(marks[1], stream));
         HIP_TRY(launch_score(dq, ds, P, nq, stream));
-        if(marks) HIP_TRY(hipEventRecord(marks[2], stream));
+        if(all) HIP_TRY(hipEventRecord(marks[2], stream));
         HIP_TRY(launch_append(dq, ds, P, W, parity, L, stream));
-        if(marks) HIP_TRY(hipEventRecord(marks[3], stream));
+        if(all) HIP_TRY(hipEventRecord(marks[3], stream));
         HIP_TRY(launch_expand(dq, ds, P, W, parity, L, stream));
-        if(marks) HIP_TRY(hipEventRecord(marks[4], stream));
+        if(all) HIP_TRY(hipEventRecord(marks[4], stream));
         HIP_TRY(launch_finish(dq, ds, W, parity, stream));
-        if(marks) HIP_TRY(hipEventRecord(marks[5], stream));
+        if(all) HIP_TRY(hipEventRecord(marks[5], stream));
         return KGMT_OK;
     }
 
-    int set_profiling(bool on)
+    int set_profiling(int on)
     {
+        if(on < 0 || on > 2) return fail(KGMT_E_ARG, "profiling level must be 0, 1 or 2");
         profiling = on;
         std::memset(&prof, 0, sizeof(prof));
         const size_t need = (size_t)(maxIter + 1) * (KGMT_PROFILE_KERNELS + 1);
@@ -382,7 +385,7 @@ struct Engine
         for(uint32_t it = 0; it < iters; ++it)
             {
                 hipEvent_t* m = &profEv[(size_t)it * (KGMT_PROFILE_KERNELS + 1)];
-                for(int k = 0; k < KGMT_PROFILE_KERNELS; ++k)
+                for(int k = 0; k < (profiling > 1 ? KGMT_PROFILE_KERNELS : 1); ++k)
                     {
                         float ms = 0.0f;
                         HIP_TRY(hipEventElapsedTime(&ms, m[k], m[k + 1]));
@@ -779,7 +782,7 @@ int kgmt_plan(kgmt_ctx* ctx, const float* start, const float* goal, const float*
 int kgmt_set_profiling(kgmt_ctx* ctx, int32_t on)
 {
     if(!ctx) return fail(KGMT_E_ARG, "null context");
-    return ctx->eng.set_profiling(on != 0);
+    return ctx->eng.set_profiling(on);
 }
 
 int kgmt_get_profile(kgmt_ctx* ctx, kgmt_profile* out)
@@ -792,7 +795,7 @@ int kgmt_get_profile(kgmt_ctx* ctx, kgmt_profile* out)
 int kgmt_batch_set_profiling(kgmt_batch* b, int32_t on)
 {
     if(!b) return fail(KGMT_E_ARG, "null batch");
-    return b->eng.set_profiling(on != 0);
+    return b->eng.set_profiling(on);
 }
 
 int kgmt_batch_get_profile(kgmt_batch* b, kgmt_profile* out)

@@ -97,8 +97,9 @@ class KGMT(Planner):
         self.cfg.seed = int(seed)
         _lib.check(_lib.lib().kgmt_set_seed(self._h, int(seed)))
 
-    def set_profiling(self, on: bool = True):
-        _lib.check(_lib.lib().kgmt_set_profiling(self._h, 1 if on else 0))
+    def set_profiling(self, on=True):
+        """Event timing: False/0 off, True/1 propagate only, 2 every kernel."""
+        _lib.check(_lib.lib().kgmt_set_profiling(self._h, int(on)))
 
     def profile(self) -> dict:
         p = KgmtProfile()
@@ -196,8 +197,9 @@ class BatchKGMT:
         except Exception:
             pass
 
-    def set_profiling(self, on: bool = True):
-        _lib.check(_lib.lib().kgmt_batch_set_profiling(self._h, 1 if on else 0))
+    def set_profiling(self, on=True):
+        """As KGMT.set_profiling."""
+        _lib.check(_lib.lib().kgmt_batch_set_profiling(self._h, int(on)))
 
     def profile(self) -> dict:
         p = KgmtProfile()
