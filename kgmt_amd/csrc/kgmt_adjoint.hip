@@ -162,13 +162,37 @@ __device__ __forceinline__ void rk4_vjp(const QueryDesc& qd, const QuadK& K, con
     for(int i = 0; i < 4; ++i) ubar[i] += gc[i];
 }
 
-// sum over boxes of tau * softplus((margin - sd(p, box)) / tau); with gp,
-// adds scale * gradient with respect to p
-__device__ __forceinline__ float penalty(const kgmt_adjoint_params& ap, const float* p, const float4* sObs, int nObs,
-                                         float scale, float* gp)
+// lanes per trajectory: they run the same rollout (same operations, same
+// bits) and split the obstacle penalty's boxes
+constexpr int kLanes = 16;
+
+// v + v of the lane the DPP control selects (all 16 lanes of the row active)
+template <int CTRL>
+__device__ __forceinline__ float dpp_add(float v)
 {
-    float total = 0.0f;
-    for(int o = 0; o < nObs; ++o)
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// sum over a row of 16 lanes, the same bits in every lane: lane ^ 1, lane ^ 2
+// (quad permutations), then the half-row and row mirrors
+__device__ __forceinline__ float group_sum(float v)
+{
+    static_assert(kLanes == 16, "one DPP row per trajectory");
+    v = dpp_add<0xB1>(v);   // quad_perm [1,0,3,2]
+    v = dpp_add<0x4E>(v);   // quad_perm [2,3,0,1]
+    v = dpp_add<0x141>(v);  // row_half_mirror
+    v = dpp_add<0x140>(v);  // row_mirror
+    return v;
+}
+
+// sum over boxes of tau * softplus((margin - sd(p, box)) / tau), boxes
+// sub, sub + kLanes, ... in this lane, summed over the trajectory's lanes;
+// with gp, adds scale * gradient with respect to p
+__device__ __forceinline__ float penalty(const kgmt_adjoint_params& ap, const float* p, const float4* sObs, int nObs,
+                                         float scale, float* gp, int sub)
+{
+    float total = 0.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
+    for(int o = sub; o < nObs; o += kLanes)
         {
             const float4 lo4 = sObs[2 * o], hi4 = sObs[2 * o + 1];
             const float lo[3] = {lo4.x, lo4.y, lo4.z}, hi[3] = {hi4.x, hi4.y, hi4.z};
@@ -192,17 +216,30 @@ __device__ __forceinline__ float penalty(const kgmt_adjoint_params& ap, const fl
                     const float dsd = -scale / (1.0f + expf(-z));
                     if(outside > 0.0f)
                         {
-                            gp[0] += dsd * (ox / outside) * sgn[0];
-                            gp[1] += dsd * (oy / outside) * sgn[1];
-                            gp[2] += dsd * (oz / outside) * sgn[2];
+                            g0 += dsd * (ox / outside) * sgn[0];
+                            g1 += dsd * (oy / outside) * sgn[1];
+                            g2 += dsd * (oz / outside) * sgn[2];
                         }
                     else
-                        gp[am] += dsd * sgn[am];
+                        {
+                            const float gd = dsd * sgn[am];
+                            g0 += am == 0 ? gd : 0.0f;
+                            g1 += am == 1 ? gd : 0.0f;
+                            g2 += am == 2 ? gd : 0.0f;
+                        }
                 }
         }
-    return total;
+    if(gp)
+        {
+            gp[0] += group_sum(g0);
+            gp[1] += group_sum(g1);
+            gp[2] += group_sum(g2);
+        }
+    return group_sum(total);
 }
 
+// kLanes lanes per trajectory; every lane of a trajectory computes and writes
+// the same values (the same address and value from each lane)
 template <bool REFINE>
 __global__ __launch_bounds__(64) void k_adjoint(const QueryDesc* __restrict__ qdp, kgmt_adjoint_params ap,
                                                 uint32_t n, uint32_t maxT, const int32_t* __restrict__ steps,
@@ -215,8 +252,9 @@ __global__ __launch_bounds__(64) void k_adjoint(const QueryDesc* __restrict__ qd
     extern __shared__ float4 sObs[];
     for(uint32_t k = threadIdx.x; k < 2 * nObs; k += blockDim.x) sObs[k] = obs[k];
     __syncthreads();
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n) return;
+    const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / kLanes;
+    const int sub    = (int)(threadIdx.x % kLanes);
+    if(i >= n) return;  // whole groups of kLanes lanes leave together
     const QueryDesc& qd  = *qdp;
     const kgmt_config& c = qd.cfg;
     QuadK K;
@@ -249,7 +287,7 @@ __global__ __launch_bounds__(64) void k_adjoint(const QueryDesc* __restrict__ qd
                     rk4_step(qd, K, x, uc);
 #pragma unroll
                     for(int j = 0; j < 12; ++j) ckpt[(((size_t)t + 1) * 12 + j) * n + i] = x[j];
-                    J += ap.w_obstacle * penalty(ap, x, sObs, (int)nObs, 1.0f, nullptr);
+                    J += ap.w_obstacle * penalty(ap, x, sObs, (int)nObs, 1.0f, nullptr, sub);
                 }
             const float ex = x[0] - gx, ey = x[1] - gy, ez = x[2] - gz;
             J += ap.w_goal * (ex * ex + ey * ey + ez * ez);
@@ -267,7 +305,7 @@ __global__ __launch_bounds__(64) void k_adjoint(const QueryDesc* __restrict__ qd
 #pragma unroll
             for(int j = 0; j < 12; ++j) lam[j] = 0.0f;
             lam[0] = 2.0f * ap.w_goal * ex, lam[1] = 2.0f * ap.w_goal * ey, lam[2] = 2.0f * ap.w_goal * ez;
-            if(T >= 1) penalty(ap, x, sObs, (int)nObs, ap.w_obstacle, lam);
+            if(T >= 1) penalty(ap, x, sObs, (int)nObs, ap.w_obstacle, lam, sub);
             for(int t = T - 1; t >= 0; --t)
                 {
                     float xt[12], uc[4], xbar[12], ubar[4];
@@ -296,7 +334,7 @@ __global__ __launch_bounds__(64) void k_adjoint(const QueryDesc* __restrict__ qd
                             else
                                 grad[idx] = gk;
                         }
-                    if(t >= 1) penalty(ap, xt, sObs, (int)nObs, ap.w_obstacle, xbar);
+                    if(t >= 1) penalty(ap, xt, sObs, (int)nObs, ap.w_obstacle, xbar, sub);
 #pragma unroll
                     for(int j = 0; j < 12; ++j) lam[j] = xbar[j];
                 }
@@ -328,7 +366,7 @@ hipError_t launch_adjoint(bool refine, const QueryDesc* qd, const kgmt_adjoint_p
                           float* mom, float* vel, const float4* obs, uint32_t nObs, float* cost, hipStream_t s)
 {
     if(n == 0) return hipSuccess;
-    const dim3 g((n + 63) / 64), b(64);
+    const dim3 g((uint32_t)(((uint64_t)n * kLanes + 63) / 64)), b(64);
     const size_t lds = (size_t)(nObs ? nObs : 1) * 2 * sizeof(float4);
     if(refine)
         hipLaunchKernelGGL(k_adjoint<true>, g, b, lds, s, qd, ap, n, maxT, steps, x0, goal, u, grad, ckpt, mom, vel, obs,
