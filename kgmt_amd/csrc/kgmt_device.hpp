@@ -606,17 +606,99 @@ constexpr int kCand = 64;  // candidate boxes per wave of the two-phase check
 // When a lane's positions are not finite, or the candidates exceed kCand, the
 // wave tests every box instead. Wave-uniform decisions only; call with the
 // lanes that propagate a child active.
+// Interval helpers for the Dubins edge bound.
+__device__ __forceinline__ void imul(float alo, float ahi, float blo, float bhi, float& lo, float& hi)
+{
+    const float p0 = alo * blo, p1 = alo * bhi, p2 = ahi * blo, p3 = ahi * bhi;
+    lo = fminf(fminf(p0, p1), fminf(p2, p3));
+    hi = fmaxf(fmaxf(p0, p1), fmaxf(p2, p3));
+}
+
+// Ranges of sin and cos over the angles [a, b]: the endpoint values, widened
+// to +-1 when an extremum (a multiple of pi/2 of the right kind) may lie
+// inside (tested on an interval widened by 1e-3 period, so rounding can only
+// widen the result).
+__device__ __forceinline__ void sincos_range(float a, float b, float& smin, float& smax, float& cmin, float& cmax)
+{
+    float sa, ca, sb, cb;
+    sincos_k(a, sa, ca);
+    sincos_k(b, sb, cb);
+    smin = fminf(sa, sb), smax = fmaxf(sa, sb), cmin = fminf(ca, cb), cmax = fmaxf(ca, cb);
+    const float inv2pi = 0.15915494f, pi = 3.14159265f, halfpi = 1.57079633f;
+    auto hits = [&](float off) {
+        const float lo = (a - off) * inv2pi - 1e-3f, hi = (b - off) * inv2pi + 1e-3f;
+        return floorf(hi) >= ceilf(lo);
+    };
+    if(hits(0.0f)) cmax = 1.0f;
+    if(hits(pi)) cmin = -1.0f;
+    if(hits(halfpi)) smax = 1.0f;
+    if(hits(-halfpi)) smin = -1.0f;
+}
+
+// Box containing every position of a Dubins edge (statePropagator.cu:147-241)
+// without integrating it. Within the edge the angles move linearly (yaw +
+// h yr and pitch + h pr per step) and the speed by h a per step, so every RK4
+// stage samples the velocity v (cos p cos y, cos p sin y, sin p) with angles
+// and speed inside the edge's ranges; a step moves the position by h times a
+// weighted mean (weights 1/6, 4/6, 1/6) of three samples, so after k steps
+// the offset lies in k h [lo, hi] of the velocity's interval bound. The box
+// is widened by a margin that covers float rounding of the steps.
+template <bool FIVE_D>
+__device__ __forceinline__ void dubins_edge_box(const QueryDesc& qd, const float* s, const float* u, int steps,
+                                                float* lo, float* hi)
+{
+    const float h = qd.cfg.dt, T = h * (float)steps;
+    const float yr = u[0], pr = u[1];
+    const float a  = FIVE_D ? 0.0f : u[2];
+    const float v0 = FIVE_D ? u[2] : s[5];
+    const float dp = pr * T, dy = yr * T, dv = a * T;
+    const float angPad = 1e-5f * (1.0f + fabsf(s[3]) + fabsf(s[4]) + fabsf(dp) + fabsf(dy));
+    const float pa = s[4] + fminf(0.0f, dp) - angPad, pb = s[4] + fmaxf(0.0f, dp) + angPad;
+    const float ya = s[3] + fminf(0.0f, dy) - angPad, yb = s[3] + fmaxf(0.0f, dy) + angPad;
+    const float vlo = v0 + fminf(0.0f, dv), vhi = v0 + fmaxf(0.0f, dv);
+    float spl, sph, cpl, cph, syl, syh, cyl, cyh;
+    sincos_range(pa, pb, spl, sph, cpl, cph);
+    sincos_range(ya, yb, syl, syh, cyl, cyh);
+    float dl[3], dh[3], tl, th;
+    imul(cpl, cph, cyl, cyh, tl, th);
+    imul(vlo, vhi, tl, th, dl[0], dh[0]);
+    imul(cpl, cph, syl, syh, tl, th);
+    imul(vlo, vhi, tl, th, dl[1], dh[1]);
+    imul(vlo, vhi, spl, sph, dl[2], dh[2]);
+    const float vmax = fmaxf(fabsf(vlo), fabsf(vhi));
+#pragma unroll
+    for(int d = 0; d < 3; ++d)
+        {
+            const float pad = 1e-4f * (1.0f + fabsf(s[d]) + T * vmax);
+            lo[d]           = s[d] + fminf(0.0f, T * dl[d]) - pad;
+            hi[d]           = s[d] + fmaxf(0.0f, T * dh[d]) + pad;
+        }
+}
+
 template <int MODEL>
 __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* s, const float* u, int steps,
                                                     const Scene& sc, float4* cand)
 {
-    constexpr int S = ModelDims<MODEL>::S;
+    constexpr int S      = ModelDims<MODEL>::S;
+    constexpr bool BOUND = MODEL == KGMT_MODEL_DUBINS_5D || MODEL == KGMT_MODEL_DUBINS_AIRPLANE;
     DryRun dry;
-    dry.start(s[0], s[1], s[2]);
     float s1[S];
+    bool dryOk = false;
+    if constexpr(BOUND)
+        {
+            // the edge's box from the bound, without a dry run
+            dubins_edge_box<MODEL == KGMT_MODEL_DUBINS_5D>(qd, s, u, steps, dry.lo, dry.hi);
+            dry.finite = __builtin_isfinite(dry.lo[0]) && __builtin_isfinite(dry.lo[1]) &&
+                         __builtin_isfinite(dry.lo[2]) && __builtin_isfinite(dry.hi[0]) &&
+                         __builtin_isfinite(dry.hi[1]) && __builtin_isfinite(dry.hi[2]);
+        }
+    else
+        {
+            dry.start(s[0], s[1], s[2]);
 #pragma unroll
-    for(int d = 0; d < S; ++d) s1[d] = s[d];
-    const bool dryOk = propagate_with<MODEL>(qd, s1, u, steps, dry);
+            for(int d = 0; d < S; ++d) s1[d] = s[d];
+            dryOk = propagate_with<MODEL>(qd, s1, u, steps, dry);
+        }
     int n = 0;
     if(!__any(!dry.finite))
         {
@@ -646,7 +728,7 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
             FullCheck<1> chk{sc};
             return propagate_with<MODEL>(qd, s, u, steps, chk);
         }
-    if(n == 0)
+    if(!BOUND && n == 0)
         {
             // no box near any lane: the dry run is the run
 #pragma unroll

@@ -168,3 +168,24 @@ def test_set_launch_rejects_bad_shapes():
     g.set_launch(propagate_block=512)
     res = g.plan(q.start, q.goal, q.obstacles, len(q.obstacles))
     assert res.tree_size >= 1
+
+
+def test_dubins_airplane_plan_matches_oracle(orc):
+    """The reference's 6D Dubins airplane (speed as a state, acceleration as
+    a control): whole plans node by node, plain and batched."""
+    import math
+    from kgmt_amd.abi import KGMT_MODEL_DUBINS_AIRPLANE
+    base = W.make_query("dubins5d", 5, capacity=1 << 14, max_iter=25)
+    pi = math.pi
+    cfg = base.cfg.replace(model=KGMT_MODEL_DUBINS_AIRPLANE, dt=0.05, steps_min=2, steps_max=12, v_min=0.05,
+                           v_max=0.4, ctrl_min=[-1.0, -0.5, -0.3, 0], ctrl_max=[1.0, 0.5, 0.3, 0])
+    start = np.zeros(10, np.float32)
+    start[:5] = base.start[:5]
+    start[5] = 0.2
+    goal = np.zeros(10, np.float32)
+    goal[:3] = base.goal[:3]
+    ref = orc.plan(cfg, start, goal, base.obstacles)
+    g = KGMT(cfg)
+    res = g.plan(start, goal, base.obstacles, len(base.obstacles))
+    _same_plan(g, res, ref)
+    assert ref["stats"]["tree_size"] > 100
