@@ -364,7 +364,7 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
 }
 
 template <int MODEL, int NT, int K>
-__global__ __launch_bounds__(NT) void k_propagate(const QueryDesc* __restrict__ qds,
+__global__ __launch_bounds__(NT, 4) void k_propagate(const QueryDesc* __restrict__ qds,
                                                   const QueryState* __restrict__ sts, Pools P, WorkLists W,
                                                   uint32_t parity)
 {
