@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define KGMT_ABI_VERSION 1
+#define KGMT_ABI_VERSION 2
 
 /* ---- status codes -------------------------------------------------------- */
 enum {
@@ -58,7 +58,7 @@ enum {
 
 /* ---- dynamics models (numbering follows config.h MODEL) ------------------ */
 enum {
-    KGMT_MODEL_UNICYCLE          = 0, /* statePropagator.cu:13-71 (not built)  */
+    KGMT_MODEL_UNICYCLE          = 0, /* statePropagator.cu:13-71 (planar, W_DIM 2) */
     KGMT_MODEL_DOUBLE_INTEGRATOR = 1, /* statePropagator.cu:76-142             */
     KGMT_MODEL_DUBINS_AIRPLANE   = 2, /* statePropagator.cu:147-241 (6D, v state) */
     KGMT_MODEL_QUADCOPTER        = 3, /* statePropagator.cu:246-370            */
@@ -76,7 +76,8 @@ enum {
     KGMT_PRESET_REFERENCE_QUAD = 0, /* config.h:209-307 as shipped (quad, W 0..100) */
     KGMT_PRESET_DI6D           = 1, /* 6D double integrator, unit cube, 2^17 nodes  */
     KGMT_PRESET_DUBINS5D       = 2, /* 5D Dubins airplane, unit cube, 2^18 nodes    */
-    KGMT_PRESET_QUAD12D        = 3  /* 12D quadcopter, unit cube, 2^19 nodes        */
+    KGMT_PRESET_QUAD12D        = 3, /* 12D quadcopter, unit cube, 2^19 nodes        */
+    KGMT_PRESET_UNICYCLE2D     = 4  /* planar unicycle, unit square, 2^17 nodes     */
 };
 
 /* Sample layout (one tree node / one row of the path), as in the reference
@@ -86,6 +87,13 @@ enum {
  *   dubins airplane   : S=6  (x y z yaw pitch v)     C=3 (yawRate pitchRate a)
  *   quadcopter        : S=12 (x y z phi theta psi u v w p q r) C=4 (Zc Lc Mc Nc)
  *   dubins 5D         : S=5  (x y z yaw pitch)       C=3 (yawRate pitchRate speed)
+ *   unicycle          : S=4  (x y theta v)           C=2 (a steering)
+ *
+ * Workspace dimension W: 2 for the unicycle (positions x y), 3 otherwise
+ * (x y z). Obstacles are axis-aligned boxes, one row of 2W floats each
+ * (min corner then max corner: xmin ymin [zmin] xmax ymax [zmax]), the
+ * layout isMotionValid reads (collisionCheck.cu:53-58). Regions tile the
+ * workspace only: W_R1_LENGTH^W regions of W_R2_LENGTH^W subregions each.
  */
 #define KGMT_MAX_STATE   12
 #define KGMT_MAX_CONTROL 4
@@ -116,6 +124,8 @@ typedef struct kgmt_config {
     float    quad_mass, quad_ix, quad_iy, quad_iz;
     float    quad_nu, quad_mu, gravity;
     uint32_t query_id;           /* Philox counter word: queries sharing a seed     */
+    float    uni_length;         /* UNI_LENGTH: unicycle wheel base (unicycle only) */
+    uint32_t reserved;           /* must be 0                                       */
     uint64_t seed;               /* replaces the wall-clock seed of KGMT.cu:80-81  */
 } kgmt_config;
 
@@ -190,10 +200,11 @@ int kgmt_set_launch(kgmt_ctx* ctx, uint32_t propagate_block, uint32_t lanes_per_
                     uint32_t chunk_grid);
 /* Use an existing hipStream_t for every launch of this context (NULL: own stream). */
 int kgmt_set_stream(kgmt_ctx* ctx, void* hip_stream);
-/* start, goal: host arrays of D floats (only state[0..S) of start and
- * state[0..3) of goal are read).  obstacles: 6*n_obs floats, one box per row
- * (xmin ymin zmin xmax ymax zmax); a device pointer when obstacles_on_device
- * is non-zero (as in the reference), else host memory. Blocking. */
+/* start, goal: host arrays of D floats (only state[0..S) of start and the
+ * W position floats of goal are read).  obstacles: 2W*n_obs floats, one box
+ * per row (xmin ymin zmin xmax ymax zmax; xmin ymin xmax ymax for W = 2); a
+ * device pointer when obstacles_on_device is non-zero (as in the reference),
+ * else host memory. Blocking. */
 int kgmt_plan(kgmt_ctx* ctx, const float* start, const float* goal, const float* obstacles,
               uint32_t n_obs, int32_t obstacles_on_device, kgmt_result* out);
 /* Host copies of the tree of the last plan: samples as rows of D floats
@@ -217,7 +228,9 @@ int kgmt_get_profile(kgmt_ctx* ctx, kgmt_profile* out);
 /* n_q queries, each with its own configuration (capacity, model, ...) and its
  * own obstacle set. obstacle_offsets[q] (in boxes) indexes `obstacles`;
  * obstacle_counts[q] boxes belong to query q. starts/goals: n_q rows of
- * KGMT_MAX_SAMPLE floats. All inputs are host memory. */
+ * KGMT_MAX_SAMPLE floats. All inputs are host memory. Every query of a batch
+ * has the same workspace dimension W (rows of 2W floats); kgmt_batch_create
+ * rejects a mix. */
 int kgmt_batch_create(const kgmt_config* cfgs, uint32_t n_q, kgmt_batch** out);
 int kgmt_batch_destroy(kgmt_batch* b);
 int kgmt_batch_set_stream(kgmt_batch* b, void* hip_stream);
@@ -244,7 +257,9 @@ int kgmt_batch_get_profile(kgmt_batch* b, kgmt_profile* out);
 int kgmt_draw_controls(const kgmt_config* cfg, uint32_t query, uint32_t itr, uint32_t u0, uint32_t n,
                        float* d_controls, int32_t* d_steps, float* d_ufront, void* stream);
 /* Obstacle scene as the propagate kernels read it, built from n rows of 6
- * floats (xmin ymin zmin xmax ymax zmax, collisionCheck.cu:53-58): boxes in
+ * floats (xmin ymin zmin xmax ymax zmax, collisionCheck.cu:53-58; the planner
+ * stores a planar box as z range [-1, 1], which contains every planar
+ * position's z = 0, so the 3D test gives the planar verdict): boxes in
  * Morton order of their centres in groups of 8, n_groups group bounds first;
  * each entry is a (lo, hi) pair of 4-float vectors, so the scene takes
  * 8 * n_groups * 9 floats. Returns KGMT_E_CAPACITY (n_groups still set) when
@@ -254,7 +269,8 @@ int kgmt_build_scene(const float* d_rows, uint32_t n, float* d_out, uint32_t out
                      void* stream);
 /* Propagate + collide: child i starts from parent_states[:, parent_idx[i]]
  * ([S][n_parents]), applies controls[:, i] for steps[i] steps. Writes child
- * samples [D][n] (state, control, duration), valid[n], r1[n], r2[n]. */
+ * samples [D][n] (state, control, duration), valid[n], r1[n], r2[n].
+ * d_obstacles: n_obs rows of 2W floats, as kgmt_plan. */
 int kgmt_propagate_collide(const kgmt_config* cfg, const int32_t* d_parent_idx,
                            const float* d_parent_states, uint32_t n_parents,
                            const float* d_controls, const int32_t* d_steps, uint32_t n_children,

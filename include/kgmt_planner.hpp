@@ -73,7 +73,8 @@ public:
     }
 
     // KGMT::plan, src/planners/KGMT.cu:41-105. d_obstacles_ptr: device memory,
-    // 6 floats per box (xmin ymin zmin xmax ymax zmax).
+    // 2 * W_DIM floats per box (xmin ymin zmin xmax ymax zmax; xmin ymin xmax
+    // ymax for the planar unicycle).
     void plan(float* h_initial, float* h_goal, float* d_obstacles_ptr, uint h_obstaclesCount) override
     {
         kgmt::check(kgmt_plan(ctx_, h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount, 1, &result_));

@@ -6,7 +6,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-KGMT_ABI_VERSION = 1
+KGMT_ABI_VERSION = 2
 
 KGMT_OK = 0
 KGMT_E_ARG = -1
@@ -29,6 +29,7 @@ KGMT_PRESET_REFERENCE_QUAD = 0
 KGMT_PRESET_DI6D = 1
 KGMT_PRESET_DUBINS5D = 2
 KGMT_PRESET_QUAD12D = 3
+KGMT_PRESET_UNICYCLE2D = 4
 
 KGMT_MAX_STATE = 12
 KGMT_MAX_CONTROL = 4
@@ -39,6 +40,7 @@ MODEL_DIMS = {  # model -> (S, C)
     KGMT_MODEL_DUBINS_AIRPLANE: (6, 3),
     KGMT_MODEL_QUADCOPTER: (12, 4),
     KGMT_MODEL_DUBINS_5D: (5, 3),
+    KGMT_MODEL_UNICYCLE: (4, 2),
 }
 
 
@@ -46,6 +48,12 @@ def dims(model: int):
     """(S, C, D) of a model."""
     s, c = MODEL_DIMS[model]
     return s, c, s + c + 1
+
+
+def wdim(model: int) -> int:
+    """Workspace dimension W: 2 for the planar unicycle, else 3. Obstacle
+    rows hold 2W floats (min corner, max corner)."""
+    return 2 if model == KGMT_MODEL_UNICYCLE else 3
 
 
 class KgmtConfig(C.Structure):
@@ -82,6 +90,8 @@ class KgmtConfig(C.Structure):
         ("quad_mu", C.c_float),
         ("gravity", C.c_float),
         ("query_id", C.c_uint32),
+        ("uni_length", C.c_float),
+        ("reserved", C.c_uint32),
         ("seed", C.c_uint64),
     ]
 
@@ -111,6 +121,15 @@ class KgmtConfig(C.Structure):
     @property
     def dims(self):
         return dims(self.model)
+
+    @property
+    def wdim(self) -> int:
+        return wdim(self.model)
+
+    @property
+    def regions(self) -> int:
+        """W_R1_LENGTH^W regions of the workspace grid."""
+        return self.r1_len ** wdim(self.model)
 
 
 class KgmtResult(C.Structure):
@@ -168,6 +187,6 @@ class KgmtProfile(C.Structure):
         }
 
 
-assert C.sizeof(KgmtConfig) == 160, C.sizeof(KgmtConfig)
+assert C.sizeof(KgmtConfig) == 168, C.sizeof(KgmtConfig)
 assert C.sizeof(KgmtResult) == 40, C.sizeof(KgmtResult)
 assert C.sizeof(KgmtAdjointParams) == 60, C.sizeof(KgmtAdjointParams)

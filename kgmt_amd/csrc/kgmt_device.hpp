@@ -51,8 +51,8 @@ struct QueryDesc
     float goal[3];
     float start[KGMT_MAX_SAMPLE];
     float qminv, qax, qbx, qay, qby, qaz, qbz;  // quad: 1/m, (Iy-Iz)/Ix, 1/Ix, ...
-    uint32_t rowOff;                 // first obstacle row (6 floats) of the query in the rows the scene is built from
-    uint32_t pad;
+    uint32_t rowOff;                 // first obstacle row (2 * wdim floats) of the query in the scene's source rows
+    int32_t wdim;                    // workspace dimension W_DIM: 2 (planar: x y, z = 0) or 3
 };
 
 struct QueryState
@@ -384,6 +384,60 @@ struct ModelDims<KGMT_MODEL_DUBINS_5D>
 {
     static constexpr int S = 5, C = 3;
 };
+template <>
+struct ModelDims<KGMT_MODEL_UNICYCLE>
+{
+    static constexpr int S = 4, C = 2;
+};
+
+// z of a state's position: the planar unicycle moves in z = 0
+template <int MODEL>
+__device__ __forceinline__ float pos_z(const float* s)
+{
+    if constexpr(MODEL == KGMT_MODEL_UNICYCLE)
+        return 0.0f;
+    else
+        return s[2];
+}
+
+// statePropagator.cu:13-71: Euler steps of the planar unicycle, workspace
+// check on x y, then the step's segment (z = 0; planar boxes span z [-1, 1],
+// so the 3D test gives the planar verdict). tan(steering) = sin / cos of
+// sincos_k.
+template <class CHK>
+__device__ __forceinline__ bool prop_unicycle(const QueryDesc& qd, float* s, const float* u, int steps, CHK& chk)
+{
+    const kgmt_config& c = qd.cfg;
+    const float h        = c.dt;
+    const float a = u[0], steer = u[1];
+    float ss, cs;
+    sincos_k(steer, ss, cs);
+    const float tn = ss / cs;
+    float x = s[0], y = s[1], theta = s[2], v = s[3];
+    bool valid = true;
+    for(int i = 0; i < steps; ++i)
+        {
+            const float px = x, py = y;
+            float st, ct;
+            sincos_k(theta, st, ct);
+            x     = x + (v * ct) * h;
+            y     = y + (v * st) * h;
+            theta = theta + ((v / c.uni_length) * tn) * h;
+            v     = v + a * h;
+            if(x < c.ws_min || x > c.ws_max || y < c.ws_min || y > c.ws_max)
+                {
+                    valid = false;
+                    break;
+                }
+            if(!chk(px, py, 0.0f, x, y, 0.0f))
+                {
+                    valid = false;
+                    break;
+                }
+        }
+    s[0] = x, s[1] = y, s[2] = theta, s[3] = v;
+    return valid;
+}
 
 // statePropagator.cu:76-142
 template <class CHK>
@@ -579,6 +633,8 @@ __device__ __forceinline__ bool propagate_with(const QueryDesc& qd, float* s, co
         return prop_dubins<false>(qd, s, u, steps, chk);
     else if constexpr(MODEL == KGMT_MODEL_DUBINS_5D)
         return prop_dubins<true>(qd, s, u, steps, chk);
+    else if constexpr(MODEL == KGMT_MODEL_UNICYCLE)
+        return prop_unicycle(qd, s, u, steps, chk);
     else
         return prop_quad(qd, s, u, steps, chk);
 }
@@ -694,7 +750,7 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
         }
     else
         {
-            dry.start(s[0], s[1], s[2]);
+            dry.start(s[0], s[1], pos_z<MODEL>(s));
 #pragma unroll
             for(int d = 0; d < S; ++d) s1[d] = s[d];
             dryOk = propagate_with<MODEL>(qd, s1, u, steps, dry);
@@ -743,7 +799,7 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
 }
 
 // -------------------------------------------------------------------------
-// Regions (workspace-only grid, z fastest). Graph.cu:99-153, 155-205; the
+// Regions (workspace-only grid, last axis fastest). Graph.cu:99-153, 155-205; the
 // region minimum corner is ws_min + index * W_R1_SIZE (Graph.cu:67-70).
 // -------------------------------------------------------------------------
 __device__ __forceinline__ int grid_index(float f, int L)
@@ -754,15 +810,21 @@ __device__ __forceinline__ int grid_index(float f, int L)
     return i >= L ? L - 1 : i;
 }
 
+// Planar queries (wdim 2) ignore z: regions ix * L + iy.
 __device__ __forceinline__ int region_of(const QueryDesc& qd, float x, float y, float z, int& ix, int& iy, int& iz)
 {
     const kgmt_config& c = qd.cfg;
     const int L          = c.r1_len;
     const float span     = c.ws_max - c.ws_min;
     const float fL       = (float)L;
-    iz                   = grid_index((fL * (z - c.ws_min)) / span, L);
     iy                   = grid_index((fL * (y - c.ws_min)) / span, L);
     ix                   = grid_index((fL * (x - c.ws_min)) / span, L);
+    if(qd.wdim == 2)
+        {
+            iz = 0;
+            return ix * L + iy;
+        }
+    iz = grid_index((fL * (z - c.ws_min)) / span, L);
     return (ix * L + iy) * L + iz;
 }
 
@@ -773,18 +835,28 @@ __device__ __forceinline__ int subregion_of(const QueryDesc& qd, float x, float 
     const int L2         = c.r2_len;
     const float fL2      = (float)L2;
     const float size     = qd.r1size;
-    const float mx = c.ws_min + (float)ix * size, my = c.ws_min + (float)iy * size, mz = c.ws_min + (float)iz * size;
-    const int lz = grid_index((fL2 * (z - mz)) / size, L2);
+    const float mx = c.ws_min + (float)ix * size, my = c.ws_min + (float)iy * size;
     const int ly = grid_index((fL2 * (y - my)) / size, L2);
     const int lx = grid_index((fL2 * (x - mx)) / size, L2);
+    if(qd.wdim == 2) return r1 * qd.P + lx * L2 + ly;
+    const float mz = c.ws_min + (float)iz * size;
+    const int lz   = grid_index((fL2 * (z - mz)) / size, L2);
     return r1 * qd.P + (lx * L2 + ly) * L2 + lz;
 }
 
-// helper.cuh:138-152 (3D goal distance)
+// helper.cuh:138-152: Euclidean distance over the W_DIM position axes
+__device__ __forceinline__ float pos_dist(const QueryDesc& qd, float ax, float ay, float az, float bx, float by,
+                                          float bz)
+{
+    const float dx = ax - bx, dy = ay - by;
+    if(qd.wdim == 2) return __builtin_sqrtf(dx * dx + dy * dy);
+    const float dz = az - bz;
+    return __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
 __device__ __forceinline__ float goal_dist(const QueryDesc& qd, float x, float y, float z)
 {
-    const float dx = x - qd.goal[0], dy = y - qd.goal[1], dz = z - qd.goal[2];
-    return __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
+    return pos_dist(qd, x, y, z, qd.goal[0], qd.goal[1], qd.goal[2]);
 }
 
 // -------------------------------------------------------------------------

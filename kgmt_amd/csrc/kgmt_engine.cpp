@@ -51,17 +51,29 @@ int model_dims(int model, int& S, int& C)
             case KGMT_MODEL_DUBINS_AIRPLANE: S = 6, C = 3; return 0;
             case KGMT_MODEL_QUADCOPTER: S = 12, C = 4; return 0;
             case KGMT_MODEL_DUBINS_5D: S = 5, C = 3; return 0;
+            case KGMT_MODEL_UNICYCLE: S = 4, C = 2; return 0;
             default: return -1;
         }
+}
+
+// workspace dimension W_DIM: the unicycle is planar
+int model_wdim(int model)
+{
+    return model == KGMT_MODEL_UNICYCLE ? 2 : 3;
+}
+
+int64_t ipow(int64_t b, int e)
+{
+    int64_t r = 1;
+    for(int i = 0; i < e; ++i) r *= b;
+    return r;
 }
 
 int check_config(const kgmt_config* c)
 {
     if(!c) return fail(KGMT_E_ARG, "config is null");
     int S, C;
-    if(model_dims(c->model, S, C))
-        return fail(c->model == KGMT_MODEL_UNICYCLE ? KGMT_E_UNSUPPORTED : KGMT_E_ARG,
-                    "model " + std::to_string(c->model) + " is not built");
+    if(model_dims(c->model, S, C)) return fail(KGMT_E_ARG, "unknown model " + std::to_string(c->model));
     if(c->capacity < 2 || c->capacity > (1u << 26)) return fail(KGMT_E_ARG, "capacity must be in [2, 2^26]");
     if(c->max_iter < 1 || c->max_iter > 1000000) return fail(KGMT_E_ARG, "max_iter must be in [1, 1e6]");
     if(!(c->dt > 0.0f) || !std::isfinite(c->dt)) return fail(KGMT_E_ARG, "dt must be positive");
@@ -69,9 +81,11 @@ int check_config(const kgmt_config* c)
         return fail(KGMT_E_ARG, "steps range must satisfy 1 <= steps_min <= steps_max <= 1e5");
     if(!(c->ws_max > c->ws_min)) return fail(KGMT_E_ARG, "ws_max must exceed ws_min");
     if(c->r1_len < 1 || c->r2_len < 1) return fail(KGMT_E_ARG, "grid lengths must be >= 1");
-    const int64_t R1 = (int64_t)c->r1_len * c->r1_len * c->r1_len;
-    const int64_t P  = (int64_t)c->r2_len * c->r2_len * c->r2_len;
-    if(R1 > 8192) return fail(KGMT_E_ARG, "at most 8192 regions (r1_len <= 20)");
+    if(c->r1_len > 8192 || c->r2_len > 8192) return fail(KGMT_E_ARG, "grid lengths must be <= 8192");
+    const int W      = model_wdim(c->model);
+    const int64_t R1 = ipow(c->r1_len, W);
+    const int64_t P  = ipow(c->r2_len, W);
+    if(R1 > 8192) return fail(KGMT_E_ARG, "at most 8192 regions (r1_len <= 20 in 3D, <= 90 planar)");
     if(R1 * P > (1ll << 28)) return fail(KGMT_E_ARG, "too many subregions");
     if(c->children_per_entry < 1 || c->children_per_entry > 1024)
         return fail(KGMT_E_ARG, "children_per_entry must be in [1, 1024]");
@@ -81,6 +95,9 @@ int check_config(const kgmt_config* c)
     if(c->model == KGMT_MODEL_QUADCOPTER &&
        (!(c->quad_mass > 0.0f) || !(c->quad_ix > 0.0f) || !(c->quad_iy > 0.0f) || !(c->quad_iz > 0.0f)))
         return fail(KGMT_E_ARG, "quad mass and inertia must be positive");
+    if(c->model == KGMT_MODEL_UNICYCLE && !(c->uni_length > 0.0f && std::isfinite(c->uni_length)))
+        return fail(KGMT_E_ARG, "uni_length must be positive");
+    if(c->reserved != 0) return fail(KGMT_E_ARG, "reserved must be 0");
     for(int i = 0; i < C; ++i)
         if(!(c->ctrl_max[i] >= c->ctrl_min[i])) return fail(KGMT_E_ARG, "control range reversed");
     return KGMT_OK;
@@ -94,15 +111,17 @@ void fill_desc(QueryDesc& qd, const kgmt_config& c)
     int S = 0, C = 0;
     model_dims(c.model, S, C);
     qd.S = S, qd.C = C, qd.D = S + C + 1;
-    qd.R1      = c.r1_len * c.r1_len * c.r1_len;
-    qd.P       = c.r2_len * c.r2_len * c.r2_len;
+    qd.wdim    = model_wdim(c.model);
+    qd.R1      = (int32_t)ipow(c.r1_len, qd.wdim);
+    qd.P       = (int32_t)ipow(c.r2_len, qd.wdim);
     qd.R2      = qd.R1 * qd.P;
     qd.R2words = (qd.R2 + 31) / 32;
     qd.cap     = c.capacity;
     qd.nChunks = (c.capacity + kChunk - 1) / kChunk;
     // W_R1_SIZE and W_R1_VOL as the float macros of config.h:247-251 evaluate
+    // (W_R1_VOL = W_R1_SIZE^W_DIM: an area for a planar workspace)
     qd.r1size = (c.ws_max - c.ws_min) / (float)c.r1_len;
-    qd.vol    = qd.r1size * qd.r1size * qd.r1size;
+    qd.vol    = qd.wdim == 2 ? qd.r1size * qd.r1size : qd.r1size * qd.r1size * qd.r1size;
     qd.qminv  = 1.0f / c.quad_mass;
     qd.qax = (c.quad_iy - c.quad_iz) / c.quad_ix, qd.qbx = 1.0f / c.quad_ix;
     qd.qay = (c.quad_iz - c.quad_ix) / c.quad_iy, qd.qby = 1.0f / c.quad_iy;
@@ -251,6 +270,8 @@ struct Engine
                 paths += (uint64_t)(cfgs[q].max_iter + 2) * qd.D;
                 if(paths > 0xffffffffull) return fail(KGMT_E_CAPACITY, "path pool too large");
                 if(cfgs[q].model != model) model = -1;
+                if(qd.wdim != hq[0].wdim)
+                    return fail(KGMT_E_ARG, "a batch cannot mix planar (unicycle) and 3D queries");
                 maxIter = std::max(maxIter, cfgs[q].max_iter);
             }
         int rc = 0;
@@ -577,6 +598,7 @@ int kgmt_config_preset(int32_t preset, kgmt_config* o)
     o->quad_mass = 1.0f, o->quad_ix = 1.0f, o->quad_iy = 1.0f, o->quad_iz = 2.0f;
     o->quad_nu = 10e-3f, o->quad_mu = 2e-6f, o->gravity = -9.81f;
     o->pitch_min = -(float)M_PI / 3.0f, o->pitch_max = (float)M_PI / 3.0f;
+    o->uni_length = 1.0f;  // UNI_LENGTH, config.h:270
     o->seed      = 1;
     const float pi = (float)M_PI;
     switch(preset)
@@ -617,6 +639,14 @@ int kgmt_config_preset(int32_t preset, kgmt_config* o)
                 for(int i = 1; i < 4; ++i) o->ctrl_min[i] = -0.05f, o->ctrl_max[i] = 0.05f;
                 o->quad_ix = 0.01f, o->quad_iy = 0.01f, o->quad_iz = 0.02f;
                 o->r1_len = 10, o->r2_len = 4;
+                return KGMT_OK;
+            case KGMT_PRESET_UNICYCLE2D:  // steering and UNI_LENGTH of config.h:266-270, A of :34-35
+                o->model    = KGMT_MODEL_UNICYCLE;
+                o->capacity = 1u << 17, o->dt = 0.1f, o->steps_min = 1, o->steps_max = 10;
+                o->goal_thresh = 0.05f, o->ws_min = 0.0f, o->ws_max = 1.0f;
+                o->ctrl_min[0] = -0.2f, o->ctrl_max[0] = 0.2f;
+                o->ctrl_min[1] = -pi / 2.0f, o->ctrl_max[1] = pi / 2.0f;
+                o->r1_len = 16, o->r2_len = 4;
                 return KGMT_OK;
             default: return fail(KGMT_E_ARG, "unknown preset");
         }
@@ -691,10 +721,10 @@ int kgmt_set_stream(kgmt_ctx* ctx, void* s)
     return ctx->eng.set_stream(s);
 }
 
-// Obstacle rows of every query -> scenes (grouped, Morton-sorted) in the pool.
-// rows may be device memory (the reference passes a device pointer); they are
-// read back once per plan (6 floats per box) to build the groups on the host.
-// The scene kernels read obstacle rows in place: reject a pointer that the
+// Obstacle rows of every query -> scenes (grouped, Morton-sorted) in the pool,
+// built on the device. Rows hold 2 * W floats (W = the queries' workspace
+// dimension); they may be device memory (the reference passes a device
+// pointer), which the scene kernels read in place: reject a pointer that the
 // device cannot read (for example host memory passed as device memory).
 static int check_device_rows(const float* p)
 {
@@ -719,7 +749,7 @@ static int upload_obstacles(Engine& e, const float* obstacles, const uint32_t* o
             const uint32_t c   = counts ? counts[q] : total;
             const uint32_t off = offsets ? offsets[q] : 0;
             if(c > kMaxObs) return fail(KGMT_E_CAPACITY, "more than 1500 obstacles in one query");
-            if((uint64_t)off + c > total) return fail(KGMT_E_ARG, "obstacle range out of bounds");
+            if((uint64_t)off + c > (uint64_t)total) return fail(KGMT_E_ARG, "obstacle range out of bounds");
             pairs += scene_pairs(c);
         }
     int rc = e.ensure_obstacles(pairs);
@@ -738,7 +768,8 @@ static int upload_obstacles(Engine& e, const float* obstacles, const uint32_t* o
             else
                 {
                     if((rc = e.ensure_rows(total))) return rc;
-                    HIP_TRY(hipMemcpyAsync(e.rowsBuf, obstacles, (size_t)total * 6 * sizeof(float),
+                    const size_t rowW = 2 * (size_t)e.hq[0].wdim;
+                    HIP_TRY(hipMemcpyAsync(e.rowsBuf, obstacles, (size_t)total * rowW * sizeof(float),
                                            hipMemcpyHostToDevice, e.stream));
                     e.sceneRows = e.rowsBuf;
                 }
@@ -769,7 +800,7 @@ int kgmt_plan(kgmt_ctx* ctx, const float* start, const float* goal, const float*
     QueryDesc& qd = e.hq[0];
     std::memset(qd.start, 0, sizeof(qd.start));
     std::memcpy(qd.start, start, sizeof(float) * qd.D);
-    qd.goal[0] = goal[0], qd.goal[1] = goal[1], qd.goal[2] = goal[2];
+    qd.goal[0] = goal[0], qd.goal[1] = goal[1], qd.goal[2] = qd.wdim == 3 ? goal[2] : 0.0f;
     int rc = upload_obstacles(e, obstacles, nullptr, nullptr, n_obs, on_device);
     if(rc) return rc;
     rc = e.run(ctx->useGraph && !e.profiling);
@@ -881,8 +912,10 @@ int kgmt_batch_plan(kgmt_batch* b, const float* starts, const float* goals, cons
             std::memset(qd.start, 0, sizeof(qd.start));
             std::memcpy(qd.start, starts + (size_t)q * KGMT_MAX_SAMPLE, sizeof(float) * qd.D);
             const float* g = goals + (size_t)q * KGMT_MAX_SAMPLE;
-            qd.goal[0] = g[0], qd.goal[1] = g[1], qd.goal[2] = g[2];
-            total = std::max(total, obstacle_offsets[q] + obstacle_counts[q]);
+            qd.goal[0] = g[0], qd.goal[1] = g[1], qd.goal[2] = qd.wdim == 3 ? g[2] : 0.0f;
+            const uint64_t end = (uint64_t)obstacle_offsets[q] + obstacle_counts[q];
+            if(end > 0xffffffffull) return fail(KGMT_E_ARG, "obstacle range out of bounds");
+            total = std::max(total, (uint32_t)end);
         }
     int rc = upload_obstacles(e, obstacles, obstacle_offsets, obstacle_counts, total, 0);
     if(rc) return rc;
@@ -967,7 +1000,7 @@ int kgmt_build_scene(const float* d_rows, uint32_t n, float* d_out, uint32_t out
     if((uint64_t)out_floats < (uint64_t)scene_pairs(n) * 8) return fail(KGMT_E_CAPACITY, "scene buffer too small");
     int rc = check_device_rows(d_rows);
     if(rc) return rc;
-    HIP_TRY(launch_build_scene(d_rows, n, nGrp, reinterpret_cast<float4*>(d_out), (hipStream_t)stream));
+    HIP_TRY(launch_build_scene(d_rows, n, nGrp, reinterpret_cast<float4*>(d_out), 3, (hipStream_t)stream));
     return KGMT_OK;
 }
 
@@ -989,7 +1022,7 @@ int kgmt_propagate_collide(const kgmt_config* cfg, const int32_t* d_parent_idx, 
     float4* d_scene     = nullptr;
     HIP_TRY(hipMallocAsync((void**)&d_scene, (size_t)std::max<uint32_t>(scene_pairs(n_obs), 1) * 2 * sizeof(float4),
                            sd.s));
-    hipError_t e = launch_build_scene(d_obstacles, n_obs, nGrp, d_scene, sd.s);
+    hipError_t e = launch_build_scene(d_obstacles, n_obs, nGrp, d_scene, model_wdim(cfg->model), sd.s);
     if(e == hipSuccess)
         e = launch_stage_propagate(cfg->model, sd.d, d_parent_idx, d_parent_states, n_parents, d_controls, d_steps,
                                    n_children, d_scene, nGrp, n_obs, d_child_samples, d_valid, d_r1, d_r2, sd.s);

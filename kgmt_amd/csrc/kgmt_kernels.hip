@@ -403,6 +403,8 @@ __global__ __launch_bounds__(NT, 4) void k_propagate(const QueryDesc* __restrict
                             case KGMT_MODEL_DUBINS_5D:
                                 child_tile<KGMT_MODEL_DUBINS_5D, NT, K>(qd, sts[q], base, P, sObs, L);
                                 break;
+                            // (a batch never mixes planar and 3D queries, so
+                            // a mixed batch holds no unicycle)
                             default: child_tile<KGMT_MODEL_QUADCOPTER, NT, K>(qd, sts[q], base, P, sObs, L); break;
                         }
                 }
@@ -764,11 +766,7 @@ __global__ __launch_bounds__(64) void k_paths(const QueryDesc* __restrict__ qds,
             const float x = T[n], y = T[(size_t)cap + n], z = T[2 * (size_t)cap + n];
             const int32_t par = P.parent[qd.nodeOff + n];
             if(par != -1) pc = pc + T[(size_t)(D - 1) * cap + n];
-            if(rows > 0)
-                {
-                    const float dx = px - x, dy = py - y, dz = pz - z;
-                    pl             = pl + __builtin_sqrtf(dx * dx + dy * dy + dz * dz);
-                }
+            if(rows > 0) pl = pl + pos_dist(qd, px, py, pz, x, y, z);
             px = x, py = y, pz = z;
             ++rows;
             n = par;
@@ -800,6 +798,18 @@ struct SceneLds
     float red[6][kBlock / 64];
 };
 
+// Face k (0..2 min x y z, 3..5 max x y z) of box i of rows of 2 * W floats; a
+// planar box (W = 2) spans z [-1, 1], which holds the planar positions' z = 0.
+template <int W>
+__device__ __forceinline__ float row_face(const float* __restrict__ rows, uint32_t i, int k)
+{
+    if constexpr(W == 3)
+        return rows[6 * (size_t)i + k];
+    else
+        return k == 2 ? -1.0f : k == 5 ? 1.0f : rows[4 * (size_t)i + (k < 3 ? k : k - 1)];
+}
+
+template <int W>
 __device__ void build_scene_wg(const float* __restrict__ rows, uint32_t n, uint32_t nGrp, float4* __restrict__ out,
                                SceneLds& L)
 {
@@ -808,7 +818,7 @@ __device__ void build_scene_wg(const float* __restrict__ rows, uint32_t n, uint3
     for(uint32_t i = threadIdx.x; i < n; i += kBlock)
         for(int d = 0; d < 3; ++d)
             {
-                const float c = 0.5f * (rows[6 * (size_t)i + d] + rows[6 * (size_t)i + 3 + d]);
+                const float c = 0.5f * (row_face<W>(rows, i, d) + row_face<W>(rows, i, 3 + d));
                 if(__builtin_isfinite(c)) lo[d] = c < lo[d] ? c : lo[d], hi[d] = c > hi[d] ? c : hi[d];
             }
     for(int d = 0; d < 3; ++d)
@@ -836,7 +846,7 @@ __device__ void build_scene_wg(const float* __restrict__ rows, uint32_t n, uint3
             uint32_t q[3];
             for(int d = 0; d < 3; ++d)
                 {
-                    const float c = 0.5f * (rows[6 * (size_t)i + d] + rows[6 * (size_t)i + 3 + d]);
+                    const float c = 0.5f * (row_face<W>(rows, i, d) + row_face<W>(rows, i, 3 + d));
                     float t       = hi[d] > lo[d] ? (c - lo[d]) / (hi[d] - lo[d]) : 0.0f;
                     if(!(t >= 0.0f)) t = 0.0f;
                     if(t > 1.0f) t = 1.0f;
@@ -855,7 +865,7 @@ __device__ void build_scene_wg(const float* __restrict__ rows, uint32_t n, uint3
                     const uint32_t kj = L.key[j];
                     rank += (kj < ki || (kj == ki && j < i)) ? 1u : 0u;
                 }
-            for(int d = 0; d < 6; ++d) L.box[rank][d] = rows[6 * (size_t)i + d];
+            for(int d = 0; d < 6; ++d) L.box[rank][d] = row_face<W>(rows, i, d);
         }
     __syncthreads();
     float4* box = out + 2 * nGrp;
@@ -898,14 +908,18 @@ __global__ __launch_bounds__(kBlock) void k_build_scenes(const QueryDesc* __rest
     __shared__ SceneLds L;
     const QueryDesc& qd = qds[blockIdx.x];
     if(qd.nObs == 0) return;
-    build_scene_wg(rows + 6 * (size_t)qd.rowOff, qd.nObs, qd.nGrp, pool + 2 * (size_t)qd.obsOff, L);
+    if(qd.wdim == 2)
+        build_scene_wg<2>(rows + 4 * (size_t)qd.rowOff, qd.nObs, qd.nGrp, pool + 2 * (size_t)qd.obsOff, L);
+    else
+        build_scene_wg<3>(rows + 6 * (size_t)qd.rowOff, qd.nObs, qd.nGrp, pool + 2 * (size_t)qd.obsOff, L);
 }
 
+template <int W>
 __global__ __launch_bounds__(kBlock) void k_build_scene(const float* __restrict__ rows, uint32_t n, uint32_t nGrp,
                                                         float4* __restrict__ out)
 {
     __shared__ SceneLds L;
-    build_scene_wg(rows, n, nGrp, out, L);
+    build_scene_wg<W>(rows, n, nGrp, out, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -1123,6 +1137,9 @@ static hipError_t launch_propagate_nt(int model, const QueryDesc* qds, QueryStat
             case KGMT_MODEL_QUADCOPTER:
                 hipLaunchKernelGGL((k_propagate<KGMT_MODEL_QUADCOPTER, NT, K>), g, b, lds, s, qds, sts, P, W, parity);
                 break;
+            case KGMT_MODEL_UNICYCLE:
+                hipLaunchKernelGGL((k_propagate<KGMT_MODEL_UNICYCLE, NT, K>), g, b, lds, s, qds, sts, P, W, parity);
+                break;
             default: hipLaunchKernelGGL((k_propagate<-1, NT, K>), g, b, lds, s, qds, sts, P, W, parity); break;
         }
     return hipGetLastError();
@@ -1168,10 +1185,13 @@ hipError_t launch_build_scenes(const QueryDesc* qds, const float* rows, float4* 
     return hipGetLastError();
 }
 
-hipError_t launch_build_scene(const float* rows, uint32_t n, uint32_t nGrp, float4* out, hipStream_t s)
+hipError_t launch_build_scene(const float* rows, uint32_t n, uint32_t nGrp, float4* out, int wdim, hipStream_t s)
 {
     if(n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_build_scene, dim3(1), dim3(kBlock), 0, s, rows, n, nGrp, out);
+    if(wdim == 2)
+        hipLaunchKernelGGL(k_build_scene<2>, dim3(1), dim3(kBlock), 0, s, rows, n, nGrp, out);
+    else
+        hipLaunchKernelGGL(k_build_scene<3>, dim3(1), dim3(kBlock), 0, s, rows, n, nGrp, out);
     return hipGetLastError();
 }
 
@@ -1202,6 +1222,7 @@ hipError_t launch_stage_draw(const QueryDesc* qd, int C, uint32_t query, uint32_
     if(n == 0) return hipSuccess;
     switch(C)
         {
+            case 2: hipLaunchKernelGGL(k_stage_draw<2>, g, b, 0, s, qd, query, itr, u0, n, controls, steps, ufront); break;
             case 3: hipLaunchKernelGGL(k_stage_draw<3>, g, b, 0, s, qd, query, itr, u0, n, controls, steps, ufront); break;
             case 4: hipLaunchKernelGGL(k_stage_draw<4>, g, b, 0, s, qd, query, itr, u0, n, controls, steps, ufront); break;
             default: return hipErrorInvalidValue;
@@ -1226,6 +1247,7 @@ hipError_t launch_stage_propagate(int model, const QueryDesc* qd, const int32_t*
             case KGMT_MODEL_DUBINS_AIRPLANE: KGMT_STAGE_PROP(KGMT_MODEL_DUBINS_AIRPLANE); break;
             case KGMT_MODEL_DUBINS_5D: KGMT_STAGE_PROP(KGMT_MODEL_DUBINS_5D); break;
             case KGMT_MODEL_QUADCOPTER: KGMT_STAGE_PROP(KGMT_MODEL_QUADCOPTER); break;
+            case KGMT_MODEL_UNICYCLE: KGMT_STAGE_PROP(KGMT_MODEL_UNICYCLE); break;
             default: return hipErrorInvalidValue;
         }
 #undef KGMT_STAGE_PROP

@@ -25,7 +25,8 @@ struct Launch
 uint32_t scene_groups(uint32_t n);
 uint32_t scene_pairs(uint32_t n);
 hipError_t launch_build_scenes(const QueryDesc* qds, const float* rows, float4* pool, uint32_t nq, hipStream_t s);
-hipError_t launch_build_scene(const float* rows, uint32_t n, uint32_t nGrp, float4* out, hipStream_t s);
+// rows of 2 * wdim floats (wdim 2: planar boxes, z range [-1, 1])
+hipError_t launch_build_scene(const float* rows, uint32_t n, uint32_t nGrp, float4* out, int wdim, hipStream_t s);
 
 // propagate launch shapes that are built
 bool launch_supported(uint32_t block, uint32_t lanes);

@@ -16,7 +16,7 @@ import os
 import numpy as np
 
 from . import _lib
-from .abi import KGMT_MAX_SAMPLE, KgmtConfig, KgmtProfile, KgmtResult, dims
+from .abi import KGMT_MAX_SAMPLE, KgmtConfig, KgmtProfile, KgmtResult, dims, wdim
 from .workloads import config as workload_config
 
 
@@ -33,9 +33,10 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def _obstacle_arg(obstacles, n):
+def _obstacle_arg(obstacles, n, w=3):
     """(pointer, on_device, keepalive) for host arrays, torch tensors or raw
-    device addresses (ints, like the reference's d_obstacles_ptr)."""
+    device addresses (ints, like the reference's d_obstacles_ptr); rows of
+    2w floats."""
     if obstacles is None:
         return None, 0, None
     if isinstance(obstacles, int):
@@ -43,10 +44,12 @@ def _obstacle_arg(obstacles, n):
     mod = type(obstacles).__module__
     if mod.startswith("torch"):
         t = obstacles.contiguous().float()
+        if t.numel() < 2 * w * n:
+            raise ValueError(f"obstacle tensor shorter than {2 * w} * count")
         return C.c_void_p(t.data_ptr()), 1 if t.is_cuda else 0, t
     a = _f32(obstacles).reshape(-1)
-    if a.size < 6 * n:
-        raise ValueError("obstacle array shorter than 6 * count")
+    if a.size < 2 * w * n:
+        raise ValueError(f"obstacle array shorter than {2 * w} * count")
     return _ptr(a), 0, a
 
 
@@ -121,7 +124,7 @@ class KGMT(Planner):
         start = _f32(h_initial, self.D)
         goal = _f32(h_goal, self.D)
         n = int(h_obstaclesCount)
-        p, on_dev, keep = _obstacle_arg(d_obstacles_ptr, n)
+        p, on_dev, keep = _obstacle_arg(d_obstacles_ptr, n, wdim(self.cfg.model))
         res = KgmtResult()
         _lib.check(_lib.lib().kgmt_plan(self._h, _ptr(start), _ptr(goal), p, n, on_dev, C.byref(res)))
         del keep
@@ -219,6 +222,10 @@ class BatchKGMT:
             obs = np.zeros(6, np.float32)
         offs = np.ascontiguousarray(offsets, np.uint32)
         cnts = np.ascontiguousarray(counts, np.uint32)
+        w = wdim(self.cfgs[0].model)
+        end = int((offs.astype(np.int64) + cnts).max()) if self.n else 0
+        if obs.size < 2 * w * end and end:
+            raise ValueError(f"obstacle array shorter than {2 * w} * (offset + count)")
         res = (KgmtResult * self.n)()
         fin = np.zeros(self.n, np.float32)
         _lib.check(_lib.lib().kgmt_batch_plan(self._h, _ptr(starts), _ptr(goals), _ptr(obs), _ptr(offs),

@@ -8,6 +8,7 @@ Workloads:
   di6d      6D double integrator, unit cube, 20 boxes sides U[0.05,0.2], 2^17 nodes
   dubins5d  5D Dubins airplane, 100 boxes sides U[0.02,0.1], 2^18 nodes
   quad12d   12D quadcopter, three slotted walls + 50 boxes, 2^19 nodes
+  unicycle2d planar unicycle, unit square, 30 boxes sides U[0.05,0.15], 2^17 nodes
   batch     512 queries, 50/50 double integrator / quadcopter, capacity
             log-uniform in [2^12, 2^17], 10..500 boxes, start/goal >= 0.6 apart
 """
@@ -20,7 +21,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .abi import (KGMT_DI_REFERENCE, KGMT_MAX_SAMPLE, KGMT_MODEL_DOUBLE_INTEGRATOR, KGMT_MODEL_DUBINS_5D,
-                  KGMT_MODEL_QUADCOPTER, KgmtConfig, dims)
+                  KGMT_MODEL_QUADCOPTER, KGMT_MODEL_UNICYCLE, KgmtConfig, dims)
 
 PI = float(np.float32(math.pi))
 
@@ -37,6 +38,7 @@ def _base(**kw) -> KgmtConfig:
     c.quad_mass, c.quad_ix, c.quad_iy, c.quad_iz = 1.0, 1.0, 1.0, 2.0
     c.quad_nu, c.quad_mu, c.gravity = 10e-3, 2e-6, -9.81
     c.pitch_min, c.pitch_max = -PI / 3.0, PI / 3.0
+    c.uni_length = 1.0
     c.seed = 1
     return c.replace(**kw)
 
@@ -60,12 +62,16 @@ def config(name: str, **overrides) -> KgmtConfig:
                   goal_thresh=0.05, ws_min=0.0, ws_max=1.0, v_min=-5.0, v_max=5.0,
                   ctrl_min=[0.0, -0.05, -0.05, -0.05], ctrl_max=[19.62, 0.05, 0.05, 0.05],
                   quad_ix=0.01, quad_iy=0.01, quad_iz=0.02, r1_len=10, r2_len=4)
+    elif name == "unicycle2d":  # steering / UNI_LENGTH of config.h:266-270, A of config.h:34-35
+        c = _base(model=KGMT_MODEL_UNICYCLE, capacity=1 << 17, dt=0.1, steps_min=1, steps_max=10,
+                  goal_thresh=0.05, ws_min=0.0, ws_max=1.0, v_min=0.0, v_max=0.0,
+                  ctrl_min=[-0.2, -PI / 2.0, 0.0, 0.0], ctrl_max=[0.2, PI / 2.0, 0.0, 0.0], r1_len=16, r2_len=4)
     else:
         raise KeyError(name)
     return c.replace(**overrides) if overrides else c
 
 
-PRESET_OF = {"reference_quad": 0, "di6d": 1, "dubins5d": 2, "quad12d": 3}
+PRESET_OF = {"reference_quad": 0, "di6d": 1, "dubins5d": 2, "quad12d": 3, "unicycle2d": 4}
 
 
 # ---------------------------------------------------------------------------
@@ -76,7 +82,7 @@ class Query:
     cfg: KgmtConfig
     start: np.ndarray          # D floats
     goal: np.ndarray           # D floats
-    obstacles: np.ndarray      # (n, 6) float32 rows xmin ymin zmin xmax ymax zmax
+    obstacles: np.ndarray      # (n, 2W) float32 rows: min corner, max corner (W = 2 planar, else 3)
     name: str = ""
     meta: dict = field(default_factory=dict)
 
@@ -86,20 +92,21 @@ def _rng(seed: int, stream: int) -> np.random.Generator:
 
 
 def box_ball_distance(boxes: np.ndarray, c: np.ndarray) -> np.ndarray:
-    lo, hi = boxes[:, :3], boxes[:, 3:]
+    w = boxes.shape[1] // 2
+    lo, hi = boxes[:, :w], boxes[:, w:]
     d = np.maximum(np.maximum(lo - c, 0.0), c - hi)
     return np.sqrt((d * d).sum(axis=1))
 
 
-def random_boxes(rng, n, side_lo, side_hi, keep_clear=(), ws=(0.0, 1.0)) -> np.ndarray:
-    """n boxes with centre U(ws)^3 and sides U(side_lo, side_hi), clipped to
+def random_boxes(rng, n, side_lo, side_hi, keep_clear=(), ws=(0.0, 1.0), dim: int = 3) -> np.ndarray:
+    """n boxes with centre U(ws)^dim and sides U(side_lo, side_hi), clipped to
     the workspace; a box that comes closer than r to any (centre, r) in
     keep_clear is drawn again."""
-    out = np.zeros((n, 6), np.float64)
+    out = np.zeros((n, 2 * dim), np.float64)
     for i in range(n):
         for _ in range(10000):
-            c = rng.uniform(ws[0], ws[1], 3)
-            s = rng.uniform(side_lo, side_hi, 3)
+            c = rng.uniform(ws[0], ws[1], dim)
+            s = rng.uniform(side_lo, side_hi, dim)
             b = np.concatenate([np.clip(c - s / 2, ws[0], ws[1]), np.clip(c + s / 2, ws[0], ws[1])])
             if all(box_ball_distance(b[None], np.asarray(p))[0] >= r for p, r in keep_clear):
                 out[i] = b
@@ -110,18 +117,22 @@ def random_boxes(rng, n, side_lo, side_hi, keep_clear=(), ws=(0.0, 1.0)) -> np.n
 
 
 def _sample(cfg: KgmtConfig, pos) -> np.ndarray:
+    """Sample row at rest at position `pos` (W floats)."""
     s = np.zeros(cfg.dims[2], np.float32)
-    s[:3] = pos
+    s[:cfg.wdim] = pos
     return s
 
 
 def make_query(name: str, seed: int, **cfg_overrides) -> Query:
-    """One query of workload `name` (di6d, dubins5d, quad12d) for `seed`."""
+    """One query of workload `name` (di6d, dubins5d, quad12d, unicycle2d) for `seed`."""
     cfg = config(name, seed=seed + 1, **cfg_overrides)
-    rng = _rng(seed, {"di6d": 1, "dubins5d": 2, "quad12d": 3}[name])
-    start, goal = np.array([0.05, 0.05, 0.05]), np.array([0.95, 0.95, 0.95])
+    rng = _rng(seed, {"di6d": 1, "dubins5d": 2, "quad12d": 3, "unicycle2d": 5}[name])
+    w = cfg.wdim
+    start, goal = np.full(w, 0.05), np.full(w, 0.95)
     clear = [(start, 0.05 + 1e-3), (goal, cfg.goal_thresh + 1e-3)]
-    if name == "di6d":
+    if name == "unicycle2d":
+        obs = random_boxes(rng, 30, 0.05, 0.15, clear, dim=2)
+    elif name == "di6d":
         obs = random_boxes(rng, 20, 0.05, 0.2, clear)
     elif name == "dubins5d":
         obs = random_boxes(rng, 100, 0.02, 0.1, clear)
@@ -218,8 +229,12 @@ def make_refine_batch(n: int = 4096, seed: int = 0, steps_lo: int = 20, steps_hi
 
 def pack_batch(queries):
     """Arrays for kgmt_batch_plan: configs, starts/goals (n, 17), obstacle
-    rows, per-query offsets and counts."""
+    rows (2W floats; every query of a batch has the same W), per-query
+    offsets and counts."""
     n = len(queries)
+    w = queries[0].cfg.wdim if n else 3
+    if any(q.cfg.wdim != w for q in queries):
+        raise ValueError("a batch cannot mix planar and 3D queries")
     starts = np.zeros((n, KGMT_MAX_SAMPLE), np.float32)
     goals = np.zeros((n, KGMT_MAX_SAMPLE), np.float32)
     offs = np.zeros(n, np.uint32)
@@ -230,9 +245,9 @@ def pack_batch(queries):
         starts[i, :len(q.start)] = q.start
         goals[i, :len(q.goal)] = q.goal
         offs[i], cnts[i] = o, len(q.obstacles)
-        rows.append(q.obstacles.reshape(-1, 6))
+        rows.append(q.obstacles.reshape(-1, 2 * w))
         o += len(q.obstacles)
-    obs = np.concatenate(rows).astype(np.float32) if rows else np.zeros((0, 6), np.float32)
+    obs = np.concatenate(rows).astype(np.float32) if rows else np.zeros((0, 2 * w), np.float32)
     return [q.cfg for q in queries], starts, goals, np.ascontiguousarray(obs), offs, cnts
 
 

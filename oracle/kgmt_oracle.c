@@ -133,8 +133,16 @@ static int dims(int model, int* S, int* C)
             case KGMT_MODEL_DUBINS_AIRPLANE: *S = 6, *C = 3; return 0;
             case KGMT_MODEL_QUADCOPTER: *S = 12, *C = 4; return 0;
             case KGMT_MODEL_DUBINS_5D: *S = 5, *C = 3; return 0;
+            case KGMT_MODEL_UNICYCLE: *S = 4, *C = 2; return 0;
             default: return -1;
         }
+}
+
+/* workspace dimension W_DIM: the unicycle is planar (x y), the others move in
+ * x y z */
+int orc_wdim(const kgmt_config* cfg)
+{
+    return cfg->model == KGMT_MODEL_UNICYCLE ? 2 : 3;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -205,10 +213,33 @@ int orc_segment_valid(const float* p0, const float* p1, const float* obs, uint32
     return 1;
 }
 
+/* The same test with W_DIM = 2: rows xmin ymin xmax ymax. */
+int orc_segment_valid2(const float* p0, const float* p1, const float* obs, uint32_t n_obs)
+{
+    float bbMin[2], bbMax[2];
+    for(int d = 0; d < 2; ++d)
+        {
+            if(p0[d] > p1[d])
+                bbMin[d] = p1[d], bbMax[d] = p0[d];
+            else
+                bbMin[d] = p0[d], bbMax[d] = p1[d];
+        }
+    for(uint32_t o = 0; o < n_obs; ++o)
+        {
+            const float* b = obs + 4 * (size_t)o;
+            int separated  = 0;
+            for(int d = 0; d < 2; ++d)
+                if(bbMax[d] <= b[d] || b[2 + d] <= bbMin[d]) separated = 1;
+            if(!separated) return 0;
+        }
+    return 1;
+}
+
 /* ------------------------------------------------------------------------ */
 /* regions: getRegion Graph.cu:99-153 and getSubRegion :155-205 for a        */
-/* workspace-only grid (z fastest). Region minimum corner as computed by     */
-/* initializeRegions_kernel Graph.cu:58-70: W_MIN + index * W_R1_SIZE.       */
+/* workspace-only grid (last axis fastest) over W_DIM axes. Region minimum   */
+/* corner as computed by initializeRegions_kernel Graph.cu:58-70:            */
+/* W_MIN + index * W_R1_SIZE.                                                */
 /* ------------------------------------------------------------------------ */
 /* (int)f clamped to [0, L-1], as Graph.cu:107-109. The clamp is applied in
  * float first so that NaN, infinities and |f| >= 2^31 (undefined for a C
@@ -225,16 +256,16 @@ static int grid_index(float f, int L)
 int orc_num_regions(const kgmt_config* cfg, int32_t* r1, int32_t* r2, int32_t* per_r1)
 {
     int32_t L = cfg->r1_len, L2 = cfg->r2_len;
-    *r1     = L * L * L;
-    *per_r1 = L2 * L2 * L2;
-    *r2     = *r1 * *per_r1;
+    *r1 = 1, *per_r1 = 1;
+    for(int i = 0; i < orc_wdim(cfg); ++i) *r1 *= L, *per_r1 *= L2;
+    *r2 = *r1 * *per_r1;
     return 0;
 }
 
 int orc_region(const kgmt_config* cfg, const float* pos)
 {
     int L = cfg->r1_len, region = 0, factor = 1;
-    for(int i = 2; i >= 0; --i)
+    for(int i = orc_wdim(cfg) - 1; i >= 0; --i)
         {
             int index = grid_index(((float)L * (pos[i] - cfg->ws_min)) / (cfg->ws_max - cfg->ws_min), L);
             region += factor * index;
@@ -245,23 +276,23 @@ int orc_region(const kgmt_config* cfg, const float* pos)
 
 int orc_subregion(const kgmt_config* cfg, const float* pos, int r1)
 {
-    int L = cfg->r1_len, L2 = cfg->r2_len;
+    int L = cfg->r1_len, L2 = cfg->r2_len, W = orc_wdim(cfg);
     float size = (cfg->ws_max - cfg->ws_min) / (float)L;
     int idx[3], t = r1;
-    for(int i = 2; i >= 0; --i)
+    for(int i = W - 1; i >= 0; --i)
         {
             idx[i] = t % L;
             t /= L;
         }
     int local = 0, factor = 1;
-    for(int i = 2; i >= 0; --i)
+    for(int i = W - 1; i >= 0; --i)
         {
             float mn  = cfg->ws_min + (float)idx[i] * size;
             int index = grid_index(((float)L2 * (pos[i] - mn)) / size, L2);
             local += factor * index;
             factor *= L2;
         }
-    return r1 * (L2 * L2 * L2) + local;
+    return r1 * factor + local;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -271,6 +302,46 @@ static int ws_out(const kgmt_config* cfg, float x, float y, float z)
 {
     return x < cfg->ws_min || x > cfg->ws_max || y < cfg->ws_min || y > cfg->ws_max || z < cfg->ws_min ||
            z > cfg->ws_max;
+}
+
+/* statePropagator.cu:13-71: Euler steps of the planar unicycle, workspace
+ * check on x y, then the planar segment test. tan(steering) is sin/cos of
+ * the sincos above (the reference calls tan; one correctly rounded quotient
+ * of two faithfully rounded values). */
+static int prop_unicycle(const kgmt_config* cfg, const float* x0, const float* ctrl, int32_t steps, const float* obs,
+                         uint32_t n_obs, float* x1)
+{
+    float h = cfg->dt, a = ctrl[0], steer = ctrl[1];
+    float x = x0[0], y = x0[1], theta = x0[2], v = x0[3];
+    float ss, cs;
+    orc_sincosf(steer, &ss, &cs);
+    float tn  = ss / cs;
+    int valid = 1;
+    for(int i = 0; i < steps; ++i)
+        {
+            float p0[2] = {x, y};
+            float st, ct;
+            orc_sincosf(theta, &st, &ct);
+            x     = x + (v * ct) * h;
+            y     = y + (v * st) * h;
+            theta = theta + ((v / cfg->uni_length) * tn) * h;
+            v     = v + a * h;
+            if(x < cfg->ws_min || x > cfg->ws_max || y < cfg->ws_min || y > cfg->ws_max)
+                {
+                    valid = 0;
+                    break;
+                }
+            float p1[2] = {x, y};
+            if(!orc_segment_valid2(p0, p1, obs, n_obs))
+                {
+                    valid = 0;
+                    break;
+                }
+        }
+    x1[0] = x, x1[1] = y, x1[2] = theta, x1[3] = v;
+    x1[4] = a, x1[5] = steer;
+    x1[6] = cfg->dt * (float)steps;
+    return valid;
 }
 
 /* statePropagator.cu:76-142. Reference-literal position update (Q1) or exact. */
@@ -489,6 +560,7 @@ int orc_propagate(const kgmt_config* cfg, const float* x0, const float* ctrl, in
             case KGMT_MODEL_DUBINS_AIRPLANE: return prop_dubins(cfg, x0, ctrl, steps, obs, n_obs, x1, 0);
             case KGMT_MODEL_DUBINS_5D: return prop_dubins(cfg, x0, ctrl, steps, obs, n_obs, x1, 1);
             case KGMT_MODEL_QUADCOPTER: return prop_quad(cfg, x0, ctrl, steps, obs, n_obs, x1);
+            case KGMT_MODEL_UNICYCLE: return prop_unicycle(cfg, x0, ctrl, steps, obs, n_obs, x1);
             default: return -1;
         }
 }
@@ -497,26 +569,26 @@ int orc_propagate(const kgmt_config* cfg, const float* x0, const float* ctrl, in
 /* FP64 twin of the dynamics (libm sin/cos, double arithmetic): the pin of   */
 /* the FP32 restatement. Same step structure and checks.                     */
 /* ------------------------------------------------------------------------ */
-static int seg_valid_f64(const double* p0, const double* p1, const double* obs, uint32_t n_obs)
+static int seg_valid_f64(const double* p0, const double* p1, const double* obs, uint32_t n_obs, int W)
 {
     for(uint32_t o = 0; o < n_obs; ++o)
         {
-            const double* b = obs + 6 * (size_t)o;
+            const double* b = obs + 2 * W * (size_t)o;
             int separated   = 0;
-            for(int d = 0; d < 3; ++d)
+            for(int d = 0; d < W; ++d)
                 {
                     double lo = p0[d] < p1[d] ? p0[d] : p1[d];
                     double hi = p0[d] < p1[d] ? p1[d] : p0[d];
-                    if(hi <= b[d] || b[3 + d] <= lo) separated = 1;
+                    if(hi <= b[d] || b[W + d] <= lo) separated = 1;
                 }
             if(!separated) return 0;
         }
     return 1;
 }
 
-static int ws_out_f64(const kgmt_config* cfg, const double* p)
+static int ws_out_f64(const kgmt_config* cfg, const double* p, int W)
 {
-    for(int d = 0; d < 3; ++d)
+    for(int d = 0; d < W; ++d)
         if(p[d] < cfg->ws_min || p[d] > cfg->ws_max) return 1;
     return 0;
 }
@@ -547,7 +619,7 @@ int orc_propagate_f64(const kgmt_config* cfg, const double* x0, const double* ct
                       uint32_t n_obs, double* x1)
 {
     double h  = (double)cfg->dt;
-    int valid = 1, S, C;
+    int valid = 1, S, C, W = orc_wdim(cfg);
     if(dims(cfg->model, &S, &C)) return -1;
     double s[12];
     memcpy(s, x0, sizeof(double) * (size_t)S);
@@ -555,7 +627,15 @@ int orc_propagate_f64(const kgmt_config* cfg, const double* x0, const double* ct
         {
             double p0[3] = {s[0], s[1], s[2]};
             int bad      = 0;
-            if(cfg->model == KGMT_MODEL_DOUBLE_INTEGRATOR)
+            if(cfg->model == KGMT_MODEL_UNICYCLE)
+                {
+                    double v = s[3], th = s[2];
+                    s[0] += v * cos(th) * h;
+                    s[1] += v * sin(th) * h;
+                    s[2] += v / (double)cfg->uni_length * tan(ctrl[1]) * h;
+                    s[3] = v + ctrl[0] * h;
+                }
+            else if(cfg->model == KGMT_MODEL_DOUBLE_INTEGRATOR)
                 {
                     for(int d = 0; d < 3; ++d)
                         {
@@ -603,8 +683,8 @@ int orc_propagate_f64(const kgmt_config* cfg, const double* x0, const double* ct
                     for(int d = 6; d < 9; ++d)
                         if(s[d] < cfg->v_min || s[d] > cfg->v_max) bad = 1;
                 }
-            if(!bad && ws_out_f64(cfg, s)) bad = 1;
-            if(!bad && !seg_valid_f64(p0, s, obs, n_obs)) bad = 1;
+            if(!bad && ws_out_f64(cfg, s, W)) bad = 1;
+            if(!bad && !seg_valid_f64(p0, s, obs, n_obs, W)) bad = 1;
             if(bad)
                 {
                     valid = 0;
@@ -651,8 +731,9 @@ void orc_score(const kgmt_config* cfg, const int32_t* total, const int32_t* vali
     int32_t R1, R2, P;
     orc_num_regions(cfg, &R1, &R2, &P);
     float size = (cfg->ws_max - cfg->ws_min) / (float)cfg->r1_len;
-    float vol  = size * size * size;
-    float eps  = cfg->epsilon;
+    /* W_R1_VOL: the region's measure, W_R1_SIZE^W_DIM (config.h:251 for W_DIM 3) */
+    float vol = orc_wdim(cfg) == 2 ? size * size : size * size * size;
+    float eps = cfg->epsilon;
     int32_t P2 = 1;
     while(P2 < R1) P2 <<= 1;
     float* a = (float*)calloc((size_t)P2, sizeof(float));
@@ -694,9 +775,12 @@ float orc_faccept(const kgmt_config* cfg, uint32_t itr, uint32_t tree_size, uint
 /* Graph.cu:207-227, updateFrontier :380-403 and updateFrontier_kernel       */
 /* :305-378.                                                                 */
 /* ------------------------------------------------------------------------ */
-static float dist3(const float* a, const float* b)
+/* helper.cuh:138-152: Euclidean distance of the W_DIM position coordinates */
+static float pos_dist(const kgmt_config* cfg, const float* a, const float* b)
 {
-    float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+    float dx = a[0] - b[0], dy = a[1] - b[1];
+    if(orc_wdim(cfg) == 2) return sqrtf(dx * dx + dy * dy);
+    float dz = a[2] - b[2];
     return sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
@@ -820,7 +904,7 @@ int orc_plan(const kgmt_config* cfg, const float* start, const float* goal, cons
                     const float* x1 = child + (size_t)u * D;
                     memcpy(samples + (size_t)t * D, x1, sizeof(float) * D);
                     parents[t]  = parentU[u];
-                    float dg    = dist3(x1, goal);
+                    float dg    = pos_dist(cfg, x1, goal);
                     costs[t]    = dg;
                     frontier[t] = 1;
                     int r1      = orc_region(cfg, x1);
@@ -866,7 +950,7 @@ int orc_plan(const kgmt_config* cfg, const float* start, const float* goal, cons
                     const float* row = samples + (size_t)n * D;
                     memcpy(path + (size_t)rows * D, row, sizeof(float) * D);
                     if(parents[n] != -1) st->path_cost += row[D - 1];
-                    if(prev) st->path_length += dist3(prev, row);
+                    if(prev) st->path_length += pos_dist(cfg, prev, row);
                     prev = row;
                     ++rows;
                     n = parents[n];
@@ -922,7 +1006,7 @@ int orc_plan_many(const kgmt_config* cfgs, uint32_t n_q, const float* starts, co
                 struct timespec t0, t1;
                 clock_gettime(CLOCK_MONOTONIC, &t0);
                 int r = orc_plan(&cfgs[q], starts + (size_t)q * KGMT_MAX_SAMPLE,
-                                 goals + (size_t)q * KGMT_MAX_SAMPLE, obs + 6 * (size_t)obs_off[q], obs_cnt[q], samples,
+                                 goals + (size_t)q * KGMT_MAX_SAMPLE, obs + 2 * orc_wdim(&cfgs[q]) * (size_t)obs_off[q], obs_cnt[q], samples,
                                  parents, costs, path, NULL, NULL, NULL, NULL, &st_out[q]);
                 if(r != KGMT_OK) rc = r;
                 clock_gettime(CLOCK_MONOTONIC, &t1);

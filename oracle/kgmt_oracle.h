@@ -31,6 +31,8 @@ float orc_draw_react(const kgmt_config* cfg, uint32_t query, uint32_t itr, uint3
 
 /* geometry ---------------------------------------------------------------- */
 int orc_segment_valid(const float* p0, const float* p1, const float* obs, uint32_t n_obs);
+int orc_segment_valid2(const float* p0, const float* p1, const float* obs, uint32_t n_obs);
+int orc_wdim(const kgmt_config* cfg);
 int orc_region(const kgmt_config* cfg, const float* pos);
 int orc_subregion(const kgmt_config* cfg, const float* pos, int r1);
 int orc_num_regions(const kgmt_config* cfg, int32_t* r1, int32_t* r2, int32_t* per_r1);

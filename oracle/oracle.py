@@ -42,6 +42,7 @@ _SIG = {
     "orc_draw_child": (None, [_cfgp, _u32, _u32, _u32, _P, _P, _P]),
     "orc_draw_react": (C.c_float, [_cfgp, _u32, _u32, _u32]),
     "orc_segment_valid": (C.c_int, [_P, _P, _P, _u32]),
+    "orc_segment_valid2": (C.c_int, [_P, _P, _P, _u32]),
     "orc_region": (C.c_int, [_cfgp, _P]),
     "orc_subregion": (C.c_int, [_cfgp, _P, C.c_int]),
     "orc_propagate": (C.c_int, [_cfgp, _P, _P, C.c_int32, _P, _u32, _P]),
@@ -136,6 +137,14 @@ def segment_valid(p0, p1, obs):
     return out
 
 
+def segment_valid2(p0, p1, obs):
+    """Planar segment test, rows xmin ymin xmax ymax (one segment)."""
+    p0 = _c(p0, np.float32)
+    p1 = _c(p1, np.float32)
+    ob = _c(obs, np.float32).reshape(-1)
+    return bool(lib().orc_segment_valid2(_p(p0), _p(p1), _p(ob) if ob.size else None, ob.size // 4))
+
+
 def regions(cfg, pos):
     pos = _c(pos, np.float32).reshape(-1, 3)
     r1 = np.zeros(len(pos), np.int32)
@@ -159,7 +168,8 @@ def propagate_children(cfg, parent_idx, parent_states, controls, steps, obs):
     r1 = np.zeros(n, np.int32)
     r2 = np.zeros(n, np.int32)
     rc = lib().orc_propagate_children(C.byref(cfg), _p(pidx), _p(ps), ps.shape[1], _p(ctrl), _p(st), n,
-                                      _p(ob) if ob.size else None, ob.size // 6, _p(child), _p(valid), _p(r1), _p(r2))
+                                      _p(ob) if ob.size else None, ob.size // (2 * cfg.wdim), _p(child), _p(valid),
+                                      _p(r1), _p(r2))
     assert rc == 0, rc
     return child, valid, r1, r2
 
@@ -175,7 +185,7 @@ def propagate_f64(cfg, x0s, ctrls, steps, obs):
     x1 = np.zeros((n, D), np.float64)
     valid = np.zeros(n, np.uint8)
     rc = lib().orc_propagate_children_f64(C.byref(cfg), _p(x0), _p(u), _p(st), n, _p(ob) if ob.size else None,
-                                          ob.size // 6, _p(x1), _p(valid))
+                                          ob.size // (2 * cfg.wdim), _p(x1), _p(valid))
     assert rc == 0, rc
     return x1, valid
 
@@ -206,7 +216,7 @@ def region_update(cfg, r1, r2, valid, ufront, score, active_prev, active_next, t
 
 
 def score(cfg, total, valid_count, active_bytes):
-    R1 = cfg.r1_len ** 3
+    R1 = cfg.regions
     out = np.zeros(R1, np.float32)
     lib().orc_score(C.byref(cfg), _p(_c(total, np.int32)), _p(_c(valid_count, np.int32)),
                     _p(_c(active_bytes, np.uint8)), _p(out))
@@ -225,7 +235,7 @@ def plan(cfg: KgmtConfig, start, goal, obs, log_iters: bool = False):
     parents = np.zeros(N, np.int32)
     costs = np.zeros(N, np.float32)
     path = np.zeros((cfg.max_iter + 2, D), np.float32)
-    R1 = cfg.r1_len ** 3
+    R1 = cfg.regions
     total = np.zeros(R1, np.int32)
     valid = np.zeros(R1, np.int32)
     score_ = np.zeros(R1, np.float32)
@@ -236,7 +246,7 @@ def plan(cfg: KgmtConfig, start, goal, obs, log_iters: bool = False):
     g = np.zeros(KGMT_MAX_SAMPLE, np.float32)
     g[:D] = np.asarray(goal, np.float32)[:D]
     ob = _c(obs, np.float32).reshape(-1)
-    rc = lib().orc_plan(C.byref(cfg), _p(s), _p(g), _p(ob) if ob.size else None, ob.size // 6, _p(samples),
+    rc = lib().orc_plan(C.byref(cfg), _p(s), _p(g), _p(ob) if ob.size else None, ob.size // (2 * cfg.wdim), _p(samples),
                         _p(parents), _p(costs), _p(path), _p(total), _p(valid), _p(score_),
                         _p(log) if log is not None else None, C.byref(st))
     assert rc == 0, rc

@@ -1,7 +1,7 @@
 """Writes tests/golden/plan_stats_<workload>.json: the CPU oracle's planning
 outcome for every seed of the statistical parity sets (50 seeds of the 6D
 double integrator, 200 of the 5D Dubins airplane, 200 of the 12D
-quadcopter), one record per seed:
+quadcopter, 200 of the planar unicycle), one record per seed:
 
   [seed, iterations, tree_size, goal_index, path_rows, path_cost bits,
    path_length bits, expanded]
@@ -26,7 +26,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-SEEDS = {"di6d": 50, "dubins5d": 200, "quad12d": 200}
+SEEDS = {"di6d": 50, "dubins5d": 200, "quad12d": 200, "unicycle2d": 200}
 
 
 def f32_bits(x: float) -> int:

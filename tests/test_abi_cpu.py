@@ -33,8 +33,9 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_struct_layouts():
-    assert C.sizeof(KgmtConfig) == 160 and C.sizeof(KgmtResult) == 40
-    assert KgmtConfig.seed.offset == 152 and KgmtConfig.query_id.offset == 148
+    assert C.sizeof(KgmtConfig) == 168 and C.sizeof(KgmtResult) == 40
+    assert KgmtConfig.query_id.offset == 148 and KgmtConfig.uni_length.offset == 152
+    assert KgmtConfig.seed.offset == 160
 
 
 def test_presets_match_python_workloads():
@@ -50,8 +51,9 @@ def test_config_validation():
     L = _lib.lib()
     good = W.config("di6d")
     bad = [
-        dict(model=0),  # unicycle is not built
         dict(model=9),
+        dict(model=-1),
+        dict(reserved=1),
         dict(capacity=1),
         dict(dt=0.0),
         dict(steps_min=5, steps_max=4),
@@ -69,15 +71,21 @@ def test_config_validation():
         assert rc in (KGMT_E_ARG, KGMT_E_UNSUPPORTED), kw
         assert L.kgmt_last_error()
     assert L.kgmt_config_check(None) == KGMT_E_ARG
+    uni = W.config("unicycle2d")
+    assert L.kgmt_config_check(C.byref(uni)) == KGMT_OK
+    # planar grids: r1_len^2 regions, so 21 is fine there; the wheel base must be positive
+    assert L.kgmt_config_check(C.byref(uni.replace(r1_len=21))) == KGMT_OK
+    for kw in (dict(uni_length=0.0), dict(uni_length=float("nan")), dict(r1_len=91)):
+        assert L.kgmt_config_check(C.byref(uni.replace(**kw))) == KGMT_E_ARG, kw
 
 
 def test_model_dims():
     L = _lib.lib()
     s, c, d = C.c_int32(), C.c_int32(), C.c_int32()
-    for m, want in [(1, (6, 3, 10)), (2, (6, 3, 10)), (3, (12, 4, 17)), (4, (5, 3, 9))]:
+    for m, want in [(0, (4, 2, 7)), (1, (6, 3, 10)), (2, (6, 3, 10)), (3, (12, 4, 17)), (4, (5, 3, 9))]:
         assert L.kgmt_model_dims(m, C.byref(s), C.byref(c), C.byref(d)) == KGMT_OK
         assert (s.value, c.value, d.value) == want
-    assert L.kgmt_model_dims(0, C.byref(s), C.byref(c), C.byref(d)) != KGMT_OK
+    assert L.kgmt_model_dims(5, C.byref(s), C.byref(c), C.byref(d)) != KGMT_OK
 
 
 def test_null_arguments_are_rejected():

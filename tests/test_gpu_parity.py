@@ -15,7 +15,7 @@ import pytest
 from kgmt_amd import _lib
 
 from kgmt_amd.abi import (KGMT_DI_EXACT, KGMT_MODEL_DOUBLE_INTEGRATOR, KGMT_MODEL_DUBINS_5D,
-                          KGMT_MODEL_DUBINS_AIRPLANE, KGMT_MODEL_QUADCOPTER, dims)
+                          KGMT_MODEL_DUBINS_AIRPLANE, KGMT_MODEL_QUADCOPTER, KGMT_MODEL_UNICYCLE, dims)
 from kgmt_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -41,13 +41,14 @@ CASES = {
     "dubins5d": lambda: W.make_query("dubins5d", 3).cfg,
     "quad12d": lambda: W.make_query("quad12d", 3).cfg,
     "quad_reference": lambda: W.config("reference_quad"),
+    "unicycle": lambda: W.make_query("unicycle2d", 3).cfg,
 }
 
 
 def obstacles_for(cfg, seed=5):
     rng = np.random.default_rng(seed)
     span = cfg.ws_max - cfg.ws_min
-    return W.random_boxes(rng, 60, 0.03 * span, 0.15 * span, ws=(cfg.ws_min, cfg.ws_max))
+    return W.random_boxes(rng, 60, 0.03 * span, 0.15 * span, ws=(cfg.ws_min, cfg.ws_max), dim=cfg.wdim)
 
 
 def parent_states(cfg, n, seed=7):
@@ -56,7 +57,10 @@ def parent_states(cfg, n, seed=7):
     lo, hi = cfg.ws_min, cfg.ws_max
     X = np.zeros((S, n), np.float32)
     X[:3] = rng.uniform(lo + 0.05 * (hi - lo), lo + 0.95 * (hi - lo), (3, n))
-    if cfg.model == KGMT_MODEL_DOUBLE_INTEGRATOR:
+    if cfg.model == KGMT_MODEL_UNICYCLE:
+        X[2] = rng.uniform(-math.pi, math.pi, n)
+        X[3] = rng.uniform(-0.6, 0.6, n)
+    elif cfg.model == KGMT_MODEL_DOUBLE_INTEGRATOR:
         X[3:6] = rng.uniform(-0.5, 0.5, (3, n))
     elif cfg.model == KGMT_MODEL_DUBINS_AIRPLANE:
         X[3] = rng.uniform(-math.pi, math.pi, n)

@@ -40,6 +40,8 @@ PLAN_CASES = [
     ("dubins5d", 0, dict(capacity=1 << 15, max_iter=40)),
     ("dubins5d", 2, dict(capacity=5000, max_iter=30)),
     ("quad12d", 0, dict(capacity=1 << 13, max_iter=12)),
+    ("unicycle2d", 0, dict()),
+    ("unicycle2d", 4, dict(capacity=20000)),
 ]
 
 
@@ -189,3 +191,32 @@ def test_dubins_airplane_plan_matches_oracle(orc):
     res = g.plan(start, goal, base.obstacles, len(base.obstacles))
     _same_plan(g, res, ref)
     assert ref["stats"]["tree_size"] > 100
+
+
+def test_unicycle_batch_and_device_obstacles(orc):
+    """Planar unicycle queries batched (rows of 4 floats per box), with and
+    without the graph, and a plain plan from device-resident planar rows."""
+    import torch
+    qs = [W.make_query("unicycle2d", s, capacity=1 << 14, query_id=s) for s in range(6)]
+    cfgs, starts, goals, obs, offs, cnts = W.pack_batch(qs)
+    assert obs.shape[1] == 4
+    b = BatchKGMT(cfgs)
+    refs = [orc.plan(q.cfg, q.start, q.goal, q.obstacles) for q in qs]
+    for use_graph in (False, True):
+        results, _ = b.plan(starts, goals, obs, offs, cnts, use_graph=use_graph)
+        for i, ref in enumerate(refs):
+            assert (results[i].iterations, results[i].tree_size) == (ref["stats"]["iterations"],
+                                                                     ref["stats"]["tree_size"]), i
+            s, p, _ = b.tree(i)
+            assert np.array_equal(s.view(np.uint32), ref["samples"].view(np.uint32)), i
+    q = qs[2]
+    g = KGMT(q.cfg)
+    d = torch.from_numpy(q.obstacles.reshape(-1)).cuda()
+    res = g.plan(q.start, q.goal, int(d.data_ptr()), len(q.obstacles))
+    _same_plan(g, res, refs[2])
+
+
+def test_batch_rejects_mixed_workspace_dims():
+    cfgs = [W.config("unicycle2d", capacity=1 << 12), W.config("di6d", capacity=1 << 12)]
+    with pytest.raises(KgmtError):
+        BatchKGMT(cfgs)

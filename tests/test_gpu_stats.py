@@ -1,7 +1,7 @@
 """Planning outcomes over many seeds on the GPU against the oracle's records
 (tests/golden/plan_stats_<workload>.json, written by make_plan_stats.py):
-50 seeds of the 6D double integrator, 200 of the 5D Dubins airplane and 200
-of the 12D quadcopter at their full benchmark sizes.
+50 seeds of the 6D double integrator, 200 of the 5D Dubins airplane, 200 of
+the 12D quadcopter and 200 of the planar unicycle at their full sizes.
 
 Every seed must match exactly (iterations, tree size, goal node, path rows,
 path cost and length bits, expanded children); on top of that the two
@@ -68,7 +68,7 @@ def _compare(name, got, ref):
     assert p > 0.05
 
 
-@pytest.mark.parametrize("name", ["di6d", "dubins5d", "quad12d"])
+@pytest.mark.parametrize("name", ["di6d", "dubins5d", "quad12d", "unicycle2d"])
 def test_plain_planner_over_seeds(name):
     ref = _load(name)
     n = ref["seeds"]
@@ -84,7 +84,7 @@ def test_plain_planner_over_seeds(name):
     _compare(name, _records(results), ref)
 
 
-@pytest.mark.parametrize("name", ["di6d", "dubins5d"])
+@pytest.mark.parametrize("name", ["di6d", "dubins5d", "unicycle2d"])
 def test_batched_planner_over_seeds(name):
     ref = _load(name)
     n = ref["seeds"]

@@ -25,11 +25,12 @@ def _check_tree(q, out):
     # controls inside their draw ranges
     for i in range(Cd):
         assert np.all(s[1:, S + i] >= cfg.ctrl_min[i]) and np.all(s[1:, S + i] <= cfg.ctrl_max[i])
-    # cost = distance of the node position to the goal
-    d = np.sqrt(((s[1:, :3].astype(np.float64) - q.goal[:3]) ** 2).sum(1))
+    # cost = distance of the node position (W coordinates) to the goal
+    w = cfg.wdim
+    d = np.sqrt(((s[1:, :w].astype(np.float64) - q.goal[:w]) ** 2).sum(1))
     assert np.allclose(c[1:], d, rtol=1e-6, atol=1e-7)
     # appended nodes are valid edges: inside the workspace
-    assert np.all(s[1:, :3] >= cfg.ws_min) and np.all(s[1:, :3] <= cfg.ws_max)
+    assert np.all(s[1:, :w] >= cfg.ws_min) and np.all(s[1:, :w] <= cfg.ws_max)
     if st["goal_index"] >= 0:
         g = st["goal_index"]
         assert c[g] < cfg.goal_thresh
@@ -55,7 +56,8 @@ def _chain(p, g):
 
 @pytest.mark.parametrize("name,seed,kw", [("di6d", 0, dict(capacity=1 << 15)),
                                           ("dubins5d", 1, dict(capacity=1 << 14, max_iter=40)),
-                                          ("quad12d", 0, dict(capacity=1 << 12, max_iter=8))])
+                                          ("quad12d", 0, dict(capacity=1 << 12, max_iter=8)),
+                                          ("unicycle2d", 2, dict())])
 def test_tree_invariants(name, seed, kw, orc):
     q = W.make_query(name, seed, **kw)
     out = orc.plan(q.cfg, q.start, q.goal, q.obstacles, log_iters=True)
