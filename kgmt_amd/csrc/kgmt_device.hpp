@@ -337,18 +337,52 @@ struct DryRun  // no boxes; records the box spanned by every checked segment
     }
 };
 
-struct CandCheck  // a wave's candidate boxes, (lo, hi) float4 pairs in LDS
+// separated() for a segment box with finite faces and a box whose faces are
+// finite, -inf low or +inf high, or NaN: on each axis the two face gaps
+// lo - max and min - hi are >= 0 exactly when the comparisons of separated()
+// hold (denormals are kept, so a difference is 0 only for equal operands),
+// a NaN gap counts as not separating, and fmaxf keeps the non-NaN operand.
+__device__ __forceinline__ bool separated_gap(float mnx, float mny, float mnz, float mxx, float mxy, float mxz,
+                                              float4 lo, float4 hi)
 {
+    const float gx = fmaxf(lo.x - mxx, mnx - hi.x);
+    const float gy = fmaxf(lo.y - mxy, mny - hi.y);
+    const float gz = fmaxf(lo.z - mxz, mnz - hi.z);
+    return fmaxf(fmaxf(gx, gy), gz) >= 0.0f;
+}
+
+// A wave's candidate boxes, (lo, hi) float4 pairs in LDS. Positions tested
+// here are finite (the dry run or the Dubins bound was finite), and a
+// candidate overlaps a finite box, so its low faces are < +inf and its high
+// faces > -inf: separated_gap gives separated()'s verdict.
+struct CandCheck
+{
+    static constexpr int R = 4;  // the first R boxes are held in registers for the whole edge
     const float4* c;
     int n;
+    float4 r[2 * R];
+    __device__ __forceinline__ CandCheck(const float4* list, int count) : c(list), n(count)
+    {
+#pragma unroll
+        for(int k = 0; k < R; ++k)
+            {
+                r[2 * k]     = k < n ? c[2 * k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                r[2 * k + 1] = k < n ? c[2 * k + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+    }
     __device__ __forceinline__ bool operator()(float ax, float ay, float az, float bx, float by, float bz) const
     {
         const float mnx = ax > bx ? bx : ax, mxx = ax > bx ? ax : bx;
         const float mny = ay > by ? by : ay, mxy = ay > by ? ay : by;
         const float mnz = az > bz ? bz : az, mxz = az > bz ? az : bz;
         bool hit = false;
-#pragma unroll 4
-        for(int m = 0; m < n; ++m) hit |= !separated(mnx, mny, mnz, mxx, mxy, mxz, c[2 * m], c[2 * m + 1]);
+#pragma unroll
+        for(int k = 0; k < R; ++k)
+            {
+                if(k >= n) break;  // wave-uniform
+                hit |= !separated_gap(mnx, mny, mnz, mxx, mxy, mxz, r[2 * k], r[2 * k + 1]);
+            }
+        for(int m = R; m < n; ++m) hit |= !separated_gap(mnx, mny, mnz, mxx, mxy, mxz, c[2 * m], c[2 * m + 1]);
         return !hit;
     }
 };
@@ -424,12 +458,9 @@ __device__ __forceinline__ bool prop_unicycle(const QueryDesc& qd, float* s, con
             y     = y + (v * st) * h;
             theta = theta + ((v / c.uni_length) * tn) * h;
             v     = v + a * h;
-            if(x < c.ws_min || x > c.ws_max || y < c.ws_min || y > c.ws_max)
-                {
-                    valid = false;
-                    break;
-                }
-            if(!chk(px, py, 0.0f, x, y, 0.0f))
+            // workspace then segment, both ending the edge: one branch
+            const bool free = chk(px, py, 0.0f, x, y, 0.0f);
+            if(x < c.ws_min || x > c.ws_max || y < c.ws_min || y > c.ws_max || !free)
                 {
                     valid = false;
                     break;
@@ -470,17 +501,10 @@ __device__ __forceinline__ bool prop_di(const QueryDesc& qd, float* s, const flo
                     vy = vy + (ay + 2.0f * ay + 2.0f * ay + ay) * h / 6.0f;
                     vz = vz + (az + 2.0f * az + 2.0f * az + az) * h / 6.0f;
                 }
-            if(vx < c.v_min || vx > c.v_max || vy < c.v_min || vy > c.v_max || vz < c.v_min || vz > c.v_max)
-                {
-                    valid = false;
-                    break;
-                }
-            if(ws_out(c, x, y, z))
-                {
-                    valid = false;
-                    break;
-                }
-            if(!chk(px, py, pz, x, y, z))
+            // velocity, workspace, segment: each ends the edge, so one branch
+            const bool free = chk(px, py, pz, x, y, z);
+            if(vx < c.v_min || vx > c.v_max || vy < c.v_min || vy > c.v_max || vz < c.v_min || vz > c.v_max ||
+               ws_out(c, x, y, z) || !free)
                 {
                     valid = false;
                     break;
@@ -491,32 +515,51 @@ __device__ __forceinline__ bool prop_di(const QueryDesc& qd, float* s, const flo
 }
 
 // statePropagator.cu:147-241; FIVE_D: speed is control u[2], no acceleration.
+// Within an edge yaw and pitch advance by the constant h*yr and h*pr per step,
+// so the sin/cos of a step's mid (+hh*rate) and end (+h*rate) angles are its
+// start values rotated by those increments (angle addition); the end values
+// start the next step. sincos_k is evaluated once per angle and increment per
+// edge instead of four times per step (the reference calls sin/cos of every
+// stage angle).
+struct Rot  // sin/cos of an angle increment
+{
+    float s, c;
+};
+__device__ __forceinline__ void rotate(float s0, float c0, Rot r, float& s, float& c)
+{
+    s = s0 * r.c + c0 * r.s;
+    c = c0 * r.c - s0 * r.s;
+}
+
 template <bool FIVE_D, class CHK>
 __device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const float* u, int steps, CHK& chk)
 {
     const kgmt_config& c = qd.cfg;
     const float h        = c.dt;
+    const float hh       = 0.5f * h;
     const float yr = u[0], pr = u[1];
     const float a = FIVE_D ? 0.0f : u[2];
     float x = s[0], y = s[1], z = s[2], yaw = s[3], pitch = s[4];
     float v    = FIVE_D ? u[2] : s[5];
     bool valid = true;
-    // sin/cos of the step-start angles; the step-end angles pitch + h*pr and
-    // yaw + h*yr are the next step's start angles (same expression, same bits)
     float sp0, cp0, sy0, cy0;
     sincos_k(pitch, sp0, cp0);
     sincos_k(yaw, sy0, cy0);
+    Rot pH, pE, yH, yE;
+    sincos_k(hh * pr, pH.s, pH.c);
+    sincos_k(h * pr, pE.s, pE.c);
+    sincos_k(hh * yr, yH.s, yH.c);
+    sincos_k(h * yr, yE.s, yE.c);
     for(int i = 0; i < steps; ++i)
         {
             const float px = x, py = y, pz = z;
-            const float hh = 0.5f * h;
-            const float vm = v + hh * a, pm = pitch + hh * pr, ym = yaw + hh * yr;
-            const float ve = v + h * a, pe = pitch + h * pr, ye = yaw + h * yr;
+            const float vm = v + hh * a;
+            const float ve = v + h * a;
             float spm, cpm, sym, cym, spe, cpe, sye, cye;
-            sincos_k(pm, spm, cpm);
-            sincos_k(ym, sym, cym);
-            sincos_k(pe, spe, cpe);
-            sincos_k(ye, sye, cye);
+            rotate(sp0, cp0, pH, spm, cpm);
+            rotate(sy0, cy0, yH, sym, cym);
+            rotate(sp0, cp0, pE, spe, cpe);
+            rotate(sy0, cy0, yE, sye, cye);
             const float mx = vm * cpm * cym;
             const float my = vm * cpm * sym;
             const float mz = vm * spm;
@@ -527,22 +570,12 @@ __device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const
             pitch          = pitch + h * pr;
             v              = v + (h / 6.0f) * (a + 2.0f * (a + a) + a);
             sp0 = spe, cp0 = cpe, sy0 = sye, cy0 = cye;
-            if(!FIVE_D && (v < c.v_min || v > c.v_max))
-                {
-                    valid = false;
-                    break;
-                }
-            if(pitch < c.pitch_min || pitch > c.pitch_max)
-                {
-                    valid = false;
-                    break;
-                }
-            if(ws_out(c, x, y, z))
-                {
-                    valid = false;
-                    break;
-                }
-            if(!chk(px, py, pz, x, y, z))
+            // the reference's checks in its order (speed, pitch, workspace,
+            // segment) all end the edge as invalid at this step, so they are
+            // combined; the segment test runs for every lane (one branch)
+            const bool free = chk(px, py, pz, x, y, z);
+            if((!FIVE_D && (v < c.v_min || v > c.v_max)) || pitch < c.pitch_min || pitch > c.pitch_max ||
+               ws_out(c, x, y, z) || !free)
                 {
                     valid = false;
                     break;
@@ -604,18 +637,10 @@ __device__ __forceinline__ bool prop_quad(const QueryDesc& qd, float* s, const f
             quad_ode(qd, st, uc, k4);
 #pragma unroll
             for(int j = 0; j < 12; ++j) s[j] = s[j] + h6 * (k1[j] + 2.0f * k2[j] + 2.0f * k3[j] + k4[j]);
+            // body velocity, workspace, segment: each ends the edge, so one branch
+            const bool free = chk(px, py, pz, s[0], s[1], s[2]);
             if(s[6] < c.v_min || s[6] > c.v_max || s[7] < c.v_min || s[7] > c.v_max || s[8] < c.v_min ||
-               s[8] > c.v_max)
-                {
-                    valid = false;
-                    break;
-                }
-            if(ws_out(c, s[0], s[1], s[2]))
-                {
-                    valid = false;
-                    break;
-                }
-            if(!chk(px, py, pz, s[0], s[1], s[2]))
+               s[8] > c.v_max || ws_out(c, s[0], s[1], s[2]) || !free)
                 {
                     valid = false;
                     break;
@@ -725,7 +750,9 @@ __device__ __forceinline__ void dubins_edge_box(const QueryDesc& qd, const float
 #pragma unroll
     for(int d = 0; d < 3; ++d)
         {
-            const float pad = 1e-4f * (1.0f + fabsf(s[d]) + T * vmax);
+            // rounding of the steps, and of the rotated sin/cos (a few ulps
+            // per step)
+            const float pad = 1e-4f * (1.0f + fabsf(s[d]) + T * vmax) + 1e-6f * (float)steps * T * vmax;
             lo[d]           = s[d] + fminf(0.0f, T * dl[d]) - pad;
             hi[d]           = s[d] + fmaxf(0.0f, T * dh[d]) + pad;
         }
@@ -758,21 +785,41 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
     int n = 0;
     if(!__any(!dry.finite))
         {
+            // groups and boxes are read kGroupBatch (4) at a time so that their
+            // LDS loads overlap; candidates keep group-then-box order
             const float4* box = sc.grp + 2 * sc.nGrp;
-            for(int g = 0; g < sc.nGrp && n <= kCand; ++g)
+            for(int g0 = 0; g0 < sc.nGrp && n <= kCand; g0 += (int)kGroupBatch)
                 {
-                    const bool ng = !separated(dry.lo[0], dry.lo[1], dry.lo[2], dry.hi[0], dry.hi[1], dry.hi[2],
-                                               sc.grp[2 * g], sc.grp[2 * g + 1]);
-                    if(!__any(ng)) continue;
-                    for(int m = 0; m < (int)kGroup; ++m)
+                    float4 gl[kGroupBatch], gh[kGroupBatch];
+#pragma unroll
+                    for(int k = 0; k < (int)kGroupBatch; ++k) gl[k] = sc.grp[2 * (g0 + k)], gh[k] = sc.grp[2 * (g0 + k) + 1];
+                    bool ng[kGroupBatch];
+#pragma unroll
+                    for(int k = 0; k < (int)kGroupBatch; ++k)
+                        ng[k] = !separated(dry.lo[0], dry.lo[1], dry.lo[2], dry.hi[0], dry.hi[1], dry.hi[2], gl[k], gh[k]);
+#pragma unroll
+                    for(int k = 0; k < (int)kGroupBatch; ++k)
                         {
-                            const float4 lo = box[2 * (kGroup * g + m)], hi = box[2 * (kGroup * g + m) + 1];
-                            const bool c = ng && !separated(dry.lo[0], dry.lo[1], dry.lo[2], dry.hi[0], dry.hi[1],
-                                                            dry.hi[2], lo, hi);
-                            if(__any(c))
+                            if(!__any(ng[k]) || n > kCand) continue;
+                            const float4* b = box + 2 * (int)kGroup * (g0 + k);
+#pragma unroll
+                            for(int h = 0; h < (int)kGroup; h += 4)
                                 {
-                                    if(n < kCand) cand[2 * n] = lo, cand[2 * n + 1] = hi;  // same value in every lane
-                                    ++n;
+                                    float4 bl[4], bh[4];
+#pragma unroll
+                                    for(int m = 0; m < 4; ++m) bl[m] = b[2 * (h + m)], bh[m] = b[2 * (h + m) + 1];
+#pragma unroll
+                                    for(int m = 0; m < 4; ++m)
+                                        {
+                                            const bool c = ng[k] && !separated(dry.lo[0], dry.lo[1], dry.lo[2], dry.hi[0],
+                                                                               dry.hi[1], dry.hi[2], bl[m], bh[m]);
+                                            if(__any(c))
+                                                {
+                                                    // same value in every lane
+                                                    if(n < kCand) cand[2 * n] = bl[m], cand[2 * n + 1] = bh[m];
+                                                    ++n;
+                                                }
+                                        }
                                 }
                         }
                 }
@@ -794,7 +841,7 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    CandCheck chk{cand, n};
+    CandCheck chk(cand, n);
     return propagate_with<MODEL>(qd, s, u, steps, chk);
 }
 

@@ -37,14 +37,18 @@ __device__ __forceinline__ void stage_scene(const float4* __restrict__ pool, uin
 }
 
 // total[key]++ for lanes with `has`, valid[key]++ for lanes with `has && ok`:
-// one atomic per distinct key in the wave (leader loop over the ballot).
+// one atomic per distinct key held by several lanes (leader loop over the
+// ballot; children of one parent share regions, and atomics of every lane on
+// a shared counter serialise in the L2). The leader rounds stop at the first key that only one lane holds: the
+// remaining lanes then add their own counts (their keys are spread, so
+// contention is low). Integer sums: the result is the same either way.
 __device__ __forceinline__ void wave_count(int32_t* total, int32_t* validc, int key, bool has, bool ok)
 {
     uint64_t pending = __ballot(has);
     while(pending)
         {
             const int leader    = __ffsll((unsigned long long)pending) - 1;
-            const int k         = __shfl(key, leader, 64);
+            const int k         = __builtin_amdgcn_readlane(key, leader);
             const uint64_t same = __ballot(has && key == k);
             const uint64_t good = __ballot(has && ok && key == k);
             if((int)lane_id() == leader)
@@ -53,7 +57,54 @@ __device__ __forceinline__ void wave_count(int32_t* total, int32_t* validc, int 
                     if(good) atomicAdd(&validc[k], (int32_t)__popcll(good));
                 }
             pending &= ~same;
+            if(__popcll(same) < 2) break;
         }
+    if((pending >> lane_id()) & 1ull)
+        {
+            atomicAdd(&total[key], 1);
+            if(ok) atomicAdd(&validc[key], 1);
+        }
+}
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_or(uint32_t v)
+{
+    return v | (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+
+// OR of v over the 64 lanes (all active), as a wave-uniform value: DPP within
+// each row of 16 lanes, then the four rows' lane 0 read into scalars.
+__device__ __forceinline__ uint32_t wave_or(uint32_t v)
+{
+    v = dpp_or<0xB1>(v);   // quad_perm [1,0,3,2]
+    v = dpp_or<0x4E>(v);   // quad_perm [2,3,0,1]
+    v = dpp_or<0x141>(v);  // row_half_mirror
+    v = dpp_or<0x140>(v);  // row_mirror
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 0) | (uint32_t)__builtin_amdgcn_readlane((int)v, 16) |
+           (uint32_t)__builtin_amdgcn_readlane((int)v, 32) | (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+}
+
+// Sets bit `bit` of `words` for every lane with `want`: one atomicOr per
+// distinct word per wave, with the OR of that word's lane masks (a wave's
+// children come from few parents and mostly fall in the same few words; one
+// atomic per lane on a shared word serialises in the L2). Call with every
+// lane of the wave.
+__device__ __forceinline__ void wave_set_bits(uint32_t* words, uint32_t bit, bool want)
+{
+    const uint32_t w = bit >> 5, m = 1u << (bit & 31u);
+    uint64_t pending = __ballot(want);
+    while(pending)
+        {
+            const int leader    = __ffsll((unsigned long long)pending) - 1;
+            const uint32_t k    = (uint32_t)__builtin_amdgcn_readlane((int)w, leader);
+            const bool mine     = want && w == k;
+            const uint64_t same = __ballot(mine);
+            const uint32_t v    = wave_or(mine ? m : 0u);
+            if((int)lane_id() == leader) atomicOr(&words[k], v);
+            pending &= ~same;
+            if(__popcll(same) < 2) break;  // spread words: one atomic per lane below
+        }
+    if((pending >> lane_id()) & 1ull) atomicOr(&words[w], m);
 }
 
 // Region scores of one query (block of NT threads). a: LDS of at least
@@ -255,7 +306,28 @@ struct TileLds
     float ufront[NT];
     uint32_t sbuf[NT / 64 + 1];
     float4 cand[NT / 64][2 * kCand];  // per-wave candidate boxes (two-phase check)
+    // per-tile sums of the region counters and ORs of the subregion bitset
+    // words, flushed with one global atomic per distinct key (open addressing,
+    // key -1 = empty; at most NT distinct keys per tile, so a probe always ends)
+    int32_t rkey[NT];
+    uint32_t rtot[NT], rval[NT];
+    int32_t wkey[NT];
+    uint32_t wmask[NT];
 };
+
+// slot of `key` in an NT-entry open-addressing table of LDS keys (claimed by
+// compare-and-swap); the table holds fewer than NT other keys
+template <int NT>
+__device__ __forceinline__ uint32_t tab_slot(int32_t* keys, int32_t key)
+{
+    uint32_t slot = ((uint32_t)key * 2654435761u) & (uint32_t)(NT - 1);
+    for(;;)
+        {
+            const int32_t old = atomicCAS(&keys[slot], -1, key);
+            if(old == -1 || old == key) return slot;
+            slot = (slot + 1) & (uint32_t)(NT - 1);
+        }
+}
 
 // One tile of TC = NT / K children u = base .. base + TC - 1 of a query (TC
 // divides kChunk, so a tile never straddles two chunks), K lanes per child.
@@ -289,7 +361,9 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
                 const int32_t k = steps - qd.cfg.steps_min;
                 bucket          = (uint32_t)(NT - 2) - (uint32_t)(k < (int32_t)NT - 2 ? k : (int32_t)NT - 2);
             }
-        L.hist[t] = 0;
+        L.hist[t]  = 0;
+        L.rkey[t]  = -1, L.rtot[t] = 0, L.rval[t] = 0;
+        L.wkey[t]  = -1, L.wmask[t] = 0;
         __syncthreads();
         uint32_t rank = 0;
         if(t < TC) rank = atomicAdd(&L.hist[bucket], 1u);
@@ -314,6 +388,8 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
     const bool lead     = K == 1 || t % K == 0;
     int r1              = 0;
     bool ok = false, fn = false;
+    bool setBit   = false;  // valid child in a subregion unseen at iteration start
+    uint32_t r2bit = 0;
     if(has)
         {
             const uint32_t per = st.per;
@@ -334,6 +410,18 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
                 ok = propagate<MODEL, K>(qd, s, uc, steps, sc);
             if(lead)
                 {
+                    // region ids and the loads they need first: a load issued
+                    // after the stores would wait for them (one in-order counter)
+                    int ix, iy, iz;
+                    r1 = region_of(qd, s[0], s[1], s[2], ix, iy, iz);
+                    if(ok)
+                        {
+                            const int r2    = subregion_of(qd, s[0], s[1], s[2], r1, ix, iy, iz);
+                            const bool seen = test_bit(P.actPrev + qd.subOff, (uint32_t)r2);
+                            fn              = (uf < P.score[qd.regOff + r1]) || !seen;
+                            setBit          = !seen;
+                            r2bit           = (uint32_t)r2;
+                        }
                     float* U = P.childS + qd.sampleOff;
 #pragma unroll
                     for(int d = 0; d < S; ++d) U[(size_t)d * cap + u] = s[d];
@@ -342,24 +430,37 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
                     U[(size_t)(D - 1) * cap + u] = qd.cfg.dt * (float)steps;
                     P.parentU[nodeO + u]          = (int32_t)p;
                     if(u % per == 0) P.frontier[nodeO + p] = 0;
-                    int ix, iy, iz;
-                    r1                   = region_of(qd, s[0], s[1], s[2], ix, iy, iz);
                     P.r1child[nodeO + u] = r1;
-                    if(ok)
-                        {
-                            const int r2    = subregion_of(qd, s[0], s[1], s[2], r1, ix, iy, iz);
-                            const bool seen = test_bit(P.actPrev + qd.subOff, (uint32_t)r2);
-                            fn              = (uf < P.score[qd.regOff + r1]) || !seen;
-                            if(!seen)
-                                atomicOr(&P.actNext[qd.subOff + ((uint32_t)r2 >> 5)], 1u << ((uint32_t)r2 & 31u));
-                        }
-                    P.fnext[nodeO + u] = fn ? 1 : 0;
+                    P.fnext[nodeO + u]   = fn ? 1 : 0;
                 }
         }
-    wave_count(P.total + qd.regOff, P.validc + qd.regOff, r1, has && lead, ok);
+    // region counters and new subregion bits: summed per tile in LDS, then
+    // one global atomic per distinct region / bitset word
+    if(has && lead)
+        {
+            const uint32_t e = tab_slot<NT>(L.rkey, r1);
+            atomicAdd(&L.rtot[e], 1u);
+            if(ok) atomicAdd(&L.rval[e], 1u);
+        }
+    if(setBit)
+        {
+            const uint32_t e = tab_slot<NT>(L.wkey, (int32_t)(r2bit >> 5));
+            atomicOr(&L.wmask[e], 1u << (r2bit & 31u));
+        }
     // the tile lies inside one 1024-element chunk
     const uint64_t fm = __ballot(fn);
     if(lane_id() == 0 && fm) atomicAdd(&P.partA[qd.chunkOff + base / kChunk], (uint32_t)__popcll(fm));
+    __syncthreads();
+    {
+        const int32_t rk = L.rkey[t];
+        if(rk >= 0)
+            {
+                atomicAdd(&P.total[qd.regOff + rk], (int32_t)L.rtot[t]);
+                if(L.rval[t]) atomicAdd(&P.validc[qd.regOff + rk], (int32_t)L.rval[t]);
+            }
+        const int32_t wk = L.wkey[t];
+        if(wk >= 0) atomicOr(&P.actNext[qd.subOff + (uint32_t)wk], L.wmask[t]);
+    }
     __syncthreads();  // L is reused by the next tile
 }
 
@@ -1009,14 +1110,15 @@ __global__ __launch_bounds__(kBlock) void k_stage_region_update(const QueryDesc*
     if(!has) key = 0;
     const bool ok = has && valid[i] != 0;
     bool fn       = false;
+    bool setBit   = false;
     if(ok)
         {
-            const uint32_t b = (uint32_t)sub;
-            const bool seen  = test_bit(actPrev, b);
-            fn               = (ufront[i] < score[key]) || !seen;
-            if(!seen) atomicOr(&actNext[b >> 5], 1u << (b & 31u));
+            const bool seen = test_bit(actPrev, (uint32_t)sub);
+            fn              = (ufront[i] < score[key]) || !seen;
+            setBit          = !seen;
         }
     if(i < n) fnext[i] = fn ? 1 : 0;
+    wave_set_bits(actNext, has ? (uint32_t)sub : 0u, setBit);
     wave_count(total, validc, key, has, ok);
 }
 

@@ -396,30 +396,48 @@ static int prop_di(const kgmt_config* cfg, const float* x0, const float* ctrl, i
     return valid;
 }
 
+/* sin/cos of angle + increment from sin/cos of the angle (s0, c0) and of the
+ * increment (rs, rc): angle addition, products rounded, then the sum */
+static void rotate(float s0, float c0, float rs, float rc, float* s, float* c)
+{
+    *s = s0 * rc + c0 * rs;
+    *c = c0 * rc - s0 * rs;
+}
+
 /* statePropagator.cu:147-241 (6D: speed v is a state, a its control) and the
- * 5D variant (speed is a control, a = 0, no speed bound). */
+ * 5D variant (speed is a control, a = 0, no speed bound). The reference takes
+ * sin/cos of the start, mid (+h/2 rate) and end (+h rate) angles of every
+ * step; here yaw and pitch advance by constant increments within the edge,
+ * so the mid and end values are the step-start values rotated by sin/cos of
+ * the increments (evaluated once per edge), and the end values start the
+ * next step. */
 static int prop_dubins(const kgmt_config* cfg, const float* x0, const float* ctrl, int32_t steps, const float* obs,
                        uint32_t n_obs, float* x1, int five_d)
 {
     float h  = cfg->dt;
+    float hh = 0.5f * h;
     float yr = ctrl[0], pr = ctrl[1];
     float a  = five_d ? 0.0f : ctrl[2];
     float x = x0[0], y = x0[1], z = x0[2], yaw = x0[3], pitch = x0[4];
     float v   = five_d ? ctrl[2] : x0[5];
     int valid = 1;
+    float sp0, cp0, sy0, cy0, phs, phc, pes, pec, yhs, yhc, yes, yec;
+    orc_sincosf(pitch, &sp0, &cp0);
+    orc_sincosf(yaw, &sy0, &cy0);
+    orc_sincosf(hh * pr, &phs, &phc);
+    orc_sincosf(h * pr, &pes, &pec);
+    orc_sincosf(hh * yr, &yhs, &yhc);
+    orc_sincosf(h * yr, &yes, &yec);
     for(int i = 0; i < steps; ++i)
         {
             float p0[3] = {x, y, z};
-            float hh    = 0.5f * h;
-            float vm = v + hh * a, pm = pitch + hh * pr, ym = yaw + hh * yr;
-            float ve = v + h * a, pe = pitch + h * pr, ye = yaw + h * yr;
-            float sp0, cp0, sy0, cy0, spm, cpm, sym, cym, spe, cpe, sye, cye;
-            orc_sincosf(pitch, &sp0, &cp0);
-            orc_sincosf(yaw, &sy0, &cy0);
-            orc_sincosf(pm, &spm, &cpm);
-            orc_sincosf(ym, &sym, &cym);
-            orc_sincosf(pe, &spe, &cpe);
-            orc_sincosf(ye, &sye, &cye);
+            float vm = v + hh * a;
+            float ve = v + h * a;
+            float spm, cpm, sym, cym, spe, cpe, sye, cye;
+            rotate(sp0, cp0, phs, phc, &spm, &cpm);
+            rotate(sy0, cy0, yhs, yhc, &sym, &cym);
+            rotate(sp0, cp0, pes, pec, &spe, &cpe);
+            rotate(sy0, cy0, yes, yec, &sye, &cye);
             float mx = vm * cpm * cym;
             float my = vm * cpm * sym;
             float mz = vm * spm;
@@ -429,6 +447,7 @@ static int prop_dubins(const kgmt_config* cfg, const float* x0, const float* ctr
             yaw      = yaw + h * yr;
             pitch    = pitch + h * pr;
             v        = v + (h / 6.0f) * (a + 2.0f * (a + a) + a);
+            sp0 = spe, cp0 = cpe, sy0 = sye, cy0 = cye;
             if(!five_d && (v < cfg->v_min || v > cfg->v_max))
                 {
                     valid = 0;
