@@ -758,9 +758,42 @@ __device__ __forceinline__ void dubins_edge_box(const QueryDesc& qd, const float
         }
 }
 
+template <int CTRL>
+__device__ __forceinline__ float dpp_minf(float v)
+{
+    return fminf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false)));
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_maxf(float v)
+{
+    return fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false)));
+}
+
+// float bits of lane `l` (readlane moves 32-bit integers)
+__device__ __forceinline__ float readlane_f(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// min / max of v over the 64 lanes (all active), wave-uniform: DPP within each
+// row of 16 lanes (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror,
+// row_mirror), then the rows' lane 0
+__device__ __forceinline__ float wave_minf(float v)
+{
+    v = dpp_minf<0x140>(dpp_minf<0x141>(dpp_minf<0x4E>(dpp_minf<0xB1>(v))));
+    return fminf(fminf(readlane_f(v, 0), readlane_f(v, 16)), fminf(readlane_f(v, 32), readlane_f(v, 48)));
+}
+__device__ __forceinline__ float wave_maxf(float v)
+{
+    v = dpp_maxf<0x140>(dpp_maxf<0x141>(dpp_maxf<0x4E>(dpp_maxf<0xB1>(v))));
+    return fmaxf(fmaxf(readlane_f(v, 0), readlane_f(v, 16)), fmaxf(readlane_f(v, 32), readlane_f(v, 48)));
+}
+
+// Called by every lane of the wave; lanes without a child (has == false)
+// pass steps = 0 and take part in the wave-wide steps with an empty box.
 template <int MODEL>
 __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* s, const float* u, int steps,
-                                                    const Scene& sc, float4* cand)
+                                                    const Scene& sc, float4* cand, bool has)
 {
     constexpr int S      = ModelDims<MODEL>::S;
     constexpr bool BOUND = MODEL == KGMT_MODEL_DUBINS_5D || MODEL == KGMT_MODEL_DUBINS_AIRPLANE;
@@ -782,44 +815,43 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
             for(int d = 0; d < S; ++d) s1[d] = s[d];
             dryOk = propagate_with<MODEL>(qd, s1, u, steps, dry);
         }
+    if(!has)
+        {
+            // an empty box: no candidate, no effect on the wave's union
+            dry.lo[0] = dry.lo[1] = dry.lo[2] = INFINITY;
+            dry.hi[0] = dry.hi[1] = dry.hi[2] = -INFINITY;
+            dry.finite = true;
+        }
     int n = 0;
     if(!__any(!dry.finite))
         {
-            // groups and boxes are read kGroupBatch (4) at a time so that their
-            // LDS loads overlap; candidates keep group-then-box order
+            // candidates = boxes that some lane's box overlaps, in box order.
+            // Lane j first tests box b0 + j against the union of the lanes'
+            // boxes (a box meeting a lane's box meets the union); the boxes
+            // that pass are then tested against every lane's box.
+            float ulo[3], uhi[3];
+#pragma unroll
+            for(int d = 0; d < 3; ++d) ulo[d] = wave_minf(dry.lo[d]), uhi[d] = wave_maxf(dry.hi[d]);
             const float4* box = sc.grp + 2 * sc.nGrp;
-            for(int g0 = 0; g0 < sc.nGrp && n <= kCand; g0 += (int)kGroupBatch)
+            const int nBox    = sc.nGrp * (int)kGroup;
+            for(int b0 = 0; b0 < nBox && n <= kCand; b0 += 64)
                 {
-                    float4 gl[kGroupBatch], gh[kGroupBatch];
-#pragma unroll
-                    for(int k = 0; k < (int)kGroupBatch; ++k) gl[k] = sc.grp[2 * (g0 + k)], gh[k] = sc.grp[2 * (g0 + k) + 1];
-                    bool ng[kGroupBatch];
-#pragma unroll
-                    for(int k = 0; k < (int)kGroupBatch; ++k)
-                        ng[k] = !separated(dry.lo[0], dry.lo[1], dry.lo[2], dry.hi[0], dry.hi[1], dry.hi[2], gl[k], gh[k]);
-#pragma unroll
-                    for(int k = 0; k < (int)kGroupBatch; ++k)
+                    const int b = b0 + (int)__lane_id();
+                    bool pass   = false;
+                    if(b < nBox)
+                        pass = !separated(ulo[0], ulo[1], ulo[2], uhi[0], uhi[1], uhi[2], box[2 * b], box[2 * b + 1]);
+                    uint64_t m = __ballot(pass);
+                    while(m && n <= kCand)
                         {
-                            if(!__any(ng[k]) || n > kCand) continue;
-                            const float4* b = box + 2 * (int)kGroup * (g0 + k);
-#pragma unroll
-                            for(int h = 0; h < (int)kGroup; h += 4)
+                            const int k = __ffsll((unsigned long long)m) - 1;
+                            m &= m - 1;
+                            const float4 lo = box[2 * (b0 + k)], hi = box[2 * (b0 + k) + 1];
+                            const bool c = !separated(dry.lo[0], dry.lo[1], dry.lo[2], dry.hi[0], dry.hi[1], dry.hi[2],
+                                                      lo, hi);
+                            if(__any(c))
                                 {
-                                    float4 bl[4], bh[4];
-#pragma unroll
-                                    for(int m = 0; m < 4; ++m) bl[m] = b[2 * (h + m)], bh[m] = b[2 * (h + m) + 1];
-#pragma unroll
-                                    for(int m = 0; m < 4; ++m)
-                                        {
-                                            const bool c = ng[k] && !separated(dry.lo[0], dry.lo[1], dry.lo[2], dry.hi[0],
-                                                                               dry.hi[1], dry.hi[2], bl[m], bh[m]);
-                                            if(__any(c))
-                                                {
-                                                    // same value in every lane
-                                                    if(n < kCand) cand[2 * n] = bl[m], cand[2 * n + 1] = bh[m];
-                                                    ++n;
-                                                }
-                                        }
+                                    if(n < kCand) cand[2 * n] = lo, cand[2 * n + 1] = hi;  // same value in every lane
+                                    ++n;
                                 }
                         }
                 }

@@ -107,43 +107,77 @@ __device__ __forceinline__ void wave_set_bits(uint32_t* words, uint32_t bit, boo
     if((pending >> lane_id()) & 1ull) atomicOr(&words[w], m);
 }
 
-// Region scores of one query (block of NT threads). a: LDS of at least
-// pow2(R1) floats. Q6 semantics: closed formula, pairwise sum.
+constexpr int kScoreThreads = 1024;
+constexpr int kMaxRegions   = 8192;
+
+// Region scores of one query (block of NT threads, R1 <= kMaxRegions). Q6
+// semantics: closed formula, then the pairwise sum a[i] += a[i + st] for
+// st = P2/2, ..., 1 over the scores padded with zeros to P2 = pow2(R1). Thread
+// t owns regions t + k NT: its inputs are loaded together, and the levels with
+// st >= NT add a thread's own registers (i and i + st differ by a multiple of
+// NT), the rest go through LDS (a: at least NT floats).
 template <int NT>
 __device__ void score_regions(const QueryDesc& qd, const int32_t* __restrict__ total,
                               const int32_t* __restrict__ validc, const uint32_t* __restrict__ act,
                               float* __restrict__ score, float* a)
 {
-    const int R1    = qd.R1;
+    constexpr int KR = kMaxRegions / NT;
+    const int R1     = qd.R1;
     const uint32_t P = (uint32_t)qd.P;
-    const float eps = qd.cfg.epsilon;
-    int P2          = 1;
+    const float eps  = qd.cfg.epsilon;
+    int P2           = 1;
     while(P2 < R1) P2 <<= 1;
-    for(int r = threadIdx.x; r < P2; r += NT)
+    const int t = (int)threadIdx.x;
+    int32_t vc[KR], tc[KR];
+    uint32_t cnt[KR];
+#pragma unroll
+    for(int k = 0; k < KR; ++k)
+        {
+            const int r = t + k * NT;
+            vc[k]       = r < R1 ? validc[r] : 0;
+            tc[k]       = r < R1 ? total[r] : 0;
+            cnt[k]      = r < R1 ? count_bits(act, (uint32_t)r * P, P) : 0u;
+        }
+    float s0[KR], sv[KR];
+#pragma unroll
+    for(int k = 0; k < KR; ++k)
         {
             float s = 0.0f;
-            if(r < R1 && validc[r] > 0)
+            if(vc[k] > 0)
                 {
-                    const uint32_t cnt = count_bits(act, (uint32_t)r * P, P);
-                    const float cov    = (float)cnt / (float)P;
-                    const float vf     = (float)validc[r];
-                    const float fv     = (eps + vf) / ((eps + vf) + (float)(total[r] - validc[r])) * qd.vol;
-                    const float f2     = fv * fv;
-                    const float f4     = f2 * f2;
-                    const float t      = (float)total[r];
-                    s                  = f4 / ((1.0f + cov) * (1.0f + t * t));
+                    const float cov = (float)cnt[k] / (float)P;
+                    const float vf  = (float)vc[k];
+                    const float fv  = (eps + vf) / ((eps + vf) + (float)(tc[k] - vc[k])) * qd.vol;
+                    const float f2  = fv * fv;
+                    const float f4  = f2 * f2;
+                    const float tt  = (float)tc[k];
+                    s               = f4 / ((1.0f + cov) * (1.0f + tt * tt));
                 }
-            a[r] = s;
-            if(r < R1) score[r] = s;
+            s0[k] = sv[k] = s;
         }
-    __syncthreads();
-    for(int st = P2 / 2; st >= 1; st >>= 1)
+    int st = P2 / 2;
+    for(; st >= NT; st >>= 1)
         {
-            for(int i = threadIdx.x; i < st; i += NT) a[i] = a[i] + a[i + st];
+            const int m = st / NT;
+#pragma unroll
+            for(int k = 0; k < KR; ++k)
+                if(k < m && k + m < KR) sv[k] = sv[k] + sv[k + m];
+        }
+    // a[i] for i < min(P2, NT): the thread's own partial sum (zero past P2)
+    a[t] = t < P2 ? sv[0] : 0.0f;
+    __syncthreads();
+    for(; st >= 1; st >>= 1)
+        {
+            if(t < st) a[t] = a[t] + a[t + st];
             __syncthreads();
         }
     const float sum = a[0];
-    for(int r = threadIdx.x; r < R1; r += NT) score[r] = validc[r] == 0 ? 1.0f : eps + score[r] / sum;
+#pragma unroll
+    for(int k = 0; k < KR; ++k)
+        {
+            const int r = t + k * NT;
+            if(r < R1) score[r] = vc[k] == 0 ? 1.0f : eps + s0[k] / sum;
+        }
     __syncthreads();
 }
 
@@ -354,12 +388,17 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
         float uc[C];
         int32_t steps = 0;
         float uf      = 0.0f;
-        uint32_t bucket = NT - 1;  // past the end: last
+        // K == 1: each wave sorts its own 64 children (they come from one or
+        // two parents, so the wave's edges stay close together and share few
+        // candidate boxes); K > 1: the whole tile is sorted
+        constexpr uint32_t BW = K == 1 ? 64u : (uint32_t)NT;  // buckets per sorted group
+        const uint32_t g0     = K == 1 ? (t / 64u) * 64u : 0u;
+        uint32_t bucket       = g0 + BW - 1;                  // past the end: last
         if(t < TC && un < nC)
             {
                 draw_child<C>(qd.cfg, qd.cfg.query_id, st.itr, un, uc, steps, uf);
                 const int32_t k = steps - qd.cfg.steps_min;
-                bucket          = (uint32_t)(NT - 2) - (uint32_t)(k < (int32_t)NT - 2 ? k : (int32_t)NT - 2);
+                bucket = g0 + (BW - 2) - (uint32_t)(k < (int32_t)BW - 2 ? k : (int32_t)BW - 2);
             }
         L.hist[t]  = 0;
         L.rkey[t]  = -1, L.rtot[t] = 0, L.rval[t] = 0;
@@ -390,24 +429,33 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
     bool ok = false, fn = false;
     bool setBit   = false;  // valid child in a subregion unseen at iteration start
     uint32_t r2bit = 0;
+    const uint32_t per = st.per;
+    uint32_t p         = 0;
+    float s[S], uc[C];
+    int32_t steps = 0;
+    float uf      = 0.0f;
+#pragma unroll
+    for(int d = 0; d < S; ++d) s[d] = 0.0f;
+#pragma unroll
+    for(int k = 0; k < C; ++k) uc[k] = 0.0f;
     if(has)
         {
-            const uint32_t per = st.per;
-            const uint32_t p   = P.repIdx[nodeO + u / per];
-            const float* T     = P.treeS + qd.sampleOff;
-            float s[S];
+            p              = P.repIdx[nodeO + u / per];
+            const float* T = P.treeS + qd.sampleOff;
 #pragma unroll
             for(int d = 0; d < S; ++d) s[d] = T[(size_t)d * cap + p];
-            float uc[C];
 #pragma unroll
             for(int k = 0; k < C; ++k) uc[k] = L.ctrl[k][i];
-            const int32_t steps = L.steps[i];
-            const float uf      = L.ufront[i];
-            const Scene sc{sObs, (int)qd.nGrp, (int)(t % K)};
-            if constexpr(K == 1 && MODEL != KGMT_MODEL_QUADCOPTER)
-                ok = propagate_two_phase<MODEL>(qd, s, uc, steps, sc, L.cand[t / 64]);
-            else
-                ok = propagate<MODEL, K>(qd, s, uc, steps, sc);
+            steps = L.steps[i];
+            uf    = L.ufront[i];
+        }
+    const Scene sc{sObs, (int)qd.nGrp, (int)(t % K)};
+    if constexpr(K == 1 && MODEL != KGMT_MODEL_QUADCOPTER)
+        ok = propagate_two_phase<MODEL>(qd, s, uc, steps, sc, L.cand[t / 64], has);  // every lane
+    else if(has)
+        ok = propagate<MODEL, K>(qd, s, uc, steps, sc);
+    if(has)
+        {
             if(lead)
                 {
                     // region ids and the loads they need first: a load issued
@@ -515,8 +563,6 @@ __global__ __launch_bounds__(NT, 4) void k_propagate(const QueryDesc* __restrict
 // ---------------------------------------------------------------------------
 // score: one workgroup (1024 threads) per query
 // ---------------------------------------------------------------------------
-constexpr int kScoreThreads = 1024;
-constexpr int kMaxRegions   = 8192;
 
 __global__ __launch_bounds__(kScoreThreads) void k_score(const QueryDesc* __restrict__ qds,
                                                          QueryState* __restrict__ sts, Pools P)
@@ -562,12 +608,21 @@ __device__ void append_children(const QueryDesc& qd, QueryState& st, uint32_t c,
     const int D            = qd.D;
     const uint32_t u0      = c * kChunk + kPer * threadIdx.x;
     uint32_t fl            = 0;
+    int32_t r1[kPer]       = {0, 0, 0, 0};
+    int32_t vcount[kPer]   = {0, 0, 0, 0};
     if(u0 < nC)
         {
+            // flags and regions together, then the regions' valid counts (for
+            // the repeat count) before the rank scan
             const uint32_t w = *reinterpret_cast<const uint32_t*>(P.fnext + nodeO + u0);
+            const int4 rr    = *reinterpret_cast<const int4*>(P.r1child + nodeO + u0);
+            r1[0] = rr.x, r1[1] = rr.y, r1[2] = rr.z, r1[3] = rr.w;
 #pragma unroll
             for(uint32_t k = 0; k < kPer; ++k)
                 if(u0 + k < nC && ((w >> (8 * k)) & 0xffu)) fl |= 1u << k;
+#pragma unroll
+            for(uint32_t k = 0; k < kPer; ++k)
+                if((fl >> k) & 1u) vcount[k] = P.validc[qd.regOff + r1[k]];
         }
     uint32_t tot;
     uint32_t rank = P.partA[qd.chunkOff + c] + block_excl_scan<kBlock>((uint32_t)__popc(fl), sbuf, tot);
@@ -589,7 +644,7 @@ __device__ void append_children(const QueryDesc& qd, QueryState& st, uint32_t c,
         {
             // all loads of the (at most four) children first, then the stores
             float v[kPer][KGMT_MAX_SAMPLE];
-            int32_t par[kPer], r1[kPer];
+            int32_t par[kPer];
 #pragma unroll
             for(uint32_t k = 0; k < kPer; ++k)
                 if((app >> k) & 1u)
@@ -599,7 +654,6 @@ __device__ void append_children(const QueryDesc& qd, QueryState& st, uint32_t c,
                         for(int d = 0; d < KGMT_MAX_SAMPLE; ++d)
                             if(d < D) v[k][d] = U[(size_t)d * cap + u];
                         par[k] = P.parentU[nodeO + u];
-                        r1[k]  = P.r1child[nodeO + u];
                     }
             uint32_t t = t0;
 #pragma unroll
@@ -614,8 +668,7 @@ __device__ void append_children(const QueryDesc& qd, QueryState& st, uint32_t c,
                         P.cost[nodeO + t]     = dg;
                         P.frontier[nodeO + t] = 1;
                         P.r1node[nodeO + t]   = r1[k];
-                        rep[k] = P.validc[qd.regOff + r1[k]] < qd.cfg.sparse_valid_below ? qd.cfg.repeat_sparse
-                                                                                         : qd.cfg.repeat_dense;
+                        rep[k] = vcount[k] < qd.cfg.sparse_valid_below ? qd.cfg.repeat_sparse : qd.cfg.repeat_dense;
                         if(dg < qd.cfg.goal_thresh) atomicMin(&st.goalIdx, (int32_t)t);
                         ++t;
                     }
@@ -733,6 +786,18 @@ __device__ __forceinline__ uint32_t block_sum(const uint32_t* a, uint32_t n, uin
     return tot;
 }
 
+// sum of a[0..na) + b[0..nb) over the workgroup (one scan)
+__device__ __forceinline__ uint32_t block_sum2(const uint32_t* a, uint32_t na, const uint32_t* b, uint32_t nb,
+                                               uint32_t* sbuf)
+{
+    uint32_t v = 0;
+    for(uint32_t i = threadIdx.x; i < na; i += kBlock) v += a[i];
+    for(uint32_t i = threadIdx.x; i < nb; i += kBlock) v += b[i];
+    uint32_t tot;
+    block_excl_scan<kBlock>(v, sbuf, tot);
+    return tot;
+}
+
 // ---------------------------------------------------------------------------
 // expand: next expansion list (re-activated ascending, then new nodes each
 // repeated repU times), truncated at capacity entries. Each job sums the
@@ -764,7 +829,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__
             const uint32_t* pC    = P.partC + qd.chunkOff;
             if(local == 0)
                 {
-                    const uint32_t R = block_sum(pB, nOC, sbuf) + block_sum(pC, nCC, sbuf);
+                    const uint32_t R = block_sum2(pB, nOC, pC, nCC, sbuf);
                     if(threadIdx.x == 0) st.R = R;
                 }
             if(local < nCC)
@@ -772,8 +837,6 @@ __global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__
                     // new nodes of chunk `local`, each repeated repU times
                     const uint32_t c    = local;
                     const uint32_t u0   = c * kChunk + kPer * threadIdx.x;
-                    const uint32_t offA = P.partA[qd.chunkOff + c];
-                    const uint32_t offC = block_sum(pB, nOC, sbuf) + block_sum(pC, c, sbuf);
                     uint32_t rep[kPer] = {0u, 0u, 0u, 0u};
                     if(u0 < nC)
                         {
@@ -783,6 +846,8 @@ __global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__
                             for(uint32_t k = 1; k < kPer; ++k)
                                 if(u0 + k >= nC) rep[k] = 0u;
                         }
+                    const uint32_t offA = P.partA[qd.chunkOff + c];
+                    const uint32_t offC = block_sum2(pB, nOC, pC, c, sbuf);
                     uint32_t nn = 0, ns = 0;
 #pragma unroll
                     for(uint32_t k = 0; k < kPer; ++k) nn += rep[k] ? 1u : 0u, ns += rep[k];
