@@ -2,23 +2,26 @@
 //
 // One iteration of the planner is five launches over a table of queries
 // (a single-query plan is a table of one):
-//   k_propagate  persistent; one lane per child: Philox draws, RK4 rollout,
-//                AABB test against obstacles staged in LDS, child written SoA,
-//                region counters via wave-aggregated atomics, subregion bitset,
-//                next-frontier flag, per-chunk flag counts.
+//   k_propagate  persistent over tiles of children: Philox draws, per-wave
+//                sort by step count, RK4 rollout, AABB test against the
+//                obstacles staged in LDS (candidate boxes per wave), child
+//                written SoA, region counters and new subregion bits summed
+//                per tile in LDS, next-frontier flag, per-chunk flag counts.
 //                (KGMT.cu:218-299, statePropagator.cu, collisionCheck.cu)
 //   k_score      one workgroup per query: region scores with a fixed pairwise
 //                summation order, bitset snapshot, scan of the flag counts.
 //                (Graph.cu:207-358, KGMT.cu:383-388)
 //   k_append     persistent over 1024-element chunk jobs: compaction of the
 //                flagged children and tree append, goal test; re-activation
-//                of old nodes. The last job of a query scans its partials.
-//                (KGMT.cu:305-378)
-//   k_expand     same jobs: writes the next expansion list (segmented repeat).
-//                (KGMT.cu:170-181, Planner.cu:52-80). The last job of each
-//                query closes its iteration: tree size, goal path, next
-//                launch regime (KGMT.cu:84-95, 183-201, 342-361, 399-402); the
-//                last query to close rebuilds the work lists of all queries.
+//                of old nodes. (KGMT.cu:305-378)
+//   k_expand     same jobs: writes the next expansion list (segmented repeat);
+//                each job sums the chunk counts before its own.
+//                (KGMT.cu:170-181, Planner.cu:52-80)
+//   k_finish     one workgroup: closes the iteration of every query (tree
+//                size, stop rules, next launch regime; KGMT.cu:84-95, 183-201,
+//                399-402) and builds the next iteration's work lists.
+// Per plan: k_build_scenes before the first iteration, k_init, and k_paths
+// (goal path walk, KGMT.cu:342-361) after the last.
 // All sizes stay on the device: the host only polls a completion word that
 // the kernels write into host memory.
 #include "kgmt_launch.hpp"
