@@ -42,6 +42,7 @@ PLAN_CASES = [
     ("quad12d", 0, dict(capacity=1 << 13, max_iter=12)),
     ("unicycle2d", 0, dict()),
     ("unicycle2d", 4, dict(capacity=20000)),
+    ("di6d", 2, dict(capacity=1 << 15, r1_len=20, r2_len=4)),  # 8000 regions: many distinct keys per tile
 ]
 
 
@@ -220,3 +221,58 @@ def test_batch_rejects_mixed_workspace_dims():
     cfgs = [W.config("unicycle2d", capacity=1 << 12), W.config("di6d", capacity=1 << 12)]
     with pytest.raises(KgmtError):
         BatchKGMT(cfgs)
+
+
+def _odd_boxes(w, rng):
+    """Boxes the separating-axis test must treat exactly as the reference's
+    comparisons do: NaN faces, infinite faces (slabs), flat and inverted
+    boxes, next to ordinary ones."""
+    nan, inf = float("nan"), float("inf")
+    base = W.random_boxes(rng, 40, 0.03, 0.12, keep_clear=[(np.full(w, 0.05), 0.08), (np.full(w, 0.95), 0.08)],
+                          dim=w).astype(np.float64)
+    odd = []
+    for k in range(12):
+        lo = rng.uniform(0.2, 0.8, w)
+        hi = lo + rng.uniform(0.02, 0.1, w)
+        b = np.concatenate([lo, hi])
+        a = k % w
+        if k % 6 == 0:
+            b[a] = nan                      # NaN low face
+        elif k % 6 == 1:
+            b[w + a] = nan                  # NaN high face
+        elif k % 6 == 2:
+            b[a], b[w + a] = -inf, 0.1      # slab open below
+        elif k % 6 == 3:
+            b[w + a] = inf                  # slab open above
+        elif k % 6 == 4:
+            b[w + a] = b[a]                 # flat box
+        else:
+            b[a], b[w + a] = b[w + a], b[a]  # inverted on one axis
+        odd.append(b)
+    return np.concatenate([base, np.asarray(odd)]).astype(np.float32)
+
+
+@pytest.mark.parametrize("name", ["dubins5d", "di6d", "unicycle2d"])
+def test_unusual_obstacles_match_oracle(name, orc):
+    """Whole plans (candidate prefilter, register-held candidates, face-gap
+    test) against the oracle's literal comparisons, on scenes with NaN and
+    infinite faces, flat and inverted boxes."""
+    rng = np.random.default_rng(21)
+    q = W.make_query(name, 7, capacity=1 << 14, max_iter=40)
+    obs = _odd_boxes(q.cfg.wdim, rng)
+    ref = orc.plan(q.cfg, q.start, q.goal, obs)
+    g = KGMT(q.cfg)
+    res = g.plan(q.start, q.goal, obs, len(obs))
+    _same_plan(g, res, ref)
+    assert ref["stats"]["tree_size"] > 50
+
+
+def test_long_dubins_edges_match_oracle(orc):
+    """Edges of 100-300 steps: the rotated sin/cos drift over many steps and
+    the edge bound's per-step padding keep every candidate."""
+    q = W.make_query("dubins5d", 8, capacity=1 << 13, max_iter=12, dt=0.005, steps_min=100, steps_max=300)
+    ref = orc.plan(q.cfg, q.start, q.goal, q.obstacles)
+    g = KGMT(q.cfg)
+    res = g.plan(q.start, q.goal, q.obstacles, len(q.obstacles))
+    _same_plan(g, res, ref)
+    assert ref["stats"]["tree_size"] > 20
