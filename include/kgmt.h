@@ -144,7 +144,8 @@ typedef struct kgmt_result {
 /* Per-kernel device time accumulated while profiling is on (HIP events
  * around every launch of the plain, non-graph iteration loop). Kernel order:
  * 0 propagate, 1 score, 2 append, 3 expand, 4 finish (closes the iteration of
- * every query and builds the next iteration's work lists). */
+ * every query and builds the next iteration's work lists; a single-query
+ * context has no finish launch: its first expand job does that). */
 #define KGMT_PROFILE_KERNELS 5
 typedef struct kgmt_profile {
     uint64_t launches[KGMT_PROFILE_KERNELS];

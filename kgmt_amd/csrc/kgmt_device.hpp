@@ -75,7 +75,25 @@ struct QueryState
     uint64_t expanded;
     uint32_t finishItr;
     uint32_t pad[3];
+    // The next iteration's values, written when an iteration closes (by the
+    // first expand job of a single-query plan, by k_finish for a batch); propagate
+    // reads them, the score kernel copies them to the fields above (k_paths
+    // after the last iteration), and append/expand read those.
+    uint32_t itrN, treeSizeN, nChildrenN, perN, doneN, successN, finishItrN, padN;
+    uint64_t expandedN;
 };
+
+__device__ __forceinline__ void commit_state(QueryState& st)
+{
+    st.itr       = st.itrN;
+    st.treeSize  = st.treeSizeN;
+    st.nChildren = st.nChildrenN;
+    st.per       = st.perN;
+    st.done      = st.doneN;
+    st.success   = st.successN;
+    st.finishItr = st.finishItrN;
+    st.expanded  = st.expandedN;
+}
 
 // Pointers of the device pools (shared by all queries of a batch).
 struct Pools

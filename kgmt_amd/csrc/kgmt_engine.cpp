@@ -381,8 +381,12 @@ struct Engine
         if(all) HIP_TRY(hipEventRecord(marks[3], stream));
         HIP_TRY(launch_expand(dq, ds, P, W, parity, L, stream));
         if(all) HIP_TRY(hipEventRecord(marks[4], stream));
-        HIP_TRY(launch_finish(dq, ds, W, parity, stream));
-        if(all) HIP_TRY(hipEventRecord(marks[5], stream));
+        // a single query's first expand job closes the iteration itself
+        if(nq > 1)
+            {
+                HIP_TRY(launch_finish(dq, ds, W, parity, stream));
+                if(all) HIP_TRY(hipEventRecord(marks[5], stream));
+            }
         return KGMT_OK;
     }
 
@@ -406,7 +410,9 @@ struct Engine
         for(uint32_t it = 0; it < iters; ++it)
             {
                 hipEvent_t* m = &profEv[(size_t)it * (KGMT_PROFILE_KERNELS + 1)];
-                for(int k = 0; k < (profiling > 1 ? KGMT_PROFILE_KERNELS : 1); ++k)
+                // no finish launch for a single query
+                const int nk = profiling > 1 ? (nq > 1 ? KGMT_PROFILE_KERNELS : KGMT_PROFILE_KERNELS - 1) : 1;
+                for(int k = 0; k < nk; ++k)
                     {
                         float ms = 0.0f;
                         HIP_TRY(hipEventElapsedTime(&ms, m[k], m[k + 1]));

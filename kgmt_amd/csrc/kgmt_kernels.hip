@@ -225,12 +225,12 @@ __device__ void build_worklists(const QueryState* sts, const WorkLists& W, uint3
             uint32_t tiles = 0, jobs = 0, alive = 0;
             if(q < nq)
                 {
-                    const uint32_t done = __hip_atomic_load(&sts[q].done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint32_t done = __hip_atomic_load(&sts[q].doneN, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if(!done)
                         {
-                            const uint32_t nc = __hip_atomic_load(&sts[q].nChildren, __ATOMIC_RELAXED,
+                            const uint32_t nc = __hip_atomic_load(&sts[q].nChildrenN, __ATOMIC_RELAXED,
                                                                   __HIP_MEMORY_SCOPE_AGENT);
-                            const uint32_t tr = __hip_atomic_load(&sts[q].treeSize, __ATOMIC_RELAXED,
+                            const uint32_t tr = __hip_atomic_load(&sts[q].treeSizeN, __ATOMIC_RELAXED,
                                                                   __HIP_MEMORY_SCOPE_AGENT);
                             tiles             = (nc + W.tile - 1) / W.tile;
                             jobs              = (nc + kChunk - 1) / kChunk + (tr + kChunk - 1) / kChunk;
@@ -324,6 +324,9 @@ __global__ __launch_bounds__(kBlock) void k_init(const QueryDesc* __restrict__ q
             st.jobsDone  = 0;
             st.expanded  = 0;
             st.finishItr = 0;
+            st.itrN = st.itr, st.treeSizeN = st.treeSize, st.nChildrenN = st.nChildren, st.perN = st.per;
+            st.doneN = 0, st.successN = 0, st.finishItrN = 0, st.padN = 0;
+            st.expandedN = 0;
         }
     if(last_arrival(W.counter, nq, &sflag)) build_worklists<kBlock>(sts, W, 0, sbuf);
 }
@@ -381,7 +384,7 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
 {
     constexpr int S = ModelDims<MODEL>::S, C = ModelDims<MODEL>::C, D = S + C + 1;
     constexpr uint32_t TC = NT / K;
-    const uint32_t nC  = st.nChildren;
+    const uint32_t nC  = st.nChildrenN;
     const uint32_t cap = qd.cap;
     const size_t nodeO = qd.nodeOff;
     const uint32_t t   = threadIdx.x;
@@ -399,7 +402,7 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
         uint32_t bucket       = g0 + BW - 1;                  // past the end: last
         if(t < TC && un < nC)
             {
-                draw_child<C>(qd.cfg, qd.cfg.query_id, st.itr, un, uc, steps, uf);
+                draw_child<C>(qd.cfg, qd.cfg.query_id, st.itrN, un, uc, steps, uf);
                 const int32_t k = steps - qd.cfg.steps_min;
                 bucket = g0 + (BW - 2) - (uint32_t)(k < (int32_t)BW - 2 ? k : (int32_t)BW - 2);
             }
@@ -432,7 +435,7 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
     bool ok = false, fn = false;
     bool setBit   = false;  // valid child in a subregion unseen at iteration start
     uint32_t r2bit = 0;
-    const uint32_t per = st.per;
+    const uint32_t per = st.perN;
     uint32_t p         = 0;
     float s[S], uc[C];
     int32_t steps = 0;
@@ -530,7 +533,7 @@ __global__ __launch_bounds__(NT, 4) void k_propagate(const QueryDesc* __restrict
     for(uint32_t t = blockIdx.x; t < nTiles; t += gridDim.x)
         {
             const uint32_t q    = locate(tileList, nq, t);
-            if(sts[q].done) continue;  // stale list entry of a finished query
+            if(sts[q].doneN) continue;  // stale list entry of a finished query
             const QueryDesc& qd = qds[q];
             if(q != staged)
                 {
@@ -574,6 +577,9 @@ __global__ __launch_bounds__(kScoreThreads) void k_score(const QueryDesc* __rest
     __shared__ uint32_t sbuf[kScoreThreads / 64 + 1];
     const uint32_t q = blockIdx.x;
     QueryState& st   = sts[q];
+    // the state of this iteration, closed by the previous one
+    if(threadIdx.x == 0) commit_state(st);
+    __syncthreads();
     if(st.done) return;
     const QueryDesc& qd = qds[q];
     score_regions<kScoreThreads>(qd, P.total + qd.regOff, P.validc + qd.regOff, P.actNext + qd.subOff,
@@ -752,31 +758,50 @@ __global__ __launch_bounds__(kBlock) void k_append(const QueryDesc* __restrict__
 // End of an iteration for one query (one thread): tree size, stop rules,
 // next launch regime. The goal path is walked once the plan has ended
 // (k_paths).
-__device__ void finish_query(const QueryDesc& qd, QueryState& st)
+__device__ void finish_query(const QueryDesc& qd, QueryState& st, uint32_t R)
 {
     const uint32_t tree = st.treeSize + st.next;
-    st.treeSize         = tree;
-    st.expanded += st.nChildren;
+    uint32_t itr = st.itr, per = st.per, nc = st.nChildren, done = 0, success = 0, fin = st.finishItr;
     if(st.goalIdx != kNoGoal)
         {
-            st.success   = 1;
-            st.done      = 1;
-            st.finishItr = st.itr;
+            success = 1;
+            done    = 1;
+            fin     = itr;
         }
-    else if(st.itr >= qd.cfg.max_iter || tree == qd.cap)
+    else if(itr >= qd.cfg.max_iter || tree == qd.cap)
         {
             // out of iterations, or a full tree that can take no further node
-            st.done      = 1;
-            st.finishItr = st.itr;
+            done = 1;
+            fin  = itr;
         }
     else
         {
-            uint32_t per, nc;
-            launch_regime(qd, st.R, tree, per, nc);
-            st.per       = per;
-            st.nChildren = nc;
-            st.itr       = st.itr + 1;
+            launch_regime(qd, R, tree, per, nc);
+            itr = itr + 1;
         }
+    st.treeSizeN  = tree;
+    st.expandedN  = st.expanded + st.nChildren;
+    st.itrN       = itr;
+    st.perN       = per;
+    st.nChildrenN = nc;
+    st.doneN      = done;
+    st.successN   = success;
+    st.finishItrN = fin;
+}
+
+// Work lists of a single-query table for its next iteration (parity
+// buffer), from the next-state fields (one thread).
+__device__ __forceinline__ void single_worklists(const QueryState& st, const WorkLists& W, uint32_t parity)
+{
+    uint32_t* tileStart = W.tileStart + parity * 2;
+    uint32_t* jobStart  = W.jobStart + parity * 2;
+    const bool alive    = st.doneN == 0;
+    tileStart[0]        = 0;
+    jobStart[0]         = 0;
+    tileStart[1]        = alive ? (st.nChildrenN + W.tile - 1) / W.tile : 0u;
+    jobStart[1] = alive ? (st.nChildrenN + kChunk - 1) / kChunk + (st.treeSizeN + kChunk - 1) / kChunk : 0u;
+    W.active[parity] = alive ? 1u : 0u;
+    __hip_atomic_store(W.allDone, alive ? 0u : 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // sum of a[0..n) over the workgroup
@@ -830,9 +855,10 @@ __global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__
             uint32_t* out         = P.repIdx + nodeO;
             const uint32_t* pB    = P.partB + qd.chunkOff;
             const uint32_t* pC    = P.partC + qd.chunkOff;
+            uint32_t R = 0;
             if(local == 0)
                 {
-                    const uint32_t R = block_sum2(pB, nOC, pC, nCC, sbuf);
+                    R = block_sum2(pB, nOC, pC, nCC, sbuf);
                     if(threadIdx.x == 0) st.R = R;
                 }
             if(local < nCC)
@@ -890,6 +916,14 @@ __global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__
                                 ++pos;
                             }
                 }
+            // a single query's first job closes the iteration (the next-state
+            // fields and work lists, read from the next kernel on; the other
+            // jobs read only the current fields), so it needs no k_finish
+            if(nq == 1 && local == 0 && threadIdx.x == 0)
+                {
+                    finish_query(qd, st, R);
+                    single_worklists(st, W, parity ^ 1u);
+                }
             __syncthreads();
         }
 }
@@ -903,7 +937,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(const QueryDesc* __restrict__
 {
     __shared__ uint32_t sbuf[kBlock / 64 + 1];
     for(uint32_t q = threadIdx.x; q < W.nq; q += kBlock)
-        if(!sts[q].done) finish_query(qds[q], sts[q]);
+        if(!sts[q].done) finish_query(qds[q], sts[q], sts[q].R);
     __syncthreads();
     build_worklists<kBlock>(sts, W, parity ^ 1u, sbuf);
 }
@@ -919,6 +953,7 @@ __global__ __launch_bounds__(64) void k_paths(const QueryDesc* __restrict__ qds,
     const uint32_t q = blockIdx.x * 64 + threadIdx.x;
     if(q >= nq) return;
     QueryState& st = sts[q];
+    commit_state(st);  // the closed last iteration
     if(!st.success || st.pathRows) return;
     const QueryDesc& qd = qds[q];
     const uint32_t cap  = qd.cap;
