@@ -405,9 +405,10 @@ struct CandCheck
     }
 };
 
+// bitwise ORs: every comparison is evaluated (no short-circuit branches)
 __device__ __forceinline__ bool ws_out(const kgmt_config& c, float x, float y, float z)
 {
-    return x < c.ws_min || x > c.ws_max || y < c.ws_min || y > c.ws_max || z < c.ws_min || z > c.ws_max;
+    return (x < c.ws_min) | (x > c.ws_max) | (y < c.ws_min) | (y > c.ws_max) | (z < c.ws_min) | (z > c.ws_max);
 }
 
 // -------------------------------------------------------------------------
@@ -478,7 +479,7 @@ __device__ __forceinline__ bool prop_unicycle(const QueryDesc& qd, float* s, con
             v     = v + a * h;
             // workspace then segment, both ending the edge: one branch
             const bool free = chk(px, py, 0.0f, x, y, 0.0f);
-            if(x < c.ws_min || x > c.ws_max || y < c.ws_min || y > c.ws_max || !free)
+            if((x < c.ws_min) | (x > c.ws_max) | (y < c.ws_min) | (y > c.ws_max) | !free)
                 {
                     valid = false;
                     break;
@@ -521,8 +522,8 @@ __device__ __forceinline__ bool prop_di(const QueryDesc& qd, float* s, const flo
                 }
             // velocity, workspace, segment: each ends the edge, so one branch
             const bool free = chk(px, py, pz, x, y, z);
-            if(vx < c.v_min || vx > c.v_max || vy < c.v_min || vy > c.v_max || vz < c.v_min || vz > c.v_max ||
-               ws_out(c, x, y, z) || !free)
+            if((vx < c.v_min) | (vx > c.v_max) | (vy < c.v_min) | (vy > c.v_max) | (vz < c.v_min) | (vz > c.v_max) |
+               ws_out(c, x, y, z) | !free)
                 {
                     valid = false;
                     break;
@@ -592,8 +593,8 @@ __device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const
             // segment) all end the edge as invalid at this step, so they are
             // combined; the segment test runs for every lane (one branch)
             const bool free = chk(px, py, pz, x, y, z);
-            if((!FIVE_D && (v < c.v_min || v > c.v_max)) || pitch < c.pitch_min || pitch > c.pitch_max ||
-               ws_out(c, x, y, z) || !free)
+            if((!FIVE_D & ((v < c.v_min) | (v > c.v_max))) | (pitch < c.pitch_min) | (pitch > c.pitch_max) |
+               ws_out(c, x, y, z) | !free)
                 {
                     valid = false;
                     break;
@@ -657,8 +658,8 @@ __device__ __forceinline__ bool prop_quad(const QueryDesc& qd, float* s, const f
             for(int j = 0; j < 12; ++j) s[j] = s[j] + h6 * (k1[j] + 2.0f * k2[j] + 2.0f * k3[j] + k4[j]);
             // body velocity, workspace, segment: each ends the edge, so one branch
             const bool free = chk(px, py, pz, s[0], s[1], s[2]);
-            if(s[6] < c.v_min || s[6] > c.v_max || s[7] < c.v_min || s[7] > c.v_max || s[8] < c.v_min ||
-               s[8] > c.v_max || ws_out(c, s[0], s[1], s[2]) || !free)
+            if((s[6] < c.v_min) | (s[6] > c.v_max) | (s[7] < c.v_min) | (s[7] > c.v_max) | (s[8] < c.v_min) |
+               (s[8] > c.v_max) | ws_out(c, s[0], s[1], s[2]) | !free)
                 {
                     valid = false;
                     break;
