@@ -965,47 +965,96 @@ __device__ __forceinline__ uint32_t lane_id()
     return __lane_id();
 }
 
-// inclusive scan of v over the 64 lanes of a wave
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+// v of the lane `CTRL` selects (DPP), 0 where the source lies outside the row
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t v)
 {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for(int d = 1; d < 64; d <<= 1)
-        {
-            const uint32_t t = __shfl_up(v, d, 64);
-            if(lane >= (uint32_t)d) v += t;
-        }
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+
+// inclusive scan of v over each row of 16 lanes (row_shr 1, 2, 4, 8)
+__device__ __forceinline__ uint32_t row_incl_scan(uint32_t v)
+{
+    v += dpp_u<0x111>(v);
+    v += dpp_u<0x112>(v);
+    v += dpp_u<0x114>(v);
+    v += dpp_u<0x118>(v);
     return v;
 }
 
+// inclusive scan of v over the 64 lanes of a wave: row scans, then the row
+// totals (lanes 15, 31, 47) added to the rows after them
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+    v                   = row_incl_scan(v);
+    const uint32_t r0   = (uint32_t)__builtin_amdgcn_readlane((int)v, 15);
+    const uint32_t r1   = (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
+    const uint32_t r2   = (uint32_t)__builtin_amdgcn_readlane((int)v, 47);
+    const uint32_t row  = lane_id() >> 4;
+    const uint32_t r01  = r0 + r1;
+    return v + (row == 0 ? 0u : row == 1 ? r0 : row == 2 ? r01 : r01 + r2);
+}
+
 // Exclusive scan of one value per thread over a block of NT threads
-// (NT/64 waves). Returns the exclusive prefix; *total receives the block sum.
-// sbuf: at least NT/64 + 1 words of LDS. Contains barriers: call uniformly.
+// (NT/64 <= 16 waves). Returns the exclusive prefix; *total receives the
+// block sum. Every wave scans the wave totals itself (one row scan), so there
+// is one barrier before and one after (sbuf reuse). sbuf: at least NT/64 + 1
+// words of LDS. Contains barriers: call uniformly.
 template <int NT>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sbuf, uint32_t& total)
 {
-    constexpr int NW     = NT / 64;
-    const uint32_t lane  = lane_id();
-    const uint32_t wave  = threadIdx.x / 64;
-    const uint32_t incl  = wave_incl_scan(v);
-    if(lane == 63) sbuf[wave] = incl;
-    __syncthreads();
-    if(threadIdx.x == 0)
+    constexpr int NW = NT / 64;
+    static_assert(NW >= 1 && NW <= 16, "block of 64 to 1024 threads");
+    const uint32_t incl = wave_incl_scan(v);
+    if constexpr(NW == 1)
         {
-            uint32_t run = 0;
-            for(int w = 0; w < NW; ++w)
-                {
-                    const uint32_t t = sbuf[w];
-                    sbuf[w]          = run;
-                    run += t;
-                }
-            sbuf[NW] = run;
+            total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+            __syncthreads();
+            return incl - v;
+        }
+    else
+        {
+            const uint32_t lane = lane_id();
+            const int wave      = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+            if(lane == 63) sbuf[wave] = incl;
+            __syncthreads();
+            const uint32_t w  = row_incl_scan(lane < (uint32_t)NW ? sbuf[lane] : 0u);
+            const uint32_t pw = wave == 0 ? 0u : (uint32_t)__builtin_amdgcn_readlane((int)w, wave - 1);
+            total             = (uint32_t)__builtin_amdgcn_readlane((int)w, NW - 1);
+            __syncthreads();
+            return pw + incl - v;
+        }
+}
+
+// Exclusive scans of NV values per thread over a block of NT threads (2 to
+// 16 waves) with one barrier pair: ex[i] = prefix of v[i], total[i] = block
+// sum of v[i]. sbuf: at least NV * NT/64 words of LDS. Call uniformly.
+template <int NT, int NV>
+__device__ __forceinline__ void block_excl_scan_n(const uint32_t (&v)[NV], uint32_t (&ex)[NV], uint32_t (&total)[NV],
+                                                  uint32_t* sbuf)
+{
+    constexpr int NW = NT / 64;
+    static_assert(NW >= 2 && NW <= 16, "block of 128 to 1024 threads");
+    const uint32_t lane = lane_id();
+    const int wave      = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+    uint32_t incl[NV];
+#pragma unroll
+    for(int i = 0; i < NV; ++i) incl[i] = wave_incl_scan(v[i]);
+    if(lane == 63)
+        {
+#pragma unroll
+            for(int i = 0; i < NV; ++i) sbuf[i * NW + wave] = incl[i];
         }
     __syncthreads();
-    const uint32_t ex = sbuf[wave] + incl - v;
-    total             = sbuf[NW];
+#pragma unroll
+    for(int i = 0; i < NV; ++i)
+        {
+            const uint32_t w  = row_incl_scan(lane < (uint32_t)NW ? sbuf[i * NW + lane] : 0u);
+            const uint32_t pw = wave == 0 ? 0u : (uint32_t)__builtin_amdgcn_readlane((int)w, wave - 1);
+            total[i]          = (uint32_t)__builtin_amdgcn_readlane((int)w, NW - 1);
+            ex[i]             = pw + incl[i] - v[i];
+        }
     __syncthreads();
-    return ex;
 }
 
 // Exclusive scan (in place) of n words with one block of NT threads,

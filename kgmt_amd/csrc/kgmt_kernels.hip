@@ -113,17 +113,33 @@ __device__ __forceinline__ void wave_set_bits(uint32_t* words, uint32_t bit, boo
 constexpr int kScoreThreads = 1024;
 constexpr int kMaxRegions   = 8192;
 
+// v of lane + C (C <= 8: row_shl, only lanes whose source stays in the row
+// are used; C >= 16: shuffle)
+template <int C>
+__device__ __forceinline__ float lane_down(float v)
+{
+    if constexpr(C <= 8)
+        return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + C, 0xF, 0xF, false));
+    else
+        return __shfl_down(v, C, 64);
+}
+
 // Region scores of one query (block of NT threads, R1 <= kMaxRegions). Q6
 // semantics: closed formula, then the pairwise sum a[i] += a[i + st] for
 // st = P2/2, ..., 1 over the scores padded with zeros to P2 = pow2(R1). Thread
 // t owns regions t + k NT: its inputs are loaded together, and the levels with
 // st >= NT add a thread's own registers (i and i + st differ by a multiple of
-// NT), the rest go through LDS (a: at least NT floats).
+// NT). The NT partial sums then go through LDS once (a: at least NT floats,
+// not read after the call): lane l of the first wave takes sums l + 64 j, so
+// the levels 64 <= st < NT add the lane's own registers too, and the last six
+// levels pair lanes l and l + st within the wave. Every level adds the same
+// two partial sums as the plain tree, so the total has the same bits.
 template <int NT>
 __device__ void score_regions(const QueryDesc& qd, const int32_t* __restrict__ total,
                               const int32_t* __restrict__ validc, const uint32_t* __restrict__ act,
                               float* __restrict__ score, float* a)
 {
+    static_assert(NT % 64 == 0 && NT <= 1024, "score block");
     constexpr int KR = kMaxRegions / NT;
     const int R1     = qd.R1;
     const uint32_t P = (uint32_t)qd.P;
@@ -166,14 +182,33 @@ __device__ void score_regions(const QueryDesc& qd, const int32_t* __restrict__ t
             for(int k = 0; k < KR; ++k)
                 if(k < m && k + m < KR) sv[k] = sv[k] + sv[k + m];
         }
-    // a[i] for i < min(P2, NT): the thread's own partial sum (zero past P2)
+    // a[i] for i < NT: the thread's own partial sum (zero past P2)
     a[t] = t < P2 ? sv[0] : 0.0f;
     __syncthreads();
-    for(; st >= 1; st >>= 1)
+    if(t < 64)
         {
-            if(t < st) a[t] = a[t] + a[t + st];
-            __syncthreads();
+            constexpr int NJ = NT / 64;
+            float x[NJ];
+#pragma unroll
+            for(int j = 0; j < NJ; ++j) x[j] = a[t + 64 * j];
+            // levels st = 64 m < min(P2, NT): slot j += slot j + m
+#pragma unroll
+            for(int m = NJ / 2; m >= 1; m >>= 1)
+                if(64 * m < P2)
+                    {
+#pragma unroll
+                        for(int j = 0; j < m; ++j) x[j] = x[j] + x[j + m];
+                    }
+            float v = x[0];
+            if(32 < P2) v = v + lane_down<32>(v);
+            if(16 < P2) v = v + lane_down<16>(v);
+            if(8 < P2) v = v + lane_down<8>(v);
+            if(4 < P2) v = v + lane_down<4>(v);
+            if(2 < P2) v = v + lane_down<2>(v);
+            if(1 < P2) v = v + lane_down<1>(v);
+            if(t == 0) a[0] = v;  // after this wave's reads of a
         }
+    __syncthreads();
     const float sum = a[0];
 #pragma unroll
     for(int k = 0; k < KR; ++k)
@@ -181,7 +216,6 @@ __device__ void score_regions(const QueryDesc& qd, const int32_t* __restrict__ t
             const int r = t + k * NT;
             if(r < R1) score[r] = vc[k] == 0 ? 1.0f : eps + s0[k] / sum;
         }
-    __syncthreads();
 }
 
 __device__ __forceinline__ float faccept(const QueryDesc& qd, uint32_t itr, uint32_t tree, uint32_t next)
@@ -573,27 +607,48 @@ __global__ __launch_bounds__(NT, 4) void k_propagate(const QueryDesc* __restrict
 __global__ __launch_bounds__(kScoreThreads) void k_score(const QueryDesc* __restrict__ qds,
                                                          QueryState* __restrict__ sts, Pools P)
 {
-    __shared__ float a[kMaxRegions];
+    __shared__ float a[kScoreThreads];
     __shared__ uint32_t sbuf[kScoreThreads / 64 + 1];
     const uint32_t q = blockIdx.x;
     QueryState& st   = sts[q];
-    // the state of this iteration, closed by the previous one
+    // the state of this iteration, closed by the previous one: every thread
+    // reads the next-state fields, the first copies them to the current ones
+    const uint32_t done = st.doneN, nC = st.nChildrenN, tree = st.treeSizeN, itr = st.itrN;
     if(threadIdx.x == 0) commit_state(st);
-    __syncthreads();
-    if(st.done) return;
+    if(done) return;
     const QueryDesc& qd = qds[q];
+    // loads issued with the score inputs: the first bitset word to copy and
+    // the first chunk counts to scan
+    const uint32_t t   = threadIdx.x;
+    const uint32_t nCC = (nC + kChunk - 1) / kChunk;
+    const uint32_t nW  = (uint32_t)qd.R2words;
+    uint32_t* pa       = P.partA + qd.chunkOff;
+    const uint32_t aw  = t < nW ? P.actNext[qd.subOff + t] : 0u;
+    const uint32_t c0  = t < nCC ? pa[t] : 0u;
     score_regions<kScoreThreads>(qd, P.total + qd.regOff, P.validc + qd.regOff, P.actNext + qd.subOff,
                                  P.score + qd.regOff, a);
-    for(int w = threadIdx.x; w < qd.R2words; w += kScoreThreads) P.actPrev[qd.subOff + w] = P.actNext[qd.subOff + w];
-    const uint32_t nCC  = (st.nChildren + kChunk - 1) / kChunk;
-    const uint32_t nAcc = block_scan_array<kScoreThreads>(P.partA + qd.chunkOff, nCC, sbuf);
+    if(t < nW) P.actPrev[qd.subOff + t] = aw;
+    for(uint32_t w = t + kScoreThreads; w < nW; w += kScoreThreads) P.actPrev[qd.subOff + w] = P.actNext[qd.subOff + w];
+    // exclusive scan of the chunk counts in place
+    uint32_t nAcc;
+    const uint32_t ex0 = block_excl_scan<kScoreThreads>(c0, sbuf, nAcc);
+    if(t < nCC) pa[t] = ex0;
+    for(uint32_t base = kScoreThreads; base < nCC; base += kScoreThreads)
+        {
+            const uint32_t i = base + t;
+            const uint32_t v = i < nCC ? pa[i] : 0u;
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan<kScoreThreads>(v, sbuf, tot);
+            if(i < nCC) pa[i] = nAcc + ex;
+            nAcc += tot;
+        }
     if(threadIdx.x == 0)
         {
-            const uint32_t avail = qd.cap - st.treeSize;
+            const uint32_t avail = qd.cap - tree;
             const uint32_t next  = nAcc < avail ? nAcc : avail;
             st.nAcc              = nAcc;
             st.next              = next;
-            st.fAccept           = faccept(qd, st.itr, st.treeSize, next);
+            st.fAccept           = faccept(qd, itr, tree, next);
         }
 }
 
@@ -616,13 +671,17 @@ __device__ void append_children(const QueryDesc& qd, QueryState& st, uint32_t c,
     float* __restrict__ T       = P.treeS + qd.sampleOff;
     const int D            = qd.D;
     const uint32_t u0      = c * kChunk + kPer * threadIdx.x;
+    const uint32_t chunkA  = P.partA[qd.chunkOff + c];  // flagged children before the chunk
     uint32_t fl            = 0;
     int32_t r1[kPer]       = {0, 0, 0, 0};
     int32_t vcount[kPer]   = {0, 0, 0, 0};
+    float v[kPer][KGMT_MAX_SAMPLE];
+    int32_t par[kPer];
     if(u0 < nC)
         {
-            // flags and regions together, then the regions' valid counts (for
-            // the repeat count) before the rank scan
+            // flags and regions together, then (before the rank scan) the
+            // regions' valid counts for the repeat count and the rows of the
+            // flagged children, which are appended unless the tree fills up
             const uint32_t w = *reinterpret_cast<const uint32_t*>(P.fnext + nodeO + u0);
             const int4 rr    = *reinterpret_cast<const int4*>(P.r1child + nodeO + u0);
             r1[0] = rr.x, r1[1] = rr.y, r1[2] = rr.z, r1[3] = rr.w;
@@ -631,10 +690,18 @@ __device__ void append_children(const QueryDesc& qd, QueryState& st, uint32_t c,
                 if(u0 + k < nC && ((w >> (8 * k)) & 0xffu)) fl |= 1u << k;
 #pragma unroll
             for(uint32_t k = 0; k < kPer; ++k)
-                if((fl >> k) & 1u) vcount[k] = P.validc[qd.regOff + r1[k]];
+                if((fl >> k) & 1u)
+                    {
+                        const uint32_t u = u0 + k;
+                        vcount[k]        = P.validc[qd.regOff + r1[k]];
+#pragma unroll
+                        for(int d = 0; d < KGMT_MAX_SAMPLE; ++d)
+                            if(d < D) v[k][d] = U[(size_t)d * cap + u];
+                        par[k] = P.parentU[nodeO + u];
+                    }
         }
     uint32_t tot;
-    uint32_t rank = P.partA[qd.chunkOff + c] + block_excl_scan<kBlock>((uint32_t)__popc(fl), sbuf, tot);
+    uint32_t rank = chunkA + block_excl_scan<kBlock>((uint32_t)__popc(fl), sbuf, tot);
     // appended slots: u0 + k with flag k set and rank < next
     uint32_t app = 0, t0 = 0;
 #pragma unroll
@@ -651,19 +718,6 @@ __device__ void append_children(const QueryDesc& qd, QueryState& st, uint32_t c,
     uint32_t rep[kPer] = {0u, 0u, 0u, 0u};
     if(app)
         {
-            // all loads of the (at most four) children first, then the stores
-            float v[kPer][KGMT_MAX_SAMPLE];
-            int32_t par[kPer];
-#pragma unroll
-            for(uint32_t k = 0; k < kPer; ++k)
-                if((app >> k) & 1u)
-                    {
-                        const uint32_t u = u0 + k;
-#pragma unroll
-                        for(int d = 0; d < KGMT_MAX_SAMPLE; ++d)
-                            if(d < D) v[k][d] = U[(size_t)d * cap + u];
-                        par[k] = P.parentU[nodeO + u];
-                    }
             uint32_t t = t0;
 #pragma unroll
             for(uint32_t k = 0; k < kPer; ++k)
@@ -752,8 +806,7 @@ __global__ __launch_bounds__(kBlock) void k_append(const QueryDesc* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// expand: next expansion list (re-activated ascending, then new nodes each
-// repeated repU times), truncated at capacity entries
+// closing an iteration
 // ---------------------------------------------------------------------------
 // End of an iteration for one query (one thread): tree size, stop rules,
 // next launch regime. The goal path is walked once the plan has ended
@@ -804,28 +857,6 @@ __device__ __forceinline__ void single_worklists(const QueryState& st, const Wor
     __hip_atomic_store(W.allDone, alive ? 0u : 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// sum of a[0..n) over the workgroup
-__device__ __forceinline__ uint32_t block_sum(const uint32_t* a, uint32_t n, uint32_t* sbuf)
-{
-    uint32_t v = 0;
-    for(uint32_t i = threadIdx.x; i < n; i += kBlock) v += a[i];
-    uint32_t tot;
-    block_excl_scan<kBlock>(v, sbuf, tot);
-    return tot;
-}
-
-// sum of a[0..na) + b[0..nb) over the workgroup (one scan)
-__device__ __forceinline__ uint32_t block_sum2(const uint32_t* a, uint32_t na, const uint32_t* b, uint32_t nb,
-                                               uint32_t* sbuf)
-{
-    uint32_t v = 0;
-    for(uint32_t i = threadIdx.x; i < na; i += kBlock) v += a[i];
-    for(uint32_t i = threadIdx.x; i < nb; i += kBlock) v += b[i];
-    uint32_t tot;
-    block_excl_scan<kBlock>(v, sbuf, tot);
-    return tot;
-}
-
 // ---------------------------------------------------------------------------
 // expand: next expansion list (re-activated ascending, then new nodes each
 // repeated repU times), truncated at capacity entries. Each job sums the
@@ -836,7 +867,7 @@ __device__ __forceinline__ uint32_t block_sum2(const uint32_t* a, uint32_t na, c
 __global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
                                                    Pools P, WorkLists W, uint32_t parity)
 {
-    __shared__ uint32_t sbuf[kBlock / 64 + 1];
+    __shared__ uint32_t sbuf[3 * (kBlock / 64)];
     const uint32_t nq       = W.nq;
     const uint32_t* jobList = W.jobs(parity);
     const uint32_t nJobs    = jobList[nq];
@@ -855,17 +886,16 @@ __global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__
             uint32_t* out         = P.repIdx + nodeO;
             const uint32_t* pB    = P.partB + qd.chunkOff;
             const uint32_t* pC    = P.partC + qd.chunkOff;
-            uint32_t R = 0;
-            if(local == 0)
-                {
-                    R = block_sum2(pB, nOC, pC, nCC, sbuf);
-                    if(threadIdx.x == 0) st.R = R;
-                }
+            const uint32_t tx     = threadIdx.x;
+            // one scan of three values per job: [0] the job's own counts,
+            // [1] the entries before the chunk, [2] (first job only) the rest
+            // of the list, so that R = total[1] + total[2] there
+            uint32_t v[3] = {0u, 0u, 0u}, ex[3], tot[3];
             if(local < nCC)
                 {
                     // new nodes of chunk `local`, each repeated repU times
                     const uint32_t c    = local;
-                    const uint32_t u0   = c * kChunk + kPer * threadIdx.x;
+                    const uint32_t u0   = c * kChunk + kPer * tx;
                     uint32_t rep[kPer] = {0u, 0u, 0u, 0u};
                     if(u0 < nC)
                         {
@@ -876,14 +906,17 @@ __global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__
                                 if(u0 + k >= nC) rep[k] = 0u;
                         }
                     const uint32_t offA = P.partA[qd.chunkOff + c];
-                    const uint32_t offC = block_sum2(pB, nOC, pC, c, sbuf);
+                    for(uint32_t i = tx; i < nOC; i += kBlock) v[1] += pB[i];
+                    for(uint32_t i = tx; i < c; i += kBlock) v[1] += pC[i];
+                    if(local == 0)
+                        for(uint32_t i = tx; i < nCC; i += kBlock) v[2] += pC[i];
                     uint32_t nn = 0, ns = 0;
 #pragma unroll
                     for(uint32_t k = 0; k < kPer; ++k) nn += rep[k] ? 1u : 0u, ns += rep[k];
-                    uint32_t tot;
-                    const uint32_t ex = block_excl_scan<kBlock>((nn << 16) | ns, sbuf, tot);
-                    uint32_t t        = tree + offA + (ex >> 16);
-                    uint32_t pos      = offC + (ex & 0xffffu);
+                    v[0] = (nn << 16) | ns;
+                    block_excl_scan_n<kBlock, 3>(v, ex, tot, sbuf);
+                    uint32_t t        = tree + offA + (ex[0] >> 16);
+                    uint32_t pos      = tot[1] + (ex[0] & 0xffffu);
 #pragma unroll
                     for(uint32_t k = 0; k < kPer; ++k)
                         if(rep[k])
@@ -894,20 +927,24 @@ __global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__
                                 ++t;
                             }
                     // chunk counts start at zero for the next propagate
-                    if(threadIdx.x == 0) P.partA[qd.chunkOff + c] = 0u;
+                    if(tx == 0) P.partA[qd.chunkOff + c] = 0u;
                 }
             else
                 {
                     // re-activated old nodes of chunk `local - nCC`, ascending
                     const uint32_t o  = local - nCC;
-                    const uint32_t i0 = o * kChunk + kPer * threadIdx.x;
+                    const uint32_t i0 = o * kChunk + kPer * tx;
                     const uint32_t rw = i0 < tree ? *reinterpret_cast<const uint32_t*>(P.react + nodeO + i0) : 0u;
-                    uint32_t fl       = 0;
+                    for(uint32_t i = tx; i < o; i += kBlock) v[1] += pB[i];
+                    if(local == 0)
+                        for(uint32_t i = tx; i < nOC; i += kBlock) v[2] += pB[i];
+                    uint32_t fl = 0;
 #pragma unroll
                     for(uint32_t k = 0; k < kPer; ++k)
                         if(i0 + k < tree && ((rw >> (8 * k)) & 0xffu)) fl |= 1u << k;
-                    uint32_t tot;
-                    uint32_t pos = block_sum(pB, o, sbuf) + block_excl_scan<kBlock>((uint32_t)__popc(fl), sbuf, tot);
+                    v[0] = (uint32_t)__popc(fl);
+                    block_excl_scan_n<kBlock, 3>(v, ex, tot, sbuf);
+                    uint32_t pos = tot[1] + ex[0];
 #pragma unroll
                     for(uint32_t k = 0; k < kPer; ++k)
                         if((fl >> k) & 1u)
@@ -916,13 +953,19 @@ __global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__
                                 ++pos;
                             }
                 }
-            // a single query's first job closes the iteration (the next-state
-            // fields and work lists, read from the next kernel on; the other
-            // jobs read only the current fields), so it needs no k_finish
-            if(nq == 1 && local == 0 && threadIdx.x == 0)
+            if(local == 0 && tx == 0)
                 {
-                    finish_query(qd, st, R);
-                    single_worklists(st, W, parity ^ 1u);
+                    const uint32_t R = tot[1] + tot[2];
+                    st.R             = R;
+                    // a single query's first job closes the iteration (the
+                    // next-state fields and work lists, read from the next
+                    // kernel on; the other jobs read only the current
+                    // fields), so it needs no k_finish
+                    if(nq == 1)
+                        {
+                            finish_query(qd, st, R);
+                            single_worklists(st, W, parity ^ 1u);
+                        }
                 }
             __syncthreads();
         }
@@ -1230,7 +1273,7 @@ __global__ __launch_bounds__(kScoreThreads) void k_stage_score(const QueryDesc* 
                                                                const int32_t* __restrict__ validc,
                                                                const uint32_t* __restrict__ act, float* score)
 {
-    __shared__ float a[kMaxRegions];
+    __shared__ float a[kScoreThreads];
     score_regions<kScoreThreads>(*qd, total, validc, act, score, a);
 }
 
