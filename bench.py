@@ -39,14 +39,15 @@ WORKLOAD_DESC = {
 }
 
 
-def bytes_per_child(model: int) -> int:
-    """Algorithmic HBM bytes of the propagate kernel per child: expansion-list
-    entry 4 B, parent state 4S, child sample 4D, parent index 4 B, child
-    region 4 B, next-frontier flag 1 B. Controls come from the counter-based
+def propagate_bytes(model: int, children: int, flagged: int) -> int:
+    """Algorithmic HBM bytes of the propagate kernel: per child the
+    expansion-list entry 4 B, parent state 4S, region 4 B and next-frontier
+    flag 1 B; per flagged child (the only ones the tree append reads) the
+    child sample 4D and parent index 4 B. Controls come from the counter-based
     RNG in registers, so no control bytes are read."""
     from kgmt_amd.abi import dims
     S, C, D = dims(model)
-    return 4 + 4 * S + 4 * D + 4 + 4 + 1
+    return children * (4 + 4 * S + 4 + 1) + flagged * (4 * D + 4)
 
 
 def run_plain(name, steps, warmup, profile=1, launch=None):
@@ -282,8 +283,8 @@ def main(argv=None):
 
     # roofline of the dominant kernel (propagate)
     pk = prof["kernels"]["propagate"]
-    B = bytes_per_child(cfg.model)
-    bytes_total = B * prof["children"]
+    bytes_total = propagate_bytes(cfg.model, prof["children"], prof["flagged"])
+    B = bytes_total / max(prof["children"], 1)
     avg_ms = pk["ms"] / max(pk["launches"], 1)
     achieved = bytes_total / (pk["ms"] * 1e-3) / 1e9
     per_launch = prof["children"] / max(pk["launches"], 1)
@@ -299,7 +300,8 @@ def main(argv=None):
         "traffic": traffic,
         "traffic_source": "profiles/k_propagate_traffic.json (rocprofv3 FETCH_SIZE x2.0, WRITE_SIZE x1.0, per child)"
         if tr else None,
-        "bytes_per_child": B,
+        "bytes_per_child": round(B, 2),
+        "flagged_fraction": round(prof["flagged"] / max(prof["children"], 1), 4),
         "children_per_launch": round(per_launch, 1),
         "avg_launch_us": round(avg_ms * 1e3, 2),
         "kernel_avg_us": breakdown,

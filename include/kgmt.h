@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define KGMT_ABI_VERSION 2
+#define KGMT_ABI_VERSION 3
 
 /* ---- status codes -------------------------------------------------------- */
 enum {
@@ -152,6 +152,8 @@ typedef struct kgmt_profile {
     double   ms[KGMT_PROFILE_KERNELS];
     uint64_t children;   /* children propagated by the profiled launches */
     uint64_t iterations; /* profiled iterations */
+    uint64_t flagged;    /* of those children, the ones flagged for the tree (the
+                            propagate kernel writes their rows only) */
 } kgmt_profile;
 
 /* Trajectory refinement (differentiable RK4 rollout of the quadcopter ODE).

@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-from .abi import KgmtAdjointParams, KgmtConfig, KgmtProfile, KgmtResult
+from .abi import KGMT_ABI_VERSION, KgmtAdjointParams, KgmtConfig, KgmtProfile, KgmtResult
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libkgmt.so")
 
@@ -84,6 +84,11 @@ def lib() -> C.CDLL:
             fn = getattr(handle, name)
             fn.restype = res
             fn.argtypes = args
+        found = handle.kgmt_abi_version()
+        if found != KGMT_ABI_VERSION:
+            raise KgmtLibraryMissing(
+                f"{LIB_PATH} has ABI version {found}, the bindings expect {KGMT_ABI_VERSION}: rebuild it "
+                "(`python -m kgmt_amd.build`)")
         _lib = handle
     return _lib
 

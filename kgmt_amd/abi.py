@@ -6,7 +6,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-KGMT_ABI_VERSION = 2
+KGMT_ABI_VERSION = 3
 
 KGMT_OK = 0
 KGMT_E_ARG = -1
@@ -176,6 +176,7 @@ class KgmtProfile(C.Structure):
         ("ms", C.c_double * KGMT_PROFILE_KERNELS),
         ("children", C.c_uint64),
         ("iterations", C.c_uint64),
+        ("flagged", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -184,9 +185,11 @@ class KgmtProfile(C.Structure):
                         for i, n in enumerate(PROFILE_KERNEL_NAMES)},
             "children": int(self.children),
             "iterations": int(self.iterations),
+            "flagged": int(self.flagged),
         }
 
 
 assert C.sizeof(KgmtConfig) == 168, C.sizeof(KgmtConfig)
 assert C.sizeof(KgmtResult) == 40, C.sizeof(KgmtResult)
 assert C.sizeof(KgmtAdjointParams) == 60, C.sizeof(KgmtAdjointParams)
+assert C.sizeof(KgmtProfile) == 104, C.sizeof(KgmtProfile)

@@ -41,6 +41,7 @@ struct QueryDesc
     int32_t R1, P, R2, R2words;      // regions, subregions per region, subregions, bitset words
     uint32_t cap, nChunks;           // capacity, ceil(cap / kChunk)
     uint64_t sampleOff;              // float offset of this query's [D][cap] sample blocks
+    uint64_t rowOffC;                // float offset of this query's flagged-child rows ([cap][row_words(D)])
     uint64_t nodeOff;                // element offset of per-node arrays (cap entries)
     uint32_t chunkOff;               // offset of per-chunk arrays (nChunks + 1 entries)
     uint32_t regOff;                 // offset of per-region arrays (R1 entries)
@@ -54,6 +55,13 @@ struct QueryDesc
     uint32_t rowOff;                 // first obstacle row (2 * wdim floats) of the query in the scene's source rows
     int32_t wdim;                    // workspace dimension W_DIM: 2 (planar: x y, z = 0) or 3
 };
+
+// words of a flagged-child row: parent index + D sample words, padded to
+// whole float4s
+__host__ __device__ constexpr int row_words(int D)
+{
+    return (D + 4) & ~3;
+}
 
 struct QueryState
 {
@@ -73,6 +81,7 @@ struct QueryState
     float pathCost, pathLen;
     uint32_t jobsDone;   // last-block counter of the append kernel
     uint64_t expanded;
+    uint64_t flagged;    // children flagged for the tree over all iterations (rows written)
     uint32_t finishItr;
     uint32_t pad[3];
     // The next iteration's values, written when an iteration closes (by the
@@ -80,7 +89,7 @@ struct QueryState
     // reads them, the score kernel copies them to the fields above (k_paths
     // after the last iteration), and append/expand read those.
     uint32_t itrN, treeSizeN, nChildrenN, perN, doneN, successN, finishItrN, padN;
-    uint64_t expandedN;
+    uint64_t expandedN, flaggedN;
 };
 
 __device__ __forceinline__ void commit_state(QueryState& st)
@@ -93,19 +102,20 @@ __device__ __forceinline__ void commit_state(QueryState& st)
     st.success   = st.successN;
     st.finishItr = st.finishItrN;
     st.expanded  = st.expandedN;
+    st.flagged   = st.flaggedN;
 }
 
 // Pointers of the device pools (shared by all queries of a batch).
 struct Pools
 {
     float* treeS;        // tree samples        [D][cap] per query
-    float* childS;       // unexplored samples  [D][cap] per query
+    float* childR;       // rows of this iteration's flagged children, [cap][row_words(D)] per query: the
+                         // parent index's bits, then the D sample words (zero padded); dense per 32-child segment
     int32_t* parent;     // tree parents
     float* cost;         // distance of node position to goal
     uint8_t* frontier;   // frontier flag per tree node
     int32_t* r1node;     // region of each tree node
-    int32_t* parentU;    // parent of each unexplored child
-    int32_t* r1child;    // region of each unexplored child
+    int32_t* r1child;    // region of each child
     uint8_t* fnext;      // child flagged for the next frontier
     uint8_t* react;      // old node re-activated this iteration
     uint32_t* repU;      // repeat count of each appended child, 0 otherwise
@@ -965,6 +975,16 @@ __device__ __forceinline__ uint32_t lane_id()
     return __lane_id();
 }
 
+// Orders one wave's LDS accesses across its lanes without a workgroup
+// barrier: the LDS performs a wave's operations in issue order, and the
+// fences and wave barrier keep the compiler from moving accesses across.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // v of the lane `CTRL` selects (DPP), 0 where the source lies outside the row
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp_u(uint32_t v)
@@ -980,6 +1000,21 @@ __device__ __forceinline__ uint32_t row_incl_scan(uint32_t v)
     v += dpp_u<0x114>(v);
     v += dpp_u<0x118>(v);
     return v;
+}
+
+// inclusive scan of v over each aligned group of 8 lanes: within quads
+// (quad_perm [0,0,1,2] then [0,0,0,1], lanes taking no term add 0), then
+// each quad's total (quad_perm [3,3,3,3]) added to the next quad of the group
+// (row_shr 4 into banks 1 and 3 only)
+__device__ __forceinline__ uint32_t group8_incl_scan(uint32_t v)
+{
+    const uint32_t q = lane_id() & 3u;
+    uint32_t t       = dpp_u<0x90>(v);
+    v += q >= 1u ? t : 0u;
+    t = dpp_u<0x40>(v);
+    v += q >= 2u ? t : 0u;
+    const uint32_t tq = dpp_u<0xFF>(v);
+    return v + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)tq, 0x114, 0xF, 0xA, false);
 }
 
 // inclusive scan of v over the 64 lanes of a wave: row scans, then the row

@@ -247,7 +247,7 @@ struct Engine
         HIP_TRY(hipGetDevice(&device));
         nq = n;
         if(!hq.resize(n) || !hs.resize(n)) return fail(KGMT_E_NOMEM, "pinned host allocation failed");
-        uint64_t samples = 0, nodes = 0, paths = 0;
+        uint64_t samples = 0, rows = 0, nodes = 0, paths = 0;
         uint32_t chunks = 0, regs = 0, subs = 0;
         model = cfgs[0].model;
         for(uint32_t q = 0; q < n; ++q)
@@ -257,12 +257,14 @@ struct Engine
                 QueryDesc& qd = hq[q];
                 fill_desc(qd, cfgs[q]);
                 qd.sampleOff = samples;
+                qd.rowOffC   = rows;
                 qd.nodeOff   = nodes;
                 qd.chunkOff  = chunks;
                 qd.regOff    = regs;
                 qd.subOff    = subs;
                 qd.pathOff   = (uint32_t)paths;
                 samples += (uint64_t)qd.D * qd.cap;
+                rows += (uint64_t)row_words(qd.D) * qd.cap;
                 nodes += ((uint64_t)qd.cap + kNodeAlign - 1) / kNodeAlign * kNodeAlign;
                 chunks += qd.nChunks + 1;
                 regs += (uint32_t)qd.R1;
@@ -276,10 +278,10 @@ struct Engine
             }
         int rc = 0;
         if((rc = dev_alloc(dq, n, owned)) || (rc = dev_alloc(ds, n, owned)) ||
-           (rc = dev_alloc(P.treeS, samples, owned)) || (rc = dev_alloc(P.childS, samples, owned)) ||
+           (rc = dev_alloc(P.treeS, samples, owned)) || (rc = dev_alloc(P.childR, rows, owned)) ||
            (rc = dev_alloc(P.parent, nodes, owned)) || (rc = dev_alloc(P.cost, nodes, owned)) ||
            (rc = dev_alloc(P.frontier, nodes, owned)) || (rc = dev_alloc(P.r1node, nodes, owned)) ||
-           (rc = dev_alloc(P.parentU, nodes, owned)) || (rc = dev_alloc(P.r1child, nodes, owned)) ||
+           (rc = dev_alloc(P.r1child, nodes, owned)) ||
            (rc = dev_alloc(P.fnext, nodes, owned)) || (rc = dev_alloc(P.react, nodes, owned)) ||
            (rc = dev_alloc(P.repU, nodes, owned)) || (rc = dev_alloc(P.repIdx, nodes, owned)) ||
            (rc = dev_alloc(P.partA, chunks, owned)) || (rc = dev_alloc(P.partB, chunks, owned)) ||
@@ -421,7 +423,7 @@ struct Engine
                     }
             }
         prof.iterations += iters;
-        for(uint32_t q = 0; q < nq; ++q) prof.children += hs[q].expanded;
+        for(uint32_t q = 0; q < nq; ++q) prof.children += hs[q].expanded, prof.flagged += hs[q].flagged;
         return KGMT_OK;
     }
 

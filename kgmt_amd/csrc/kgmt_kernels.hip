@@ -1,12 +1,14 @@
 // kgmt_kernels.hip — HIP kernels of the KGMT per-iteration expansion path (gfx950).
 //
-// One iteration of the planner is five launches over a table of queries
-// (a single-query plan is a table of one):
+// One iteration of the planner is five launches over a table of queries, four
+// for a single query (its first k_expand job does k_finish's work):
 //   k_propagate  persistent over tiles of children: Philox draws, per-wave
 //                sort by step count, RK4 rollout, AABB test against the
-//                obstacles staged in LDS (candidate boxes per wave), child
-//                written SoA, region counters and new subregion bits summed
-//                per tile in LDS, next-frontier flag, per-chunk flag counts.
+//                obstacles staged in LDS (candidate boxes per wave), region
+//                and next-frontier flag of every child, the row of each
+//                flagged child (dense per 32-child segment), region counters
+//                and new subregion bits summed per tile in LDS, per-chunk
+//                flag counts.
 //                (KGMT.cu:218-299, statePropagator.cu, collisionCheck.cu)
 //   k_score      one workgroup per query: region scores with a fixed pairwise
 //                summation order, bitset snapshot, scan of the flag counts.
@@ -17,7 +19,7 @@
 //   k_expand     same jobs: writes the next expansion list (segmented repeat);
 //                each job sums the chunk counts before its own.
 //                (KGMT.cu:170-181, Planner.cu:52-80)
-//   k_finish     one workgroup: closes the iteration of every query (tree
+//   k_finish     (batches) one workgroup: closes the iteration of every query (tree
 //                size, stop rules, next launch regime; KGMT.cu:84-95, 183-201,
 //                399-402) and builds the next iteration's work lists.
 // Per plan: k_build_scenes before the first iteration, k_init, and k_paths
@@ -357,10 +359,11 @@ __global__ __launch_bounds__(kBlock) void k_init(const QueryDesc* __restrict__ q
             st.pathCost = 0.0f, st.pathLen = 0.0f;
             st.jobsDone  = 0;
             st.expanded  = 0;
+            st.flagged   = 0;
             st.finishItr = 0;
             st.itrN = st.itr, st.treeSizeN = st.treeSize, st.nChildrenN = st.nChildren, st.perN = st.per;
             st.doneN = 0, st.successN = 0, st.finishItrN = 0, st.padN = 0;
-            st.expandedN = 0;
+            st.expandedN = 0, st.flaggedN = 0;
         }
     if(last_arrival(W.counter, nq, &sflag)) build_worklists<kBlock>(sts, W, 0, sbuf);
 }
@@ -387,7 +390,20 @@ struct TileLds
     uint32_t rtot[NT], rval[NT];
     int32_t wkey[NT];
     uint32_t wmask[NT];
+    // flagged children of the tile, one bit per child, by 32-child segment;
+    // two buffers alternating by tile, so a tile empties the other one
+    uint32_t fbits[2][NT / 32];
 };
+
+// empties thread t's slot of both tables and of flag buffer `fb`
+template <int NT>
+__device__ __forceinline__ void clear_tables(TileLds<NT>& L, uint32_t fb)
+{
+    const uint32_t t = threadIdx.x;
+    L.rkey[t] = -1, L.rtot[t] = 0, L.rval[t] = 0;
+    L.wkey[t] = -1, L.wmask[t] = 0;
+    if(t < NT / 32) L.fbits[fb][t] = 0;
+}
 
 // slot of `key` in an NT-entry open-addressing table of LDS keys (claimed by
 // compare-and-swap); the table holds fewer than NT other keys
@@ -405,6 +421,10 @@ __device__ __forceinline__ uint32_t tab_slot(int32_t* keys, int32_t key)
 
 // One tile of TC = NT / K children u = base .. base + TC - 1 of a query (TC
 // divides kChunk, so a tile never straddles two chunks), K lanes per child.
+// Every child writes its flag and region; a flagged child (the only ones the
+// tree append reads) also writes its row and parent, at the next free slot of
+// its 32-child segment (segment start + flagged children before it in the
+// segment), so the rows of a segment are dense. fb: this tile's flag buffer.
 // The children are drawn in tile order, then handed to the lane groups
 // longest-first (counting sort on the step count) so that each wave's groups
 // integrate about the same number of steps. The K lanes of a group integrate
@@ -414,7 +434,7 @@ __device__ __forceinline__ uint32_t tab_slot(int32_t* keys, int32_t key)
 // the order or on K.
 template <int MODEL, int NT, int K>
 __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState& st, uint32_t base,
-                                           const Pools& P, const float4* sObs, TileLds<NT>& L)
+                                           const Pools& P, const float4* sObs, TileLds<NT>& L, uint32_t fb)
 {
     constexpr int S = ModelDims<MODEL>::S, C = ModelDims<MODEL>::C, D = S + C + 1;
     constexpr uint32_t TC = NT / K;
@@ -440,26 +460,47 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
                 const int32_t k = steps - qd.cfg.steps_min;
                 bucket = g0 + (BW - 2) - (uint32_t)(k < (int32_t)BW - 2 ? k : (int32_t)BW - 2);
             }
-        L.hist[t]  = 0;
-        L.rkey[t]  = -1, L.rtot[t] = 0, L.rval[t] = 0;
-        L.wkey[t]  = -1, L.wmask[t] = 0;
-        __syncthreads();
-        uint32_t rank = 0;
-        if(t < TC) rank = atomicAdd(&L.hist[bucket], 1u);
-        __syncthreads();
-        uint32_t tot;
-        const uint32_t off = block_excl_scan<NT>(L.hist[t], L.sbuf, tot);
-        L.hist[t]          = off;
-        __syncthreads();
-        if(t < TC)
+        L.hist[t] = 0;
+        if constexpr(K == 1)
             {
+                // The wave's 64 buckets hold exactly its 64 lanes, so its
+                // offsets are a wave scan from g0, and every LDS word written
+                // here is read only by this wave: no workgroup barrier (the
+                // LDS runs one wave's operations in order; the wave barriers
+                // keep the compiler from reordering them).
+                wave_sync();
+                const uint32_t rank = atomicAdd(&L.hist[bucket], 1u);
+                wave_sync();
+                const uint32_t h = L.hist[t];
+                L.hist[t]        = g0 + wave_incl_scan(h) - h;
+                wave_sync();
                 L.slot[L.hist[bucket] + rank] = (uint16_t)t;
 #pragma unroll
                 for(int k = 0; k < C; ++k) L.ctrl[k][t] = uc[k];
                 L.steps[t]  = steps;
                 L.ufront[t] = uf;
+                wave_sync();
             }
-        __syncthreads();
+        else
+            {
+                __syncthreads();
+                uint32_t rank = 0;
+                if(t < TC) rank = atomicAdd(&L.hist[bucket], 1u);
+                __syncthreads();
+                uint32_t tot;
+                const uint32_t off = block_excl_scan<NT>(L.hist[t], L.sbuf, tot);
+                L.hist[t]          = off;
+                __syncthreads();
+                if(t < TC)
+                    {
+                        L.slot[L.hist[bucket] + rank] = (uint16_t)t;
+#pragma unroll
+                        for(int k = 0; k < C; ++k) L.ctrl[k][t] = uc[k];
+                        L.steps[t]  = steps;
+                        L.ufront[t] = uf;
+                    }
+                __syncthreads();
+            }
     }
     const uint32_t i    = L.slot[t / K];
     const uint32_t u    = base + i;
@@ -510,16 +551,10 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
                             setBit          = !seen;
                             r2bit           = (uint32_t)r2;
                         }
-                    float* U = P.childS + qd.sampleOff;
-#pragma unroll
-                    for(int d = 0; d < S; ++d) U[(size_t)d * cap + u] = s[d];
-#pragma unroll
-                    for(int k = 0; k < C; ++k) U[(size_t)(S + k) * cap + u] = uc[k];
-                    U[(size_t)(D - 1) * cap + u] = qd.cfg.dt * (float)steps;
-                    P.parentU[nodeO + u]          = (int32_t)p;
                     if(u % per == 0) P.frontier[nodeO + p] = 0;
                     P.r1child[nodeO + u] = r1;
                     P.fnext[nodeO + u]   = fn ? 1 : 0;
+                    if(fn) atomicOr(&L.fbits[fb][i >> 5], 1u << (i & 31u));
                 }
         }
     // region counters and new subregion bits: summed per tile in LDS, then
@@ -539,6 +574,25 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
     const uint64_t fm = __ballot(fn);
     if(lane_id() == 0 && fm) atomicAdd(&P.partA[qd.chunkOff + base / kChunk], (uint32_t)__popcll(fm));
     __syncthreads();
+    if(fn)
+        {
+            // row slot: segment start + flagged children before it; the row
+            // (parent bits, sample, zero padding) in whole float4 stores
+            constexpr int RW = row_words(D);
+            const uint32_t v = base + (i & ~31u) + (uint32_t)__popc(L.fbits[fb][i >> 5] & ((1u << (i & 31u)) - 1u));
+            float w[RW];
+            w[0] = __int_as_float((int32_t)p);
+#pragma unroll
+            for(int d = 0; d < S; ++d) w[1 + d] = s[d];
+#pragma unroll
+            for(int k = 0; k < C; ++k) w[1 + S + k] = uc[k];
+            w[D] = qd.cfg.dt * (float)steps;
+#pragma unroll
+            for(int e = D + 1; e < RW; ++e) w[e] = 0.0f;
+            float4* row = reinterpret_cast<float4*>(P.childR + qd.rowOffC + (size_t)v * RW);
+#pragma unroll
+            for(int j = 0; j < RW / 4; ++j) row[j] = make_float4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
+        }
     {
         const int32_t rk = L.rkey[t];
         if(rk >= 0)
@@ -548,6 +602,9 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
             }
         const int32_t wk = L.wkey[t];
         if(wk >= 0) atomicOr(&P.actNext[qd.subOff + (uint32_t)wk], L.wmask[t]);
+        // empty for the next tile, which uses the other flag buffer (this
+        // tile's is emptied by the next one, after the barrier below)
+        clear_tables(L, fb ^ 1u);
     }
     __syncthreads();  // L is reused by the next tile
 }
@@ -564,6 +621,12 @@ __global__ __launch_bounds__(NT, 4) void k_propagate(const QueryDesc* __restrict
     const uint32_t* tileList = W.tiles(parity);
     const uint32_t nTiles    = tileList[nq];
     uint32_t staged          = 0xffffffffu;
+    uint32_t fb              = 0;  // flag buffer of the next tile
+    // the tables and both flag buffers start empty (each tile empties them
+    // again after its flush; the first tile's staging barrier comes before
+    // any insert)
+    clear_tables(L, 0u);
+    clear_tables(L, 1u);
     for(uint32_t t = blockIdx.x; t < nTiles; t += gridDim.x)
         {
             const uint32_t q    = locate(tileList, nq, t);
@@ -578,25 +641,26 @@ __global__ __launch_bounds__(NT, 4) void k_propagate(const QueryDesc* __restrict
                 }
             const uint32_t base = (t - tileList[q]) * (NT / K);
             if constexpr(MODEL >= 0)
-                child_tile<MODEL, NT, K>(qd, sts[q], base, P, sObs, L);
+                child_tile<MODEL, NT, K>(qd, sts[q], base, P, sObs, L, fb);
             else
                 {
                     switch(qd.cfg.model)
                         {
                             case KGMT_MODEL_DOUBLE_INTEGRATOR:
-                                child_tile<KGMT_MODEL_DOUBLE_INTEGRATOR, NT, K>(qd, sts[q], base, P, sObs, L);
+                                child_tile<KGMT_MODEL_DOUBLE_INTEGRATOR, NT, K>(qd, sts[q], base, P, sObs, L, fb);
                                 break;
                             case KGMT_MODEL_DUBINS_AIRPLANE:
-                                child_tile<KGMT_MODEL_DUBINS_AIRPLANE, NT, K>(qd, sts[q], base, P, sObs, L);
+                                child_tile<KGMT_MODEL_DUBINS_AIRPLANE, NT, K>(qd, sts[q], base, P, sObs, L, fb);
                                 break;
                             case KGMT_MODEL_DUBINS_5D:
-                                child_tile<KGMT_MODEL_DUBINS_5D, NT, K>(qd, sts[q], base, P, sObs, L);
+                                child_tile<KGMT_MODEL_DUBINS_5D, NT, K>(qd, sts[q], base, P, sObs, L, fb);
                                 break;
                             // (a batch never mixes planar and 3D queries, so
                             // a mixed batch holds no unicycle)
-                            default: child_tile<KGMT_MODEL_QUADCOPTER, NT, K>(qd, sts[q], base, P, sObs, L); break;
+                            default: child_tile<KGMT_MODEL_QUADCOPTER, NT, K>(qd, sts[q], base, P, sObs, L, fb); break;
                         }
                 }
+            fb ^= 1u;
         }
 }
 
@@ -667,7 +731,6 @@ __device__ void append_children(const QueryDesc& qd, QueryState& st, uint32_t c,
 {
     const uint32_t cap = qd.cap, nC = st.nChildren, tree = st.treeSize, next = st.next;
     const size_t nodeO     = qd.nodeOff;
-    const float* __restrict__ U = P.childS + qd.sampleOff;
     float* __restrict__ T       = P.treeS + qd.sampleOff;
     const int D            = qd.D;
     const uint32_t u0      = c * kChunk + kPer * threadIdx.x;
@@ -675,31 +738,40 @@ __device__ void append_children(const QueryDesc& qd, QueryState& st, uint32_t c,
     uint32_t fl            = 0;
     int32_t r1[kPer]       = {0, 0, 0, 0};
     int32_t vcount[kPer]   = {0, 0, 0, 0};
-    float v[kPer][KGMT_MAX_SAMPLE];
-    int32_t par[kPer];
+    constexpr int kRowMax = row_words(KGMT_MAX_SAMPLE);
+    const int RW          = row_words(D);
+    float v[kPer][kRowMax];  // rows as stored: parent bits, then the sample
     if(u0 < nC)
         {
-            // flags and regions together, then (before the rank scan) the
-            // regions' valid counts for the repeat count and the rows of the
-            // flagged children, which are appended unless the tree fills up
+            // flags and regions together
             const uint32_t w = *reinterpret_cast<const uint32_t*>(P.fnext + nodeO + u0);
             const int4 rr    = *reinterpret_cast<const int4*>(P.r1child + nodeO + u0);
             r1[0] = rr.x, r1[1] = rr.y, r1[2] = rr.z, r1[3] = rr.w;
 #pragma unroll
             for(uint32_t k = 0; k < kPer; ++k)
                 if(u0 + k < nC && ((w >> (8 * k)) & 0xffu)) fl |= 1u << k;
-#pragma unroll
-            for(uint32_t k = 0; k < kPer; ++k)
-                if((fl >> k) & 1u)
-                    {
-                        const uint32_t u = u0 + k;
-                        vcount[k]        = P.validc[qd.regOff + r1[k]];
-#pragma unroll
-                        for(int d = 0; d < KGMT_MAX_SAMPLE; ++d)
-                            if(d < D) v[k][d] = U[(size_t)d * cap + u];
-                        par[k] = P.parentU[nodeO + u];
-                    }
         }
+    // Before the rank scan: the regions' valid counts for the repeat count
+    // and the rows of the flagged children (appended unless the tree fills
+    // up). A flagged child's row sits at its 32-child segment's start plus
+    // the flagged children before it in the segment (eight threads of four).
+    const uint32_t nf    = (uint32_t)__popc(fl);
+    uint32_t slot        = (u0 & ~31u) + group8_incl_scan(nf) - nf;
+#pragma unroll
+    for(uint32_t k = 0; k < kPer; ++k)
+        if((fl >> k) & 1u)
+            {
+                vcount[k]         = P.validc[qd.regOff + r1[k]];
+                const float4* row = reinterpret_cast<const float4*>(P.childR + qd.rowOffC + (size_t)slot * RW);
+#pragma unroll
+                for(int j = 0; j < kRowMax / 4; ++j)
+                    if(4 * j < RW)
+                        {
+                            const float4 x = row[j];
+                            v[k][4 * j] = x.x, v[k][4 * j + 1] = x.y, v[k][4 * j + 2] = x.z, v[k][4 * j + 3] = x.w;
+                        }
+                ++slot;
+            }
     uint32_t tot;
     uint32_t rank = chunkA + block_excl_scan<kBlock>((uint32_t)__popc(fl), sbuf, tot);
     // appended slots: u0 + k with flag k set and rank < next
@@ -725,9 +797,9 @@ __device__ void append_children(const QueryDesc& qd, QueryState& st, uint32_t c,
                     {
 #pragma unroll
                         for(int d = 0; d < KGMT_MAX_SAMPLE; ++d)
-                            if(d < D) T[(size_t)d * cap + t] = v[k][d];
-                        P.parent[nodeO + t]   = par[k];
-                        const float dg        = goal_dist(qd, v[k][0], v[k][1], v[k][2]);
+                            if(d < D) T[(size_t)d * cap + t] = v[k][1 + d];
+                        P.parent[nodeO + t]   = __float_as_int(v[k][0]);
+                        const float dg        = goal_dist(qd, v[k][1], v[k][2], v[k][3]);
                         P.cost[nodeO + t]     = dg;
                         P.frontier[nodeO + t] = 1;
                         P.r1node[nodeO + t]   = r1[k];
@@ -834,6 +906,7 @@ __device__ void finish_query(const QueryDesc& qd, QueryState& st, uint32_t R)
         }
     st.treeSizeN  = tree;
     st.expandedN  = st.expanded + st.nChildren;
+    st.flaggedN   = st.flagged + st.nAcc;
     st.itrN       = itr;
     st.perN       = per;
     st.nChildrenN = nc;

@@ -72,6 +72,31 @@ def test_full_tree_ends_the_plan(orc):
     _same_plan(g, res, ref)
 
 
+def test_profile_counts_children_and_flags(orc):
+    """Profiling leaves the plan unchanged and counts the children and the
+    flagged children (every flagged child is appended while the tree has
+    room, so they are the tree's nodes after the root; a full tree appends
+    fewer than it flags)."""
+    for name, seed, kw, full in [("unicycle2d", 0, dict(), False),
+                                 ("dubins5d", 1, dict(capacity=3000, max_iter=60), True)]:
+        q = W.make_query(name, seed, **kw)
+        obs = np.array([[0.8, 0.8, 0.8, 1.0, 1.0, 1.0]], np.float32) if full else q.obstacles
+        ref = orc.plan(q.cfg, q.start, q.goal, obs)
+        g = KGMT(q.cfg)
+        for level in (1, 2):
+            g.set_profiling(level)
+            res = g.plan(q.start, q.goal, obs, len(obs))
+            p = g.profile()
+            g.set_profiling(0)
+            _same_plan(g, res, ref)
+            assert p["children"] == res.expanded
+            if full:
+                assert res.tree_size == q.cfg.capacity and p["flagged"] >= res.tree_size - 1
+            else:
+                assert res.tree_size < q.cfg.capacity and p["flagged"] == res.tree_size - 1
+            assert p["kernels"]["propagate"]["launches"] >= res.iterations
+
+
 def test_plan_device_obstacles(orc):
     torch = pytest.importorskip("torch")
     q = W.make_query("di6d", 4, capacity=1 << 14)
