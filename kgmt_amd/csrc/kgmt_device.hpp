@@ -379,24 +379,42 @@ __device__ __forceinline__ bool separated_gap(float mnx, float mny, float mnz, f
     return fmaxf(fmaxf(gx, gy), gz) >= 0.0f;
 }
 
-// A wave's candidate boxes, (lo, hi) float4 pairs in LDS. Positions tested
-// here are finite (the dry run or the Dubins bound was finite), and a
-// candidate overlaps a finite box, so its low faces are < +inf and its high
-// faces > -inf: separated_gap gives separated()'s verdict.
-struct CandCheck
+// A lane's own candidate boxes: the entries of the wave's candidate list
+// ((lo, hi) float4 pairs in LDS) that overlap the lane's edge box. Every
+// segment the lane tests lies inside its edge box, so a box outside it can
+// meet none of them, and testing only these gives the verdicts of testing the
+// whole list. The first R are held in registers for the whole edge (the loop
+// runs to the most any lane of the wave holds; a lane tests only its own),
+// the rest are read from LDS. Positions tested here are finite (the dry run
+// or the Dubins bound was finite), and a candidate overlaps a finite box, so
+// its low faces are < +inf and its high faces > -inf: separated_gap gives
+// separated()'s verdict.
+__device__ __forceinline__ float wave_maxf(float v);
+
+struct LaneCandCheck
 {
-    static constexpr int R = 4;  // the first R boxes are held in registers for the whole edge
+    static constexpr int R = 4;
     const float4* c;
-    int n;
+    uint64_t rest;  // own candidates past the first R (bit = list index)
+    int mine;       // own candidates in registers
+    int top;        // most own candidates in registers over the wave
+    bool anyRest;   // some lane of the wave has candidates past R
     float4 r[2 * R];
-    __device__ __forceinline__ CandCheck(const float4* list, int count) : c(list), n(count)
+    __device__ __forceinline__ LaneCandCheck(const float4* list, uint64_t own) : c(list)
     {
+        mine = 0;
 #pragma unroll
         for(int k = 0; k < R; ++k)
             {
-                r[2 * k]     = k < n ? c[2 * k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                r[2 * k + 1] = k < n ? c[2 * k + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                const int idx = own ? __ffsll((unsigned long long)own) - 1 : 0;
+                r[2 * k]      = own ? c[2 * idx] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                r[2 * k + 1]  = own ? c[2 * idx + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                mine += own ? 1 : 0;
+                own &= own - 1;
             }
+        rest    = own;
+        top     = (int)wave_maxf((float)mine);
+        anyRest = __any(rest != 0);
     }
     __device__ __forceinline__ bool operator()(float ax, float ay, float az, float bx, float by, float bz) const
     {
@@ -407,10 +425,15 @@ struct CandCheck
 #pragma unroll
         for(int k = 0; k < R; ++k)
             {
-                if(k >= n) break;  // wave-uniform
-                hit |= !separated_gap(mnx, mny, mnz, mxx, mxy, mxz, r[2 * k], r[2 * k + 1]);
+                if(k >= top) break;  // wave-uniform
+                hit |= (k < mine) & !separated_gap(mnx, mny, mnz, mxx, mxy, mxz, r[2 * k], r[2 * k + 1]);
             }
-        for(int m = R; m < n; ++m) hit |= !separated_gap(mnx, mny, mnz, mxx, mxy, mxz, c[2 * m], c[2 * m + 1]);
+        if(anyRest)
+            for(uint64_t m = rest; m; m &= m - 1)
+                {
+                    const int idx = __ffsll((unsigned long long)m) - 1;
+                    hit |= !separated_gap(mnx, mny, mnz, mxx, mxy, mxz, c[2 * idx], c[2 * idx + 1]);
+                }
         return !hit;
     }
 };
@@ -851,7 +874,8 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
             dry.hi[0] = dry.hi[1] = dry.hi[2] = -INFINITY;
             dry.finite = true;
         }
-    int n = 0;
+    int n        = 0;
+    uint64_t own = 0;  // list entries overlapping this lane's box
     if(!__any(!dry.finite))
         {
             // candidates = boxes that some lane's box overlaps, in box order.
@@ -879,7 +903,11 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
                                                       lo, hi);
                             if(__any(c))
                                 {
-                                    if(n < kCand) cand[2 * n] = lo, cand[2 * n + 1] = hi;  // same value in every lane
+                                    if(n < kCand)
+                                        {
+                                            cand[2 * n] = lo, cand[2 * n + 1] = hi;  // same value in every lane
+                                            if(c) own |= 1ull << n;
+                                        }
                                     ++n;
                                 }
                         }
@@ -902,7 +930,7 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    CandCheck chk(cand, n);
+    LaneCandCheck chk(cand, own);
     return propagate_with<MODEL>(qd, s, u, steps, chk);
 }
 
