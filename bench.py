@@ -31,6 +31,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 VALU_PEAK_TFLOPS = 157.3
+# FP32 vector issue without FMA or packed math: 256 CUs x 4 SIMDs x 16 lanes
+# x 2.4 GHz (the 157.3 TFLOP/s peak counts FMA as 2 and packed FP32 as 2)
+VALU_PEAK_TLANEOPS = VALU_PEAK_TFLOPS / 4
 
 WORKLOAD_DESC = {
     "dubins5d": "5D Dubins airplane, plain single-query planner, 100 boxes sides U[0.02,0.1], 2^18 nodes, grid 16^3/2^3, dt 0.02, 5-25 steps/edge",
@@ -290,6 +293,14 @@ def main(argv=None):
     per_launch = prof["children"] / max(pk["launches"], 1)
     tr = measured_traffic(a.workload)
     traffic = round(tr["hbm_bytes_per_child"] * per_launch) if tr else None
+    valu = None
+    if tr and "valu_insts_per_child" in tr:
+        # issue rate of the same launches: VALU wave instructions per child
+        # (rocprofv3 SQ_INSTS_VALU) x 64 lanes x children per launch
+        v_ach = tr["valu_insts_per_child"] * 64 * per_launch / (avg_ms * 1e-3) / 1e12
+        valu = {"achieved": round(v_ach, 2), "peak": round(VALU_PEAK_TLANEOPS, 2), "unit": "T lane-ops/s",
+                "frac": round(v_ach / VALU_PEAK_TLANEOPS, 4),
+                "source": "profiles/k_propagate_traffic.json valu_insts_per_child (SQ_INSTS_VALU)"}
     roofline = {
         "kernel": "k_propagate",
         "bound": "hbm",
@@ -305,7 +316,8 @@ def main(argv=None):
         "children_per_launch": round(per_launch, 1),
         "avg_launch_us": round(avg_ms * 1e3, 2),
         "kernel_avg_us": breakdown,
-        "note": "avg_launch_us from HIP events around every propagate launch in a second pass over the timed queries (identical plans); propagate+collide is latency-bound at one query (RK4 steps + box tests per child, ~1 working wave per SIMD), not HBM-bound; HBM fraction reported as required",
+        "valu": valu,
+        "note": "avg_launch_us from HIP events around every propagate launch in a second pass over the timed queries (identical plans); achieved = algorithmic bytes (per child: list entry, parent state, region, flag; per flagged child: row and parent index) / avg launch; propagate+collide is bound by the VALU issue of its RK4 steps and box tests (~2 working waves per SIMD; see valu), not by HBM; HBM fraction reported as required",
     }
 
     cpu = None
