@@ -28,7 +28,8 @@ constexpr uint32_t kMaxObs  = 1500;  // obstacles a query may carry
 constexpr uint32_t kGroup   = 8;     // boxes per bounding group of a scene
 constexpr uint32_t kGroupBatch = 4;  // groups tested together (scenes pad to a multiple)
 // LDS of one scene: G groups + 8 G boxes, G = ceil(n / 32) * 4, as (lo, hi)
-// float4 pairs: at most 1692 pairs = 54.1 KB (+ 8.5 KB of tile state)
+// float4 pairs: at most 1692 pairs = 54.1 KB (+ 58.5 KB of tile state and
+// candidate lists for a 512-thread propagate tile: 113 KB of gfx950's 160 KB)
 constexpr int32_t kNoGoal   = 0x7fffffff;
 
 // -------------------------------------------------------------------------
@@ -395,26 +396,32 @@ struct LaneCandCheck
 {
     static constexpr int R = 4;
     const float4* c;
-    uint64_t rest;  // own candidates past the first R (bit = list index)
+    uint64_t restLo, restHi;  // own candidates past the first R (bit = list index, minus 64 for Hi)
     int mine;       // own candidates in registers
     int top;        // most own candidates in registers over the wave
     bool anyRest;   // some lane of the wave has candidates past R
     float4 r[2 * R];
-    __device__ __forceinline__ LaneCandCheck(const float4* list, uint64_t own) : c(list)
+    __device__ __forceinline__ LaneCandCheck(const float4* list, uint64_t ownLo, uint64_t ownHi) : c(list)
     {
         mine = 0;
 #pragma unroll
         for(int k = 0; k < R; ++k)
             {
-                const int idx = own ? __ffsll((unsigned long long)own) - 1 : 0;
-                r[2 * k]      = own ? c[2 * idx] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                r[2 * k + 1]  = own ? c[2 * idx + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                mine += own ? 1 : 0;
-                own &= own - 1;
+                const bool lo    = ownLo != 0;
+                const uint64_t w = lo ? ownLo : ownHi;
+                const int idx    = w ? __ffsll((unsigned long long)w) - 1 + (lo ? 0 : 64) : 0;
+                r[2 * k]         = w ? c[2 * idx] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                r[2 * k + 1]     = w ? c[2 * idx + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                mine += w ? 1 : 0;
+                if(lo)
+                    ownLo &= ownLo - 1;
+                else
+                    ownHi &= ownHi - 1;
             }
-        rest    = own;
+        restLo  = ownLo;
+        restHi  = ownHi;
         top     = (int)wave_maxf((float)mine);
-        anyRest = __any(rest != 0);
+        anyRest = __any((restLo | restHi) != 0);
     }
     __device__ __forceinline__ bool operator()(float ax, float ay, float az, float bx, float by, float bz) const
     {
@@ -429,11 +436,18 @@ struct LaneCandCheck
                 hit |= (k < mine) & !separated_gap(mnx, mny, mnz, mxx, mxy, mxz, r[2 * k], r[2 * k + 1]);
             }
         if(anyRest)
-            for(uint64_t m = rest; m; m &= m - 1)
-                {
-                    const int idx = __ffsll((unsigned long long)m) - 1;
-                    hit |= !separated_gap(mnx, mny, mnz, mxx, mxy, mxz, c[2 * idx], c[2 * idx + 1]);
-                }
+            {
+                for(uint64_t m = restLo; m; m &= m - 1)
+                    {
+                        const int idx = __ffsll((unsigned long long)m) - 1;
+                        hit |= !separated_gap(mnx, mny, mnz, mxx, mxy, mxz, c[2 * idx], c[2 * idx + 1]);
+                    }
+                for(uint64_t m = restHi; m; m &= m - 1)
+                    {
+                        const int idx = __ffsll((unsigned long long)m) + 63;
+                        hit |= !separated_gap(mnx, mny, mnz, mxx, mxy, mxz, c[2 * idx], c[2 * idx + 1]);
+                    }
+            }
         return !hit;
     }
 };
@@ -726,7 +740,7 @@ __device__ __forceinline__ bool propagate(const QueryDesc& qd, float* s, const f
     return propagate_with<MODEL>(qd, s, u, steps, chk);
 }
 
-constexpr int kCand = 64;  // candidate boxes per wave of the two-phase check
+constexpr int kCand = 128;  // candidate boxes per wave of the two-phase check (two 64-bit lane masks)
 
 // Two-phase edge check with the same verdict and final state as propagate():
 // 1. integrate without boxes, recording the box T spanned by every segment the
@@ -875,7 +889,7 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
             dry.finite = true;
         }
     int n        = 0;
-    uint64_t own = 0;  // list entries overlapping this lane's box
+    uint64_t ownLo = 0, ownHi = 0;  // list entries 0..63 / 64..127 overlapping this lane's box
     if(!__any(!dry.finite))
         {
             // candidates = boxes that some lane's box overlaps, in box order.
@@ -906,7 +920,13 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
                                     if(n < kCand)
                                         {
                                             cand[2 * n] = lo, cand[2 * n + 1] = hi;  // same value in every lane
-                                            if(c) own |= 1ull << n;
+                                            if(c)
+                                                {
+                                                    if(n < 64)
+                                                        ownLo |= 1ull << n;
+                                                    else
+                                                        ownHi |= 1ull << (n - 64);
+                                                }
                                         }
                                     ++n;
                                 }
@@ -930,7 +950,7 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    LaneCandCheck chk(cand, own);
+    LaneCandCheck chk(cand, ownLo, ownHi);
     return propagate_with<MODEL>(qd, s, u, steps, chk);
 }
 

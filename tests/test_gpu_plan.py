@@ -72,6 +72,19 @@ def test_full_tree_ends_the_plan(orc):
     _same_plan(g, res, ref)
 
 
+@pytest.mark.parametrize("block", [512, 256])
+def test_max_obstacles_plan_matches_oracle(block, orc):
+    """1500 boxes, the most a query may carry: the scene and the per-wave
+    candidate lists fill the most LDS a propagate workgroup uses."""
+    q = W.make_query("dubins5d", 3, capacity=4000, max_iter=10)
+    obs = W.random_boxes(np.random.default_rng(5), 1500, 0.005, 0.03)
+    ref = orc.plan(q.cfg, q.start, q.goal, obs)
+    g = KGMT(q.cfg)
+    g.set_launch(propagate_block=block, propagate_grid=256, chunk_grid=128)
+    res = g.plan(q.start, q.goal, obs, len(obs))
+    _same_plan(g, res, ref)
+
+
 def test_profile_counts_children_and_flags(orc):
     """Profiling leaves the plan unchanged and counts the children and the
     flagged children (every flagged child is appended while the tree has
