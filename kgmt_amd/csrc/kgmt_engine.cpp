@@ -205,6 +205,7 @@ struct Engine
     float* rowsBuf   = nullptr;
     const float* sceneRows = nullptr;  // device rows the scenes of the next run are built from
     bool sceneBuild  = false;
+    bool counterDirty = false;  // W.counter may be non-zero (a plan did not complete)
     uint32_t maxPairs = 0;   // largest per-query scene (pairs) of the current plan
     int model       = -1;    // common model or -1
     uint32_t maxIter = 0;
@@ -453,10 +454,12 @@ struct Engine
     {
         const auto t0 = std::chrono::steady_clock::now();
         HIP_TRY(hipMemcpyAsync(dq, hq.data(), sizeof(QueryDesc) * nq, hipMemcpyHostToDevice, stream));
-        if(sceneBuild) HIP_TRY(launch_build_scenes(dq, sceneRows, P.obs, nq, stream));
-        HIP_TRY(hipMemsetAsync(W.counter, 0, sizeof(uint32_t), stream));
+        // k_init's last workgroup leaves the arrival counter at zero; after a
+        // plan that did not complete it is cleared here
+        if(counterDirty) HIP_TRY(hipMemsetAsync(W.counter, 0, sizeof(uint32_t), stream));
+        counterDirty = true;
         __atomic_store_n(pinned, 0u, __ATOMIC_RELEASE);
-        HIP_TRY(launch_init(dq, ds, P, W, stream));
+        HIP_TRY(launch_init(dq, ds, P, W, sceneBuild ? sceneRows : nullptr, stream));
         if(useGraph && (!graphExec || graphObs != maxPairs || graphLaunch.propGrid != L.propGrid ||
                         graphLaunch.chunkGrid != L.chunkGrid || graphLaunch.propBlock != L.propBlock ||
                         graphLaunch.lanes != L.lanes))
@@ -502,7 +505,8 @@ struct Engine
         HIP_TRY(launch_paths(dq, ds, P, nq, stream));
         HIP_TRY(hipMemcpyAsync(hs.data(), ds, sizeof(QueryState) * nq, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
-        itersRun = std::min(it, maxIter);
+        counterDirty = false;
+        itersRun     = std::min(it, maxIter);
         if(prof) return collect_profile(it);
         return KGMT_OK;
     }

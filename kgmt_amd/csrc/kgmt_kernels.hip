@@ -22,7 +22,8 @@
 //   k_finish     (batches) one workgroup: closes the iteration of every query (tree
 //                size, stop rules, next launch regime; KGMT.cu:84-95, 183-201,
 //                399-402) and builds the next iteration's work lists.
-// Per plan: k_build_scenes before the first iteration, k_init, and k_paths
+// Per plan: k_init (scene build and first iteration's state) before the first
+// iteration, and k_paths
 // (goal path walk, KGMT.cu:342-361) after the last.
 // All sizes stay on the device: the host only polls a completion word that
 // the kernels write into host memory.
@@ -303,7 +304,11 @@ __device__ __forceinline__ bool last_arrival(uint32_t* counter, uint32_t expecte
     if(threadIdx.x == 0)
         {
             const uint32_t prev = atomicAdd(counter, 1u);
-            *sflag              = prev == expected - 1 ? 1u : 0u;
+            const bool last     = prev == expected - 1;
+            // every workgroup has arrived: the counter starts at zero for the
+            // next launch
+            if(last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            *sflag = last ? 1u : 0u;
         }
     __syncthreads();
     const bool last = *sflag != 0;
@@ -312,16 +317,11 @@ __device__ __forceinline__ bool last_arrival(uint32_t* counter, uint32_t expecte
 }
 
 // ---------------------------------------------------------------------------
-// init: one workgroup per query (KGMT.cu:50-81)
+// init (KGMT.cu:50-81): per-query part of k_init, defined after the scene build
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_init(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
-                                                 Pools P, WorkLists W, uint32_t nq)
+// root, cleared counters and the first iteration's state of one query
+__device__ void init_query(const QueryDesc& qd, QueryState& st, const Pools& P)
 {
-    __shared__ uint32_t sbuf[kBlock / 64 + 1];
-    __shared__ uint32_t sflag;
-    const uint32_t q    = blockIdx.x;
-    const QueryDesc& qd = qds[q];
-    QueryState& st      = sts[q];
     for(int r = threadIdx.x; r < qd.R1; r += kBlock)
         {
             P.total[qd.regOff + r]  = 0;
@@ -365,7 +365,6 @@ __global__ __launch_bounds__(kBlock) void k_init(const QueryDesc* __restrict__ q
             st.doneN = 0, st.successN = 0, st.finishItrN = 0, st.padN = 0;
             st.expandedN = 0, st.flaggedN = 0;
         }
-    if(last_arrival(W.counter, nq, &sflag)) build_worklists<kBlock>(sts, W, 0, sbuf);
 }
 
 // ---------------------------------------------------------------------------
@@ -1222,16 +1221,29 @@ __device__ void build_scene_wg(const float* __restrict__ rows, uint32_t n, uint3
         }
 }
 
-__global__ __launch_bounds__(kBlock) void k_build_scenes(const QueryDesc* __restrict__ qds,
-                                                         const float* __restrict__ rows, float4* __restrict__ pool)
+// ---------------------------------------------------------------------------
+// init: one workgroup per query (KGMT.cu:50-81): the query's scene from the
+// caller's obstacle rows (rows null: no obstacles), then the root, cleared
+// counters and the first iteration's state; the last workgroup to arrive
+// builds the first work lists.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_init(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
+                                                 Pools P, WorkLists W, uint32_t nq, const float* __restrict__ rows)
 {
     __shared__ SceneLds L;
-    const QueryDesc& qd = qds[blockIdx.x];
-    if(qd.nObs == 0) return;
-    if(qd.wdim == 2)
-        build_scene_wg<2>(rows + 4 * (size_t)qd.rowOff, qd.nObs, qd.nGrp, pool + 2 * (size_t)qd.obsOff, L);
-    else
-        build_scene_wg<3>(rows + 6 * (size_t)qd.rowOff, qd.nObs, qd.nGrp, pool + 2 * (size_t)qd.obsOff, L);
+    __shared__ uint32_t sbuf[kBlock / 64 + 1];
+    __shared__ uint32_t sflag;
+    const uint32_t q    = blockIdx.x;
+    const QueryDesc& qd = qds[q];
+    if(rows && qd.nObs)
+        {
+            if(qd.wdim == 2)
+                build_scene_wg<2>(rows + 4 * (size_t)qd.rowOff, qd.nObs, qd.nGrp, P.obs + 2 * (size_t)qd.obsOff, L);
+            else
+                build_scene_wg<3>(rows + 6 * (size_t)qd.rowOff, qd.nObs, qd.nGrp, P.obs + 2 * (size_t)qd.obsOff, L);
+        }
+    init_query(qd, sts[q], P);
+    if(last_arrival(W.counter, nq, &sflag)) build_worklists<kBlock>(sts, W, 0, sbuf);
 }
 
 template <int W>
@@ -1429,9 +1441,10 @@ __global__ __launch_bounds__(kBlock) void k_stage_repeat(const uint32_t* __restr
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_init(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, hipStream_t s)
+hipError_t launch_init(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, const float* rows,
+                       hipStream_t s)
 {
-    hipLaunchKernelGGL(k_init, dim3(W.nq), dim3(kBlock), 0, s, qds, sts, P, W, W.nq);
+    hipLaunchKernelGGL(k_init, dim3(W.nq), dim3(kBlock), 0, s, qds, sts, P, W, W.nq, rows);
     return hipGetLastError();
 }
 
@@ -1496,13 +1509,6 @@ hipError_t launch_append(const QueryDesc* qds, QueryState* sts, const Pools& P, 
                          uint32_t parity, const Launch& L, hipStream_t s)
 {
     hipLaunchKernelGGL(k_append, dim3(L.chunkGrid), dim3(kBlock), 0, s, qds, sts, P, W, parity);
-    return hipGetLastError();
-}
-
-hipError_t launch_build_scenes(const QueryDesc* qds, const float* rows, float4* pool, uint32_t nq, hipStream_t s)
-{
-    if(nq == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_build_scenes, dim3(nq), dim3(kBlock), 0, s, qds, rows, pool);
     return hipGetLastError();
 }
 

@@ -24,7 +24,6 @@ struct Launch
 // boxes with nGrp = scene_groups(n) groups.
 uint32_t scene_groups(uint32_t n);
 uint32_t scene_pairs(uint32_t n);
-hipError_t launch_build_scenes(const QueryDesc* qds, const float* rows, float4* pool, uint32_t nq, hipStream_t s);
 // rows of 2 * wdim floats (wdim 2: planar boxes, z range [-1, 1])
 hipError_t launch_build_scene(const float* rows, uint32_t n, uint32_t nGrp, float4* out, int wdim, hipStream_t s);
 
@@ -33,7 +32,9 @@ bool launch_supported(uint32_t block, uint32_t lanes);
 
 // planner iteration kernels; `model` = KGMT_MODEL_* when every query shares
 // it, -1 otherwise; `parity` = (iteration - 1) & 1 selects the work lists
-hipError_t launch_init(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, hipStream_t s);
+// scenes of the queries built from `rows` first (null: no obstacles)
+hipError_t launch_init(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, const float* rows,
+                       hipStream_t s);
 hipError_t launch_propagate(int model, const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
                             uint32_t parity, uint32_t maxPairs, const Launch& L, hipStream_t s);
 hipError_t launch_score(const QueryDesc* qds, QueryState* sts, const Pools& P, uint32_t nq, hipStream_t s);
