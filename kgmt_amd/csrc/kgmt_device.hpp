@@ -57,11 +57,11 @@ struct QueryDesc
     int32_t wdim;                    // workspace dimension W_DIM: 2 (planar: x y, z = 0) or 3
 };
 
-// words of a flagged-child row: parent index + D sample words, padded to
-// whole float4s
+// words of a flagged-child row: parent and grandparent indices + D sample
+// words, padded to whole float4s
 __host__ __device__ constexpr int row_words(int D)
 {
-    return (D + 4) & ~3;
+    return (D + 5) & ~3;
 }
 
 struct QueryState
@@ -111,8 +111,10 @@ struct Pools
 {
     float* treeS;        // tree samples        [D][cap] per query
     float* childR;       // rows of this iteration's flagged children, [cap][row_words(D)] per query: the
-                         // parent index's bits, then the D sample words (zero padded); dense per 32-child segment
+                         // parent's and grandparent's index bits, then the D sample words (zero padded);
+                         // dense per 32-child segment
     int32_t* parent;     // tree parents
+    int32_t* grand;      // tree grandparents (-1 for the root and its children)
     float* cost;         // distance of node position to goal
     uint8_t* frontier;   // frontier flag per tree node
     int32_t* r1node;     // region of each tree node

@@ -280,7 +280,7 @@ struct Engine
         int rc = 0;
         if((rc = dev_alloc(dq, n, owned)) || (rc = dev_alloc(ds, n, owned)) ||
            (rc = dev_alloc(P.treeS, samples, owned)) || (rc = dev_alloc(P.childR, rows, owned)) ||
-           (rc = dev_alloc(P.parent, nodes, owned)) || (rc = dev_alloc(P.cost, nodes, owned)) ||
+           (rc = dev_alloc(P.parent, nodes, owned)) || (rc = dev_alloc(P.grand, nodes, owned)) || (rc = dev_alloc(P.cost, nodes, owned)) ||
            (rc = dev_alloc(P.frontier, nodes, owned)) || (rc = dev_alloc(P.r1node, nodes, owned)) ||
            (rc = dev_alloc(P.r1child, nodes, owned)) ||
            (rc = dev_alloc(P.fnext, nodes, owned)) || (rc = dev_alloc(P.react, nodes, owned)) ||

@@ -340,6 +340,7 @@ __device__ void init_query(const QueryDesc& qd, QueryState& st, const Pools& P)
     if(threadIdx.x == 0)
         {
             P.parent[qd.nodeOff]   = -1;
+            P.grand[qd.nodeOff]    = -1;
             P.cost[qd.nodeOff]     = 0.0f;
             P.frontier[qd.nodeOff] = 1;
             int ix, iy, iz;
@@ -511,6 +512,7 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
     uint32_t r2bit = 0;
     const uint32_t per = st.perN;
     uint32_t p         = 0;
+    int32_t gp         = -1;  // parent of the parent
     float s[S], uc[C];
     int32_t steps = 0;
     float uf      = 0.0f;
@@ -524,6 +526,7 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
             const float* T = P.treeS + qd.sampleOff;
 #pragma unroll
             for(int d = 0; d < S; ++d) s[d] = T[(size_t)d * cap + p];
+            gp = P.parent[nodeO + p];
 #pragma unroll
             for(int k = 0; k < C; ++k) uc[k] = L.ctrl[k][i];
             steps = L.steps[i];
@@ -581,13 +584,14 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
             const uint32_t v = base + (i & ~31u) + (uint32_t)__popc(L.fbits[fb][i >> 5] & ((1u << (i & 31u)) - 1u));
             float w[RW];
             w[0] = __int_as_float((int32_t)p);
+            w[1] = __int_as_float(gp);
 #pragma unroll
-            for(int d = 0; d < S; ++d) w[1 + d] = s[d];
+            for(int d = 0; d < S; ++d) w[2 + d] = s[d];
 #pragma unroll
-            for(int k = 0; k < C; ++k) w[1 + S + k] = uc[k];
-            w[D] = qd.cfg.dt * (float)steps;
+            for(int k = 0; k < C; ++k) w[2 + S + k] = uc[k];
+            w[D + 1] = qd.cfg.dt * (float)steps;
 #pragma unroll
-            for(int e = D + 1; e < RW; ++e) w[e] = 0.0f;
+            for(int e = D + 2; e < RW; ++e) w[e] = 0.0f;
             float4* row = reinterpret_cast<float4*>(P.childR + qd.rowOffC + (size_t)v * RW);
 #pragma unroll
             for(int j = 0; j < RW / 4; ++j) row[j] = make_float4(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3]);
@@ -739,7 +743,7 @@ __device__ void append_children(const QueryDesc& qd, QueryState& st, uint32_t c,
     int32_t vcount[kPer]   = {0, 0, 0, 0};
     constexpr int kRowMax = row_words(KGMT_MAX_SAMPLE);
     const int RW          = row_words(D);
-    float v[kPer][kRowMax];  // rows as stored: parent bits, then the sample
+    float v[kPer][kRowMax];  // rows as stored: parent and grandparent bits, then the sample
     if(u0 < nC)
         {
             // flags and regions together
@@ -796,9 +800,10 @@ __device__ void append_children(const QueryDesc& qd, QueryState& st, uint32_t c,
                     {
 #pragma unroll
                         for(int d = 0; d < KGMT_MAX_SAMPLE; ++d)
-                            if(d < D) T[(size_t)d * cap + t] = v[k][1 + d];
+                            if(d < D) T[(size_t)d * cap + t] = v[k][2 + d];
                         P.parent[nodeO + t]   = __float_as_int(v[k][0]);
-                        const float dg        = goal_dist(qd, v[k][1], v[k][2], v[k][3]);
+                        P.grand[nodeO + t]    = __float_as_int(v[k][1]);
+                        const float dg        = goal_dist(qd, v[k][2], v[k][3], v[k][4]);
                         P.cost[nodeO + t]     = dg;
                         P.frontier[nodeO + t] = 1;
                         P.r1node[nodeO + t]   = r1[k];
@@ -1074,21 +1079,45 @@ __global__ __launch_bounds__(64) void k_paths(const QueryDesc* __restrict__ qds,
     const uint32_t cap  = qd.cap;
     const int D         = qd.D;
     const float* T      = P.treeS + qd.sampleOff;
+    const int32_t* par  = P.parent + qd.nodeOff;
+    const int32_t* gpar = P.grand + qd.nodeOff;
     float* out          = P.path + qd.pathOff;
-    int32_t n           = st.goalIdx;
     uint32_t rows       = 0;
     float pc = 0.0f, pl = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
     const uint32_t maxRows = qd.cfg.max_iter + 2;
-    while(n != -1 && rows < maxRows)
+    // one node's row, in walk order (its parent pn decides whether its edge
+    // duration counts)
+    auto emit = [&](const float* r, int32_t pn) {
+        for(int d = 0; d < D; ++d) out[rows * D + d] = r[d];
+        if(pn != -1) pc = pc + r[D - 1];
+        if(rows > 0) pl = pl + pos_dist(qd, px, py, pz, r[0], r[1], r[2]);
+        px = r[0], py = r[1], pz = r[2];
+        ++rows;
+    };
+    // two levels per round of loads: a and its parent b are known; their
+    // rows, b's parent c and grandparent d are loaded together
+    int32_t a = st.goalIdx;
+    int32_t b = par[a];
+    while(a != -1 && rows < maxRows)
         {
-            for(int d = 0; d < D; ++d) out[rows * D + d] = T[(size_t)d * cap + n];
-            const float x = T[n], y = T[(size_t)cap + n], z = T[2 * (size_t)cap + n];
-            const int32_t par = P.parent[qd.nodeOff + n];
-            if(par != -1) pc = pc + T[(size_t)(D - 1) * cap + n];
-            if(rows > 0) pl = pl + pos_dist(qd, px, py, pz, x, y, z);
-            px = x, py = y, pz = z;
-            ++rows;
-            n = par;
+            float ra[KGMT_MAX_SAMPLE], rb[KGMT_MAX_SAMPLE];
+            int32_t c = -1, d = -1;
+#pragma unroll
+            for(int k = 0; k < KGMT_MAX_SAMPLE; ++k)
+                if(k < D) ra[k] = T[(size_t)k * cap + a];
+            if(b != -1)
+                {
+#pragma unroll
+                    for(int k = 0; k < KGMT_MAX_SAMPLE; ++k)
+                        if(k < D) rb[k] = T[(size_t)k * cap + b];
+                    c = par[b];
+                    d = gpar[b];
+                }
+            emit(ra, b);
+            if(b == -1 || rows >= maxRows) break;
+            emit(rb, c);
+            a = c;
+            b = d;
         }
     st.pathRows = rows;
     st.pathCost = pc;
