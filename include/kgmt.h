@@ -207,7 +207,12 @@ int kgmt_set_stream(kgmt_ctx* ctx, void* hip_stream);
  * W position floats of goal are read).  obstacles: 2W*n_obs floats, one box
  * per row (xmin ymin zmin xmax ymax zmax; xmin ymin xmax ymax for W = 2); a
  * device pointer when obstacles_on_device is non-zero (as in the reference),
- * else host memory. Blocking. */
+ * else host memory. Blocking: returns with the result and the goal path
+ * (kgmt_get_path) complete. The device writes them into host memory when the
+ * last iteration closes, so the call does not wait for the few launches that
+ * were queued past it; they exit at once on the context's stream, and the
+ * tree and region getters wait for them. The obstacle rows are not read after
+ * the call returns. */
 int kgmt_plan(kgmt_ctx* ctx, const float* start, const float* goal, const float* obstacles,
               uint32_t n_obs, int32_t obstacles_on_device, kgmt_result* out);
 /* Host copies of the tree of the last plan: samples as rows of D floats

@@ -135,6 +135,18 @@ struct Pools
     float* path;         // goal->root rows
 };
 
+// Result of a single-query plan, written by the device into host-mapped
+// memory before the completion word (the host then returns at once, leaving
+// the launches it queued past the last iteration to exit on the GPU).
+struct HostResult
+{
+    uint32_t success, iterations, treeSize, pathRows;
+    int32_t goalIdx;
+    float pathCost, pathLen;
+    uint32_t pad;
+    uint64_t expanded, flagged;
+};
+
 // Work lists over queries, double-buffered by iteration parity: the kernels
 // of iteration i read buffer (i-1)&1 while the last finisher of that
 // iteration writes the other one. Prefix sums of nq + 1 entries each.
@@ -145,6 +157,8 @@ struct WorkLists
     uint32_t* active;      // 2: queries not done in each list
     uint32_t* counter;     // arrivals of query finishers (last-block pattern)
     uint32_t* allDone;     // host-mapped word: 1 once every query is done
+    HostResult* result;    // host-mapped result of a single-query plan (null: batch, or no early result)
+    float* hostPath;       // host-mapped goal -> root rows of that plan ((max_iter + 2) x D floats)
     uint32_t nq;
     uint32_t tile;         // children per propagate tile (= its workgroup size)
 

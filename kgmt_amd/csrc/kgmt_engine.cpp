@@ -141,6 +141,7 @@ int dev_alloc(T*& p, size_t count, std::vector<void*>& owned)
 }
 
 constexpr int kRing = 8;  // completion polls in flight
+constexpr size_t kHostPathFloats = size_t(1) << 20;  // host-mapped path rows of a single query (4 MiB)
 constexpr int kLag  = 2;  // launch units the host stays behind the GPU
 
 }  // namespace
@@ -206,6 +207,8 @@ struct Engine
     const float* sceneRows = nullptr;  // device rows the scenes of the next run are built from
     bool sceneBuild  = false;
     bool counterDirty = false;  // W.counter may be non-zero (a plan did not complete)
+    HostResult* hostRes = nullptr;  // host-mapped result of a single-query plan
+    float* hostPath     = nullptr;  // host-mapped path rows of that plan
     uint32_t maxPairs = 0;   // largest per-query scene (pairs) of the current plan
     int model       = -1;    // common model or -1
     uint32_t maxIter = 0;
@@ -229,6 +232,8 @@ struct Engine
 
     ~Engine()
     {
+        // a plan may have returned while its trailing launches still drain
+        if(stream) (void)hipStreamSynchronize(stream);
         for(auto& e : profEv)
             if(e) (void)hipEventDestroy(e);
         if(graphExec) (void)hipGraphExecDestroy(graphExec);
@@ -236,6 +241,8 @@ struct Engine
         for(auto& e : ev)
             if(e) (void)hipEventDestroy(e);
         if(pinned) (void)hipHostFree(pinned);
+        if(hostRes) (void)hipHostFree(hostRes);
+        if(hostPath) (void)hipHostFree(hostPath);
         for(void* p : owned) (void)hipFree(p);
         if(ownStream && stream) (void)hipStreamDestroy(stream);
     }
@@ -301,6 +308,20 @@ struct Engine
         HIP_TRY(hipHostMalloc((void**)&pinned, sizeof(uint32_t), hipHostMallocMapped));
         *pinned = 0;
         HIP_TRY(hipHostGetDevicePointer((void**)&W.allDone, pinned, 0));
+        W.result   = nullptr;
+        W.hostPath = nullptr;
+        const size_t pathFloats = (size_t)(cfgs[0].max_iter + 2) * (size_t)hq[0].D;
+        if(n == 1 && pathFloats <= kHostPathFloats)
+            {
+                // a single query publishes its result and path rows into
+                // host-mapped memory (too long a possible path: the plan
+                // synchronises and copies instead)
+                HIP_TRY(hipHostMalloc((void**)&hostRes, sizeof(HostResult), hipHostMallocMapped));
+                std::memset((void*)hostRes, 0, sizeof(HostResult));
+                HIP_TRY(hipHostGetDevicePointer((void**)&W.result, hostRes, 0));
+                HIP_TRY(hipHostMalloc((void**)&hostPath, pathFloats * sizeof(float), hipHostMallocMapped));
+                HIP_TRY(hipHostGetDevicePointer((void**)&W.hostPath, hostPath, 0));
+            }
         for(auto& e : ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         // launch shape: persistent workgroups sized to the device
         hipDeviceProp_t prop;
@@ -313,6 +334,7 @@ struct Engine
 
     int set_stream(void* s)
     {
+        if(stream) HIP_TRY(hipStreamSynchronize(stream));
         if(ownStream && stream) HIP_TRY(hipStreamDestroy(stream));
         ownStream = false;
         stream    = (hipStream_t)s;
@@ -327,6 +349,7 @@ struct Engine
 
     void drop_graph()
     {
+        if(graphExec && stream) (void)hipStreamSynchronize(stream);  // a replay may still drain
         if(graphExec) (void)hipGraphExecDestroy(graphExec);
         if(graph) (void)hipGraphDestroy(graph);
         graphExec = nullptr, graph = nullptr, graphObs = 0xffffffffu;
@@ -469,11 +492,23 @@ struct Engine
             }
         const bool prof        = profiling && !useGraph;
         const uint32_t perUnit = useGraph ? 2 : 1;
+        // one query, not profiling: the device publishes the result into
+        // host-mapped memory before the completion word, and the host returns
+        // as soon as it sees it (no path launch, copy or stream wait)
+        const bool early = hostRes != nullptr && !prof;
+        if(hostRes) hostRes->iterations = 0;
+        bool published = false;
+        auto seen      = [&]() { return early && __atomic_load_n(pinned, __ATOMIC_ACQUIRE) != 0; };
         iterMs.assign(maxIter + 2, 0.0);
         uint32_t it = 0, units = 0, observed = 0;
         bool finished = false;
         while(it < maxIter && !finished)
             {
+                if(seen())
+                    {
+                        finished = published = true;
+                        break;
+                    }
                 const int slot = (int)(units % kRing);
                 if(useGraph)
                     HIP_TRY(hipGraphLaunch(graphExec, stream));
@@ -489,7 +524,13 @@ struct Engine
                 while(observed + kLag < units || (it >= maxIter && observed < units))
                     {
                         hipError_t q;
-                        while((q = hipEventQuery(ev[observed % kRing])) == hipErrorNotReady) {}
+                        while((q = hipEventQuery(ev[observed % kRing])) == hipErrorNotReady)
+                            if(seen()) break;
+                        if(seen())
+                            {
+                                finished = published = true;
+                                break;
+                            }
                         if(q != hipSuccess) return fail(KGMT_E_HIP, std::string("iteration: ") + hipGetErrorString(q));
                         const double t =
                           std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -501,6 +542,21 @@ struct Engine
                                 break;
                             }
                     }
+            }
+        if(published && hostRes->iterations != 0)
+            {
+                const volatile HostResult* h = hostRes;
+                QueryState& s                = hs[0];
+                s.done = 1, s.success = h->success;
+                s.itr = s.finishItr = h->iterations;
+                s.treeSize          = h->treeSize;
+                s.goalIdx           = h->success ? h->goalIdx : kNoGoal;
+                s.pathRows          = h->pathRows;
+                s.pathCost = h->pathCost, s.pathLen = h->pathLen;
+                s.expanded = h->expanded, s.flagged = h->flagged;
+                counterDirty = false;
+                itersRun     = std::min(it, maxIter);
+                return KGMT_OK;
             }
         HIP_TRY(launch_paths(dq, ds, P, nq, stream));
         HIP_TRY(hipMemcpyAsync(hs.data(), ds, sizeof(QueryState) * nq, hipMemcpyDeviceToHost, stream));
@@ -527,6 +583,7 @@ struct Engine
 
     int get_tree(uint32_t q, float* samples, int32_t* parents, float* costs, uint32_t cap, uint32_t* n)
     {
+        HIP_TRY(hipStreamSynchronize(stream));  // a plan returns before its trailing launches drain
         const QueryDesc& qd = hq[q];
         const uint32_t ts   = hs[q].treeSize;
         if(n) *n = ts;
@@ -554,6 +611,14 @@ struct Engine
         if(nRows) *nRows = r;
         if(!rows || r == 0) return KGMT_OK;
         if(capRows < r) return fail(KGMT_E_CAPACITY, "path output buffer too small");
+        if(hostPath)
+            {
+                // a single query's rows were written to host memory before
+                // its result was published (or the plan synchronised)
+                std::memcpy(rows, (const void*)hostPath, sizeof(float) * r * qd.D);
+                return KGMT_OK;
+            }
+        HIP_TRY(hipStreamSynchronize(stream));  // a plan returns before its trailing launches drain
         HIP_TRY(hipMemcpy(rows, P.path + qd.pathOff, sizeof(float) * r * qd.D, hipMemcpyDeviceToHost));
         return KGMT_OK;
     }
@@ -867,6 +932,7 @@ int kgmt_get_regions(kgmt_ctx* ctx, int32_t* total, int32_t* valid, float* score
     const QueryDesc& qd = e.hq[0];
     if(n) *n = (uint32_t)qd.R1;
     if(cap < (uint32_t)qd.R1 && (total || valid || score)) return fail(KGMT_E_CAPACITY, "region buffer too small");
+    HIP_TRY(hipStreamSynchronize(e.stream));  // a plan returns before its trailing launches drain
     if(total) HIP_TRY(hipMemcpy(total, e.P.total + qd.regOff, sizeof(int32_t) * qd.R1, hipMemcpyDeviceToHost));
     if(valid) HIP_TRY(hipMemcpy(valid, e.P.validc + qd.regOff, sizeof(int32_t) * qd.R1, hipMemcpyDeviceToHost));
     if(score) HIP_TRY(hipMemcpy(score, e.P.score + qd.regOff, sizeof(float) * qd.R1, hipMemcpyDeviceToHost));

@@ -920,7 +920,8 @@ __device__ void finish_query(const QueryDesc& qd, QueryState& st, uint32_t R)
 }
 
 // Work lists of a single-query table for its next iteration (parity
-// buffer), from the next-state fields (one thread).
+// buffer), from the next-state fields (one thread). The completion word stays
+// 0 until publish_single sets it.
 __device__ __forceinline__ void single_worklists(const QueryState& st, const WorkLists& W, uint32_t parity)
 {
     uint32_t* tileStart = W.tileStart + parity * 2;
@@ -931,7 +932,35 @@ __device__ __forceinline__ void single_worklists(const QueryState& st, const Wor
     tileStart[1]        = alive ? (st.nChildrenN + W.tile - 1) / W.tile : 0u;
     jobStart[1] = alive ? (st.nChildrenN + kChunk - 1) / kChunk + (st.treeSizeN + kChunk - 1) / kChunk : 0u;
     W.active[parity] = alive ? 1u : 0u;
-    __hip_atomic_store(W.allDone, alive ? 0u : 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__device__ void walk_path(const QueryDesc& qd, QueryState& st, const Pools& P, float* out);
+
+// End of a single-query plan (one thread, next-state fields final): the goal
+// path rows and the result into host-mapped memory, then the completion word,
+// so the host can return without waiting for the launches it queued past the
+// last iteration (they find the query done and exit). Without a host result
+// buffer only the completion word is set.
+__device__ void publish_single(const QueryDesc& qd, QueryState& st, const Pools& P, const WorkLists& W)
+{
+    if(!W.result)
+        {
+            __hip_atomic_store(W.allDone, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            return;
+        }
+    if(st.successN && st.pathRows == 0) walk_path(qd, st, P, W.hostPath);
+    HostResult* h = W.result;
+    h->success    = st.successN;
+    h->iterations = st.finishItrN;
+    h->treeSize   = st.treeSizeN;
+    h->pathRows   = st.successN ? st.pathRows : 0u;
+    h->goalIdx    = st.successN ? st.goalIdx : -1;
+    h->pathCost   = st.successN ? st.pathCost : 0.0f;
+    h->pathLen    = st.successN ? st.pathLen : 0.0f;
+    h->expanded   = st.expandedN;
+    h->flagged    = st.flaggedN;
+    __threadfence_system();
+    __hip_atomic_store(W.allDone, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ---------------------------------------------------------------------------
@@ -1042,6 +1071,7 @@ __global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__
                         {
                             finish_query(qd, st, R);
                             single_worklists(st, W, parity ^ 1u);
+                            if(st.doneN) publish_single(qd, st, P, W);
                         }
                 }
             __syncthreads();
@@ -1067,21 +1097,16 @@ __global__ __launch_bounds__(kBlock) void k_finish(const QueryDesc* __restrict__
 // cost = sum of edge durations, length = sum of segment lengths between the
 // node positions
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_paths(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
-                                              Pools P, uint32_t nq)
+// goal -> root rows of a solved query (pathRows etc. of st); two levels per
+// round of loads: a and its parent b are known, their rows, b's parent c and
+// grandparent d are loaded together
+__device__ void walk_path(const QueryDesc& qd, QueryState& st, const Pools& P, float* out)
 {
-    const uint32_t q = blockIdx.x * 64 + threadIdx.x;
-    if(q >= nq) return;
-    QueryState& st = sts[q];
-    commit_state(st);  // the closed last iteration
-    if(!st.success || st.pathRows) return;
-    const QueryDesc& qd = qds[q];
     const uint32_t cap  = qd.cap;
     const int D         = qd.D;
     const float* T      = P.treeS + qd.sampleOff;
     const int32_t* par  = P.parent + qd.nodeOff;
     const int32_t* gpar = P.grand + qd.nodeOff;
-    float* out          = P.path + qd.pathOff;
     uint32_t rows       = 0;
     float pc = 0.0f, pl = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
     const uint32_t maxRows = qd.cfg.max_iter + 2;
@@ -1094,8 +1119,6 @@ __global__ __launch_bounds__(64) void k_paths(const QueryDesc* __restrict__ qds,
         px = r[0], py = r[1], pz = r[2];
         ++rows;
     };
-    // two levels per round of loads: a and its parent b are known; their
-    // rows, b's parent c and grandparent d are loaded together
     int32_t a = st.goalIdx;
     int32_t b = par[a];
     while(a != -1 && rows < maxRows)
@@ -1122,6 +1145,17 @@ __global__ __launch_bounds__(64) void k_paths(const QueryDesc* __restrict__ qds,
     st.pathRows = rows;
     st.pathCost = pc;
     st.pathLen  = pl;
+}
+
+__global__ __launch_bounds__(64) void k_paths(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
+                                              Pools P, uint32_t nq)
+{
+    const uint32_t q = blockIdx.x * 64 + threadIdx.x;
+    if(q >= nq) return;
+    QueryState& st = sts[q];
+    commit_state(st);  // the closed last iteration
+    if(!st.success || st.pathRows) return;
+    walk_path(qds[q], st, P, P.path + qds[q].pathOff);
 }
 
 // ---------------------------------------------------------------------------
