@@ -73,6 +73,7 @@ def run_plain(name, steps, warmup, profile=1, launch=None):
         g.set_seed(q.cfg.seed)
         r = g.plan(q.start, q.goal, int(d.data_ptr()), len(q.obstacles))
         res.append(r.as_dict())
+    g.synchronize()  # the planner's own stream (torch's synchronize does not cover it)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     # propagate launch durations: the same queries planned again with HIP
@@ -159,6 +160,8 @@ def run_batch(n, seed=0, tune=True):
     res_p = []
     for g, q, d in zip(ctxs, qs, d_obs):
         res_p.append(g.plan(q.start, q.goal, int(d.data_ptr()), len(q.obstacles)))
+    for g in ctxs:
+        g.synchronize()
     torch.cuda.synchronize()
     dt_p = time.perf_counter() - t0
     same = all(a.tree_size == c.tree_size and a.success == c.success and a.iterations == c.iterations

@@ -203,6 +203,10 @@ int kgmt_set_launch(kgmt_ctx* ctx, uint32_t propagate_block, uint32_t lanes_per_
                     uint32_t chunk_grid);
 /* Use an existing hipStream_t for every launch of this context (NULL: own stream). */
 int kgmt_set_stream(kgmt_ctx* ctx, void* hip_stream);
+/* Waits until every launch of this context has finished (including the few
+ * that a plan leaves to drain after it returns). No counterpart in the
+ * reference (its plan() synchronises the device). */
+int kgmt_synchronize(kgmt_ctx* ctx);
 /* start, goal: host arrays of D floats (only state[0..S) of start and the
  * W position floats of goal are read).  obstacles: 2W*n_obs floats, one box
  * per row (xmin ymin zmin xmax ymax zmax; xmin ymin xmax ymax for W = 2); a

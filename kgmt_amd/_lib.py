@@ -42,6 +42,7 @@ SIGNATURES = {
     "kgmt_set_graph": (C.c_int, [_P, _i32]),
     "kgmt_set_launch": (C.c_int, [_P, _u32, _u32, _u32, _u32]),
     "kgmt_set_stream": (C.c_int, [_P, _P]),
+    "kgmt_synchronize": (C.c_int, [_P]),
     "kgmt_plan": (C.c_int, [_P, _P, _P, _P, _u32, _i32, _resp]),
     "kgmt_get_tree": (C.c_int, [_P, _P, _P, _P, _u32, C.POINTER(_u32)]),
     "kgmt_get_path": (C.c_int, [_P, _P, _u32, C.POINTER(_u32)]),

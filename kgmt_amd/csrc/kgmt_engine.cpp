@@ -798,6 +798,13 @@ int kgmt_set_stream(kgmt_ctx* ctx, void* s)
     return ctx->eng.set_stream(s);
 }
 
+int kgmt_synchronize(kgmt_ctx* ctx)
+{
+    if(!ctx) return fail(KGMT_E_ARG, "null context");
+    HIP_TRY(hipStreamSynchronize(ctx->eng.stream));
+    return KGMT_OK;
+}
+
 // Obstacle rows of every query -> scenes (grouped, Morton-sorted) in the pool,
 // built on the device. Rows hold 2 * W floats (W = the queries' workspace
 // dimension); they may be device memory (the reference passes a device

@@ -109,6 +109,11 @@ class KGMT(Planner):
         _lib.check(_lib.lib().kgmt_get_profile(self._h, C.byref(p)))
         return p.as_dict()
 
+    def synchronize(self):
+        """Wait for every launch of this planner (a plan returns with its result
+        while the launches queued past its last iteration still drain)."""
+        _lib.check(_lib.lib().kgmt_synchronize(self._h))
+
     def set_graph(self, on: bool = True):
         """Replay iterations from a captured hipGraph (ignored while profiling)."""
         _lib.check(_lib.lib().kgmt_set_graph(self._h, 1 if on else 0))

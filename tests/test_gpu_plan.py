@@ -72,6 +72,36 @@ def test_full_tree_ends_the_plan(orc):
     _same_plan(g, res, ref)
 
 
+def test_plans_back_to_back_and_synchronize(orc):
+    """A plan returns once its result is published, leaving launches queued
+    past its last iteration to drain: the next plan on the same context, the
+    getters and synchronize() still see the right state."""
+    qa = W.make_query("unicycle2d", 0)
+    qb = W.make_query("unicycle2d", 4)  # same configuration apart from the seed
+    ra = orc.plan(qa.cfg, qa.start, qa.goal, qa.obstacles)
+    rb = orc.plan(qb.cfg, qb.start, qb.goal, qb.obstacles)
+    g = KGMT(qa.cfg)
+    for _ in range(2):
+        g.set_seed(qa.cfg.seed)
+        res = g.plan(qa.start, qa.goal, qa.obstacles, len(qa.obstacles))
+        assert res.tree_size == ra["stats"]["tree_size"] and res.iterations == ra["stats"]["iterations"]
+        g.set_seed(qb.cfg.seed)
+        res = g.plan(qb.start, qb.goal, qb.obstacles, len(qb.obstacles))
+        _same_plan(g, res, rb)
+    g.synchronize()
+
+
+def test_long_max_iter_plans_synchronously(orc):
+    """A configuration whose longest possible path does not fit the host path
+    buffer (max_iter 200000) publishes no early result: the plan waits for its
+    launches and copies the path from the device, with the same answers."""
+    q = W.make_query("unicycle2d", 0, max_iter=200000)
+    ref = orc.plan(q.cfg, q.start, q.goal, q.obstacles)
+    g = KGMT(q.cfg)
+    res = g.plan(q.start, q.goal, q.obstacles, len(q.obstacles))
+    _same_plan(g, res, ref)
+
+
 @pytest.mark.parametrize("block", [512, 256])
 def test_max_obstacles_plan_matches_oracle(block, orc):
     """1500 boxes, the most a query may carry: the scene and the per-wave
