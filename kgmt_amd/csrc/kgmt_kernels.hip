@@ -371,18 +371,22 @@ __device__ void init_query(const QueryDesc& qd, QueryState& st, const Pools& P)
 // ---------------------------------------------------------------------------
 // propagate: persistent over 256-child tiles
 // ---------------------------------------------------------------------------
-// LDS of one propagate tile: the draws of its NT children and the order in
-// which the lanes take them.
-template <int NT>
+// LDS of one propagate tile: the draws of its NT / K children and the order
+// in which the lanes take them. The per-wave candidate lists of the two-phase
+// check exist only with one lane per child (the only shape that runs it), so
+// they take no LDS, and no occupancy, from the other shapes.
+template <int NT, int K>
 struct TileLds
 {
+    static constexpr int kCandWaves = K == 1 ? NT / 64 : 1;
+    static constexpr int kCandBoxes = K == 1 ? kCand : 1;
     uint32_t hist[NT];      // children per step bucket, then bucket offsets
     uint16_t slot[NT];      // lane -> child of the tile
     float ctrl[KGMT_MAX_CONTROL][NT];
     int32_t steps[NT];
     float ufront[NT];
     uint32_t sbuf[NT / 64 + 1];
-    float4 cand[NT / 64][2 * kCand];  // per-wave candidate boxes (two-phase check)
+    float4 cand[kCandWaves][2 * kCandBoxes];  // per-wave candidate boxes (two-phase check)
     // per-tile sums of the region counters and ORs of the subregion bitset
     // words, flushed with one global atomic per distinct key (open addressing,
     // key -1 = empty; at most NT distinct keys per tile, so a probe always ends)
@@ -396,8 +400,8 @@ struct TileLds
 };
 
 // empties thread t's slot of both tables and of flag buffer `fb`
-template <int NT>
-__device__ __forceinline__ void clear_tables(TileLds<NT>& L, uint32_t fb)
+template <int NT, int K>
+__device__ __forceinline__ void clear_tables(TileLds<NT, K>& L, uint32_t fb)
 {
     const uint32_t t = threadIdx.x;
     L.rkey[t] = -1, L.rtot[t] = 0, L.rval[t] = 0;
@@ -434,7 +438,7 @@ __device__ __forceinline__ uint32_t tab_slot(int32_t* keys, int32_t key)
 // the order or on K.
 template <int MODEL, int NT, int K>
 __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState& st, uint32_t base,
-                                           const Pools& P, const float4* sObs, TileLds<NT>& L, uint32_t fb)
+                                           const Pools& P, const float4* sObs, TileLds<NT, K>& L, uint32_t fb)
 {
     constexpr int S = ModelDims<MODEL>::S, C = ModelDims<MODEL>::C, D = S + C + 1;
     constexpr uint32_t TC = NT / K;
@@ -619,7 +623,7 @@ __global__ __launch_bounds__(NT, 4) void k_propagate(const QueryDesc* __restrict
 {
     static_assert(NT % 64 == 0 && kChunk % (NT / K) == 0, "tile size");
     extern __shared__ float4 sObs[];
-    __shared__ TileLds<NT> L;
+    __shared__ TileLds<NT, K> L;
     const uint32_t nq        = W.nq;
     const uint32_t* tileList = W.tiles(parity);
     const uint32_t nTiles    = tileList[nq];
@@ -934,7 +938,7 @@ __device__ __forceinline__ void single_worklists(const QueryState& st, const Wor
     W.active[parity] = alive ? 1u : 0u;
 }
 
-__device__ void walk_path(const QueryDesc& qd, QueryState& st, const Pools& P, float* out);
+__device__ void walk_path(const QueryDesc& qd, QueryState& st, const Pools& P, float* __restrict__ out);
 
 // End of a single-query plan (one thread, next-state fields final): the goal
 // path rows and the result into host-mapped memory, then the completion word,
@@ -1100,21 +1104,24 @@ __global__ __launch_bounds__(kBlock) void k_finish(const QueryDesc* __restrict__
 // goal -> root rows of a solved query (pathRows etc. of st); two levels per
 // round of loads: a and its parent b are known, their rows, b's parent c and
 // grandparent d are loaded together
-__device__ void walk_path(const QueryDesc& qd, QueryState& st, const Pools& P, float* out)
+__device__ void walk_path(const QueryDesc& qd, QueryState& st, const Pools& P, float* __restrict__ out)
 {
-    const uint32_t cap  = qd.cap;
-    const int D         = qd.D;
-    const float* T      = P.treeS + qd.sampleOff;
-    const int32_t* par  = P.parent + qd.nodeOff;
-    const int32_t* gpar = P.grand + qd.nodeOff;
-    uint32_t rows       = 0;
+    const uint32_t cap            = qd.cap;
+    const int D                   = qd.D;
+    const float* __restrict__ T   = P.treeS + qd.sampleOff;
+    const float* __restrict__ dur = T + (size_t)(D - 1) * cap;  // edge durations
+    const int32_t* par            = P.parent + qd.nodeOff;
+    const int32_t* gpar           = P.grand + qd.nodeOff;
+    uint32_t rows                 = 0;
     float pc = 0.0f, pl = 0.0f, px = 0.0f, py = 0.0f, pz = 0.0f;
     const uint32_t maxRows = qd.cfg.max_iter + 2;
     // one node's row, in walk order (its parent pn decides whether its edge
-    // duration counts)
-    auto emit = [&](const float* r, int32_t pn) {
-        for(int d = 0; d < D; ++d) out[rows * D + d] = r[d];
-        if(pn != -1) pc = pc + r[D - 1];
+    // duration dn counts); indices into r are compile-time constants
+    auto emit = [&](const float* r, float dn, int32_t pn) {
+#pragma unroll
+        for(int k = 0; k < KGMT_MAX_SAMPLE; ++k)
+            if(k < D) out[rows * D + k] = r[k];
+        if(pn != -1) pc = pc + dn;
         if(rows > 0) pl = pl + pos_dist(qd, px, py, pz, r[0], r[1], r[2]);
         px = r[0], py = r[1], pz = r[2];
         ++rows;
@@ -1124,6 +1131,7 @@ __device__ void walk_path(const QueryDesc& qd, QueryState& st, const Pools& P, f
     while(a != -1 && rows < maxRows)
         {
             float ra[KGMT_MAX_SAMPLE], rb[KGMT_MAX_SAMPLE];
+            float da = dur[a], db = 0.0f;
             int32_t c = -1, d = -1;
 #pragma unroll
             for(int k = 0; k < KGMT_MAX_SAMPLE; ++k)
@@ -1133,12 +1141,13 @@ __device__ void walk_path(const QueryDesc& qd, QueryState& st, const Pools& P, f
 #pragma unroll
                     for(int k = 0; k < KGMT_MAX_SAMPLE; ++k)
                         if(k < D) rb[k] = T[(size_t)k * cap + b];
-                    c = par[b];
-                    d = gpar[b];
+                    db = dur[b];
+                    c  = par[b];
+                    d  = gpar[b];
                 }
-            emit(ra, b);
+            emit(ra, da, b);
             if(b == -1 || rows >= maxRows) break;
-            emit(rb, c);
+            emit(rb, db, c);
             a = c;
             b = d;
         }
