@@ -71,7 +71,6 @@ struct QueryState
     uint32_t nChildren;  // children to propagate this iteration
     uint32_t per;        // children per expansion entry
     uint32_t R;          // size of the expansion multiset
-    uint32_t nReact;     // re-activated old nodes this iteration
     uint32_t nAcc;       // children flagged for the next frontier
     uint32_t next;       // children appended (nAcc clamped to capacity)
     float fAccept;       // re-activation bonus
@@ -80,7 +79,6 @@ struct QueryState
     int32_t goalIdx;     // smallest tree index within goal_thresh (kNoGoal if none)
     uint32_t pathRows;
     float pathCost, pathLen;
-    uint32_t jobsDone;   // last-block counter of the append kernel
     uint64_t expanded;
     uint64_t flagged;    // children flagged for the tree over all iterations (rows written)
     uint32_t finishItr;
@@ -155,7 +153,7 @@ struct WorkLists
     uint32_t* tileStart;   // 2 x (nq + 1): propagate tiles
     uint32_t* jobStart;    // 2 x (nq + 1): append / expand chunk jobs
     uint32_t* active;      // 2: queries not done in each list
-    uint32_t* counter;     // arrivals of query finishers (last-block pattern)
+    uint32_t* counter;     // arrivals of k_init's workgroups (the last builds the first work lists)
     uint32_t* allDone;     // host-mapped word: 1 once every query is done
     HostResult* result;    // host-mapped result of a single-query plan (null: batch, or no early result)
     float* hostPath;       // host-mapped goal -> root rows of that plan ((max_iter + 2) x D floats)

@@ -394,7 +394,8 @@ struct Engine
         return KGMT_OK;
     }
 
-    // One iteration: 5 launches. marks: timing events around them (profiling).
+    // One iteration: 4 launches for one query, 5 for a batch (k_finish).
+    // marks: timing events around them (profiling).
     int enqueue_iteration(uint32_t parity, hipEvent_t* marks = nullptr)
     {
         const bool all = marks && profiling > 1;

@@ -352,13 +352,12 @@ __device__ void init_query(const QueryDesc& qd, QueryState& st, const Pools& P)
             launch_regime(qd, st.R, 1, per, nc);
             st.per       = per;
             st.nChildren = nc;
-            st.nReact = 0, st.nAcc = 0, st.next = 0;
+            st.nAcc = 0, st.next = 0;
             st.fAccept = 0.0f;
             st.done = 0, st.success = 0;
             st.goalIdx  = kNoGoal;
             st.pathRows = 0;
             st.pathCost = 0.0f, st.pathLen = 0.0f;
-            st.jobsDone  = 0;
             st.expanded  = 0;
             st.flagged   = 0;
             st.finishItr = 0;
