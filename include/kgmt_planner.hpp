@@ -99,6 +99,9 @@ public:
     }
 
     void setSeed(uint64_t seed) { kgmt::check(kgmt_set_seed(ctx_, seed)); }
+    // plan() returns with its results complete; this also waits for the few
+    // launches it leaves to drain on the planner's stream
+    void synchronize() { kgmt::check(kgmt_synchronize(ctx_)); }
 
     // host copies of d_treeSamples_ (row-major, D floats per node),
     // d_treeSamplesParentIdxs_, d_treeSampleCosts_
