@@ -605,10 +605,12 @@ struct Rot  // sin/cos of an angle increment
 {
     float s, c;
 };
+// angle addition with one fused multiply-add per component (the oracle
+// evaluates the same fused operations)
 __device__ __forceinline__ void rotate(float s0, float c0, Rot r, float& s, float& c)
 {
-    s = s0 * r.c + c0 * r.s;
-    c = c0 * r.c - s0 * r.s;
+    s = __builtin_fmaf(s0, r.c, c0 * r.s);
+    c = __builtin_fmaf(c0, r.c, -(s0 * r.s));
 }
 
 template <bool FIVE_D, class CHK>
@@ -633,8 +635,10 @@ __device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const
     for(int i = 0; i < steps; ++i)
         {
             const float px = x, py = y, pz = z;
-            const float vm = v + hh * a;
-            const float ve = v + h * a;
+            // RK4 with fused multiply-adds at fixed places (the oracle's
+            // prop_dubins evaluates the same operations)
+            const float vm = __builtin_fmaf(hh, a, v);
+            const float ve = __builtin_fmaf(h, a, v);
             float spm, cpm, sym, cym, spe, cpe, sye, cye;
             rotate(sp0, cp0, pH, spm, cpm);
             rotate(sy0, cy0, yH, sym, cym);
@@ -643,12 +647,13 @@ __device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const
             const float mx = vm * cpm * cym;
             const float my = vm * cpm * sym;
             const float mz = vm * spm;
-            x              = x + (h / 6.0f) * (v * cp0 * cy0 + 2.0f * (mx + mx) + ve * cpe * cye);
-            y              = y + (h / 6.0f) * (v * cp0 * sy0 + 2.0f * (my + my) + ve * cpe * sye);
-            z              = z + (h / 6.0f) * (v * sp0 + 2.0f * (mz + mz) + ve * spe);
-            yaw            = yaw + h * yr;
-            pitch          = pitch + h * pr;
-            v              = v + (h / 6.0f) * (a + 2.0f * (a + a) + a);
+            const float h6 = h / 6.0f;
+            x = __builtin_fmaf(h6, __builtin_fmaf(ve * cpe, cye, __builtin_fmaf(2.0f, mx + mx, v * cp0 * cy0)), x);
+            y = __builtin_fmaf(h6, __builtin_fmaf(ve * cpe, sye, __builtin_fmaf(2.0f, my + my, v * cp0 * sy0)), y);
+            z = __builtin_fmaf(h6, __builtin_fmaf(ve, spe, __builtin_fmaf(2.0f, mz + mz, v * sp0)), z);
+            yaw   = __builtin_fmaf(h, yr, yaw);
+            pitch = __builtin_fmaf(h, pr, pitch);
+            v     = __builtin_fmaf(h6, __builtin_fmaf(2.0f, a + a, a) + a, v);
             sp0 = spe, cp0 = cpe, sy0 = sye, cy0 = cye;
             // the reference's checks in its order (speed, pitch, workspace,
             // segment) all end the edge as invalid at this step, so they are

@@ -397,11 +397,12 @@ static int prop_di(const kgmt_config* cfg, const float* x0, const float* ctrl, i
 }
 
 /* sin/cos of angle + increment from sin/cos of the angle (s0, c0) and of the
- * increment (rs, rc): angle addition, products rounded, then the sum */
+ * increment (rs, rc): angle addition, the second product rounded, then one
+ * fused multiply-add (C99 fmaf, correctly rounded like the GPU's v_fma_f32) */
 static void rotate(float s0, float c0, float rs, float rc, float* s, float* c)
 {
-    *s = s0 * rc + c0 * rs;
-    *c = c0 * rc - s0 * rs;
+    *s = fmaf(s0, rc, c0 * rs);
+    *c = fmaf(c0, rc, -(s0 * rs));
 }
 
 /* statePropagator.cu:147-241 (6D: speed v is a state, a its control) and the
@@ -431,8 +432,9 @@ static int prop_dubins(const kgmt_config* cfg, const float* x0, const float* ctr
     for(int i = 0; i < steps; ++i)
         {
             float p0[3] = {x, y, z};
-            float vm = v + hh * a;
-            float ve = v + h * a;
+            /* fused multiply-adds (C99 fmaf: one rounding) at fixed places */
+            float vm = fmaf(hh, a, v);
+            float ve = fmaf(h, a, v);
             float spm, cpm, sym, cym, spe, cpe, sye, cye;
             rotate(sp0, cp0, phs, phc, &spm, &cpm);
             rotate(sy0, cy0, yhs, yhc, &sym, &cym);
@@ -441,12 +443,13 @@ static int prop_dubins(const kgmt_config* cfg, const float* x0, const float* ctr
             float mx = vm * cpm * cym;
             float my = vm * cpm * sym;
             float mz = vm * spm;
-            x        = x + (h / 6.0f) * (v * cp0 * cy0 + 2.0f * (mx + mx) + ve * cpe * cye);
-            y        = y + (h / 6.0f) * (v * cp0 * sy0 + 2.0f * (my + my) + ve * cpe * sye);
-            z        = z + (h / 6.0f) * (v * sp0 + 2.0f * (mz + mz) + ve * spe);
-            yaw      = yaw + h * yr;
-            pitch    = pitch + h * pr;
-            v        = v + (h / 6.0f) * (a + 2.0f * (a + a) + a);
+            float h6 = h / 6.0f;
+            x        = fmaf(h6, fmaf(ve * cpe, cye, fmaf(2.0f, mx + mx, v * cp0 * cy0)), x);
+            y        = fmaf(h6, fmaf(ve * cpe, sye, fmaf(2.0f, my + my, v * cp0 * sy0)), y);
+            z        = fmaf(h6, fmaf(ve, spe, fmaf(2.0f, mz + mz, v * sp0)), z);
+            yaw      = fmaf(h, yr, yaw);
+            pitch    = fmaf(h, pr, pitch);
+            v        = fmaf(h6, fmaf(2.0f, a + a, a) + a, v);
             sp0 = spe, cp0 = cpe, sy0 = sye, cy0 = cye;
             if(!five_d && (v < cfg->v_min || v > cfg->v_max))
                 {
