@@ -172,6 +172,48 @@ def test_plan_no_obstacles_and_unreachable(orc):
     _same_plan(g2, res, ref)
 
 
+def test_degenerate_starts_match_oracle(orc):
+    q = W.make_query("di6d", 5, capacity=1 << 13, max_iter=15)
+    g = KGMT(q.cfg)
+    # start inside a box: every edge from the root collides, the tree stays at
+    # the root, which is re-activated every iteration until max_iter
+    s = np.asarray(q.start, np.float32)
+    box = np.concatenate([s[:3] - 0.05, s[:3] + 0.05])[None].astype(np.float32)
+    ref = orc.plan(q.cfg, q.start, q.goal, box)
+    res = g.plan(q.start, q.goal, box, 1)
+    assert not res.success and res.tree_size == 1 and res.iterations == q.cfg.max_iter
+    _same_plan(g, res, ref)
+    # start at the goal: only nodes appended by an iteration are goal-tested (KGMT.cu:342), so
+    # the plan ends at the first child within the threshold, not at the root
+    empty = np.zeros((0, 6), np.float32)
+    ref = orc.plan(q.cfg, q.start, q.start, empty)
+    res = g.plan(q.start, q.start, empty, 0)
+    assert res.success and res.goal_index > 0 and res.iterations == 1
+    _same_plan(g, res, ref)
+    # a single iteration
+    one = q.cfg.replace(max_iter=1)
+    ref = orc.plan(one, q.start, q.goal, q.obstacles)
+    g1 = KGMT(one)
+    res = g1.plan(q.start, q.goal, q.obstacles, len(q.obstacles))
+    assert res.iterations == 1
+    _same_plan(g1, res, ref)
+    # the same queries in one batch, beside an ordinary one
+    qs = [W.Query(q.cfg, q.start, q.goal, box), W.Query(q.cfg, q.start, q.start, empty),
+          W.Query(one, q.start, q.goal, q.obstacles), q]
+    b = BatchKGMT([x.cfg for x in qs])
+    results, _ = b.plan(*W.pack_batch(qs)[1:])
+    for i, x in enumerate(qs):
+        ref = orc.plan(x.cfg, x.start, x.goal, x.obstacles)
+        st, r = ref["stats"], results[i]
+        assert (r.iterations, r.tree_size, r.goal_index if r.success else -1) == \
+            (st["iterations"], st["tree_size"], st["goal_index"]), i
+        s, p, c = b.tree(i)
+        assert np.array_equal(p, ref["parents"]), i
+        assert np.array_equal(s.view(np.uint32), ref["samples"].view(np.uint32)), i
+        if r.success:
+            assert np.array_equal(b.path(i).view(np.uint32), ref["path"].view(np.uint32)), i
+
+
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_batch_matches_oracle(use_graph, orc):
     qs = W.make_batch(12, seed=3, max_iter=25)
