@@ -138,28 +138,62 @@ __device__ __forceinline__ float lane_down(float v)
 // levels pair lanes l and l + st within the wave. Every level adds the same
 // two partial sums as the plain tree, so the total has the same bits.
 template <int NT>
+struct ScoreInputs
+{
+    static constexpr int KR = kMaxRegions / NT;
+    int32_t vc[KR], tc[KR];  // valid and total children of regions t + k NT
+    uint32_t cnt[KR];        // their active subregions
+};
+
+// the inputs of score_regions_sum for thread t (regions t + k NT), loaded
+// together; a caller may issue them before it knows whether it needs them
+template <int NT>
+__device__ __forceinline__ void score_load(const QueryDesc& qd, const int32_t* __restrict__ total,
+                                           const int32_t* __restrict__ validc, const uint32_t* __restrict__ act,
+                                           ScoreInputs<NT>& in)
+{
+    const int R1     = qd.R1;
+    const uint32_t P = (uint32_t)qd.P;
+    const int t      = (int)threadIdx.x;
+#pragma unroll
+    for(int k = 0; k < ScoreInputs<NT>::KR; ++k)
+        {
+            const int r = t + k * NT;
+            in.vc[k]    = r < R1 ? validc[r] : 0;
+            in.tc[k]    = r < R1 ? total[r] : 0;
+            in.cnt[k]   = r < R1 ? count_bits(act, (uint32_t)r * P, P) : 0u;
+        }
+}
+
+template <int NT>
+__device__ void score_regions_sum(const QueryDesc& qd, const ScoreInputs<NT>& in, float* __restrict__ score,
+                                  float* a);
+
+template <int NT>
 __device__ void score_regions(const QueryDesc& qd, const int32_t* __restrict__ total,
                               const int32_t* __restrict__ validc, const uint32_t* __restrict__ act,
                               float* __restrict__ score, float* a)
 {
+    ScoreInputs<NT> in;
+    score_load<NT>(qd, total, validc, act, in);
+    score_regions_sum<NT>(qd, in, score, a);
+}
+
+template <int NT>
+__device__ void score_regions_sum(const QueryDesc& qd, const ScoreInputs<NT>& in, float* __restrict__ score,
+                                  float* a)
+{
     static_assert(NT % 64 == 0 && NT <= 1024, "score block");
-    constexpr int KR = kMaxRegions / NT;
+    constexpr int KR = ScoreInputs<NT>::KR;
     const int R1     = qd.R1;
     const uint32_t P = (uint32_t)qd.P;
     const float eps  = qd.cfg.epsilon;
     int P2           = 1;
     while(P2 < R1) P2 <<= 1;
     const int t = (int)threadIdx.x;
-    int32_t vc[KR], tc[KR];
-    uint32_t cnt[KR];
-#pragma unroll
-    for(int k = 0; k < KR; ++k)
-        {
-            const int r = t + k * NT;
-            vc[k]       = r < R1 ? validc[r] : 0;
-            tc[k]       = r < R1 ? total[r] : 0;
-            cnt[k]      = r < R1 ? count_bits(act, (uint32_t)r * P, P) : 0u;
-        }
+    const int32_t* vc   = in.vc;
+    const int32_t* tc   = in.tc;
+    const uint32_t* cnt = in.cnt;
     float s0[KR], sv[KR];
 #pragma unroll
     for(int k = 0; k < KR; ++k)
@@ -679,24 +713,27 @@ __global__ __launch_bounds__(kScoreThreads) void k_score(const QueryDesc* __rest
 {
     __shared__ float a[kScoreThreads];
     __shared__ uint32_t sbuf[kScoreThreads / 64 + 1];
-    const uint32_t q = blockIdx.x;
-    QueryState& st   = sts[q];
-    // the state of this iteration, closed by the previous one: every thread
-    // reads the next-state fields, the first copies them to the current ones
+    const uint32_t q    = blockIdx.x;
+    QueryState& st      = sts[q];
+    const QueryDesc& qd = qds[q];
+    // the state of this iteration (closed by the previous one; every thread
+    // reads the next-state fields) and everything that depends only on the
+    // query's description are loaded together, before the done test: the
+    // score inputs, the first bitset word to copy, the first chunk counts
     const uint32_t done = st.doneN, nC = st.nChildrenN, tree = st.treeSizeN, itr = st.itrN;
+    const uint32_t t    = threadIdx.x;
+    const uint32_t nW   = (uint32_t)qd.R2words;
+    uint32_t* pa        = P.partA + qd.chunkOff;
+    const uint32_t aw   = t < nW ? P.actNext[qd.subOff + t] : 0u;
+    const uint32_t cRaw = t < qd.nChunks ? pa[t] : 0u;
+    ScoreInputs<kScoreThreads> in;
+    score_load<kScoreThreads>(qd, P.total + qd.regOff, P.validc + qd.regOff, P.actNext + qd.subOff, in);
+    // the first thread copies the next-state fields to the current ones
     if(threadIdx.x == 0) commit_state(st);
     if(done) return;
-    const QueryDesc& qd = qds[q];
-    // loads issued with the score inputs: the first bitset word to copy and
-    // the first chunk counts to scan
-    const uint32_t t   = threadIdx.x;
     const uint32_t nCC = (nC + kChunk - 1) / kChunk;
-    const uint32_t nW  = (uint32_t)qd.R2words;
-    uint32_t* pa       = P.partA + qd.chunkOff;
-    const uint32_t aw  = t < nW ? P.actNext[qd.subOff + t] : 0u;
-    const uint32_t c0  = t < nCC ? pa[t] : 0u;
-    score_regions<kScoreThreads>(qd, P.total + qd.regOff, P.validc + qd.regOff, P.actNext + qd.subOff,
-                                 P.score + qd.regOff, a);
+    const uint32_t c0  = t < nCC ? cRaw : 0u;
+    score_regions_sum<kScoreThreads>(qd, in, P.score + qd.regOff, a);
     if(t < nW) P.actPrev[qd.subOff + t] = aw;
     for(uint32_t w = t + kScoreThreads; w < nW; w += kScoreThreads) P.actPrev[qd.subOff + w] = P.actNext[qd.subOff + w];
     // exclusive scan of the chunk counts in place
