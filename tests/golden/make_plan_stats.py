@@ -1,7 +1,10 @@
 """Writes tests/golden/plan_stats_<workload>.json: the CPU oracle's planning
 outcome for every seed of the statistical parity sets (50 seeds of the 6D
 double integrator, 200 of the 5D Dubins airplane, 200 of the 12D
-quadcopter, 200 of the planar unicycle), one record per seed:
+quadcopter, 200 of the planar unicycle, and the 512 queries of the mixed
+double-integrator / quadcopter batch workload, workloads.make_batch(512,
+seed=0): capacities log-uniform in [2^12, 2^17], 10..500 boxes), one record
+per seed (per query for the batch):
 
   [seed, iterations, tree_size, goal_index, path_rows, path_cost bits,
    path_length bits, expanded]
@@ -26,7 +29,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-SEEDS = {"di6d": 50, "dubins5d": 200, "quad12d": 200, "unicycle2d": 200}
+SEEDS = {"di6d": 50, "dubins5d": 200, "quad12d": 200, "unicycle2d": 200, "batch512": 512}
+
+
+def queries(name, n):
+    from kgmt_amd import workloads as W
+    if name == "batch512":
+        return W.make_batch(n, seed=0)
+    return [W.make_query(name, s) for s in range(n)]
 
 
 def f32_bits(x: float) -> int:
@@ -44,7 +54,7 @@ def main(argv=None):
 
     for name in a.workloads:
         n = SEEDS[name]
-        qs = [W.make_query(name, s) for s in range(n)]
+        qs = queries(name, n)
         partial = os.path.join(HERE, f"plan_stats_{name}.partial.json")
         rows = []
         if os.path.exists(partial):
@@ -64,7 +74,8 @@ def main(argv=None):
                 json.dump(rows, f)
             print(f"{name}: {len(rows)}/{n} seeds, {time.perf_counter() - t0:.0f} s", flush=True)
         dt = time.perf_counter() - t0
-        out = {"workload": name, "seeds": n, "max_iter": qs[0].cfg.max_iter, "capacity": qs[0].cfg.capacity,
+        out = {"workload": name, "seeds": n, "max_iter": qs[0].cfg.max_iter,
+               "capacity": qs[0].cfg.capacity if name != "batch512" else [q.cfg.capacity for q in qs],
                "fields": ["seed", "iterations", "tree_size", "goal_index", "path_rows", "path_cost_bits",
                           "path_length_bits", "expanded"],
                "records": rows}

@@ -1,7 +1,9 @@
 """Planning outcomes over many seeds on the GPU against the oracle's records
 (tests/golden/plan_stats_<workload>.json, written by make_plan_stats.py):
 50 seeds of the 6D double integrator, 200 of the 5D Dubins airplane, 200 of
-the 12D quadcopter and 200 of the planar unicycle at their full sizes.
+the 12D quadcopter and 200 of the planar unicycle at their full sizes, and
+the full mixed batch workload (512 queries, double integrator / quadcopter,
+capacities log-uniform in [2^12, 2^17], 10..500 boxes each).
 
 Every seed must match exactly (iterations, tree size, goal node, path rows,
 path cost and length bits, expanded children); on top of that the two
@@ -84,7 +86,7 @@ def test_plain_planner_over_seeds(name):
     _compare(name, _records(results), ref)
 
 
-@pytest.mark.parametrize("name", ["di6d", "dubins5d", "unicycle2d"])
+@pytest.mark.parametrize("name", ["di6d", "dubins5d", "quad12d", "unicycle2d"])
 def test_batched_planner_over_seeds(name):
     ref = _load(name)
     n = ref["seeds"]
@@ -93,3 +95,16 @@ def test_batched_planner_over_seeds(name):
     b = BatchKGMT([q.cfg for q in qs])
     results, _ = b.plan(*W.pack_batch(qs)[1:], use_graph=True)
     _compare(name, _records(results), ref)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_full_mixed_batch(use_graph):
+    """All 512 queries of the mixed batch workload in one batch, every query
+    identical to the oracle's record of it."""
+    ref = _load("batch512")
+    qs = W.make_batch(ref["seeds"], seed=0)
+    assert [q.cfg.capacity for q in qs] == ref["capacity"]
+    print(f"mixed batch: {len(qs)} queries, graph={use_graph}", flush=True)
+    b = BatchKGMT([q.cfg for q in qs])
+    results, _ = b.plan(*W.pack_batch(qs)[1:], use_graph=use_graph)
+    _compare("batch512", _records(results), ref)
