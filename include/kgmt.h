@@ -288,6 +288,35 @@ int kgmt_propagate_collide(const kgmt_config* cfg, const int32_t* d_parent_idx,
                            const float* d_controls, const int32_t* d_steps, uint32_t n_children,
                            const float* d_obstacles, uint32_t n_obs, float* d_child_samples,
                            uint8_t* d_valid, int32_t* d_r1, int32_t* d_r2, void* stream);
+/* Checks the planner's propagate path can decide a child's verdict with
+ * (kgmt_propagate_collide_planner reports which one ran). */
+enum {
+    KGMT_CHECK_SCENE      = 0, /* every step against every box of the scene          */
+    KGMT_CHECK_CANDIDATES = 1, /* two-phase: the wave's candidate list, each lane its  */
+                               /* own candidates inside its edge box                  */
+    KGMT_CHECK_DRY_RUN    = 2, /* two-phase: no box near the wave, the dry run decides */
+    KGMT_CHECK_ONE_PASS   = 3  /* quadcopter: one pass against every box             */
+};
+/* Propagate + collide through the planner's own per-child path, for checking
+ * it on identical inputs: children in waves of 64 consecutive indices, sorted
+ * within the wave by step count (longest first) as the planner's propagate
+ * kernel sorts them, then the check the planner runs for the model: the
+ * two-phase check for the double integrator, both Dubins models and the
+ * unicycle (an edge box from a dry run, or for the Dubins models from an
+ * interval bound of the edge; the wave's candidate boxes = boxes meeting some
+ * lane's edge box; each lane tests only its own candidates; the whole scene
+ * when a box is not finite or the candidates exceed 128), the one-pass check
+ * for the quadcopter. Same arguments and outputs as kgmt_propagate_collide
+ * (isMotionValid, collisionCheck.cu:49-62, per step), plus two optional
+ * outputs (NULL: not written): d_check[n], the KGMT_CHECK_* that decided
+ * each child, and d_edge_box [6][n], each child's edge box (lo xyz, hi xyz;
+ * not written for the quadcopter). */
+int kgmt_propagate_collide_planner(const kgmt_config* cfg, const int32_t* d_parent_idx,
+                                   const float* d_parent_states, uint32_t n_parents,
+                                   const float* d_controls, const int32_t* d_steps, uint32_t n_children,
+                                   const float* d_obstacles, uint32_t n_obs, float* d_child_samples,
+                                   uint8_t* d_valid, int32_t* d_r1, int32_t* d_r2, uint8_t* d_check,
+                                   float* d_edge_box, void* stream);
 /* Region bookkeeping of one iteration: total[r1]++ for every child,
  * valid[r1]++ and active_next[r2]=1 for valid ones, and
  * fnext[i] = valid && (ufront[i] < score[r1] || !active_prev[r2]).

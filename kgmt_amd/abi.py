@@ -25,6 +25,11 @@ KGMT_MODEL_DUBINS_5D = 4
 KGMT_DI_REFERENCE = 0
 KGMT_DI_EXACT = 1
 
+KGMT_CHECK_SCENE = 0
+KGMT_CHECK_CANDIDATES = 1
+KGMT_CHECK_DRY_RUN = 2
+KGMT_CHECK_ONE_PASS = 3
+
 KGMT_PRESET_REFERENCE_QUAD = 0
 KGMT_PRESET_DI6D = 1
 KGMT_PRESET_DUBINS5D = 2

@@ -874,11 +874,22 @@ __device__ __forceinline__ float wave_maxf(float v)
     return fmaxf(fmaxf(readlane_f(v, 0), readlane_f(v, 16)), fmaxf(readlane_f(v, 32), readlane_f(v, 48)));
 }
 
+// What propagate_two_phase did for one lane: the check that decided the
+// verdict (KGMT_CHECK_* of kgmt.h) and the lane's edge box (the box holding
+// every segment the edge tests). Only the stage-level entry asks for it; the
+// planner passes no trace and the recording folds away.
+struct TwoPhaseTrace
+{
+    int path;
+    float lo[3], hi[3];
+};
+
 // Called by every lane of the wave; lanes without a child (has == false)
 // pass steps = 0 and take part in the wave-wide steps with an empty box.
 template <int MODEL>
 __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* s, const float* u, int steps,
-                                                    const Scene& sc, float4* cand, bool has)
+                                                    const Scene& sc, float4* cand, bool has,
+                                                    TwoPhaseTrace* trace = nullptr)
 {
     constexpr int S      = ModelDims<MODEL>::S;
     constexpr bool BOUND = MODEL == KGMT_MODEL_DUBINS_5D || MODEL == KGMT_MODEL_DUBINS_AIRPLANE;
@@ -899,6 +910,11 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
 #pragma unroll
             for(int d = 0; d < S; ++d) s1[d] = s[d];
             dryOk = propagate_with<MODEL>(qd, s1, u, steps, dry);
+        }
+    if(trace)
+        {
+#pragma unroll
+            for(int d = 0; d < 3; ++d) trace->lo[d] = dry.lo[d], trace->hi[d] = dry.hi[d];
         }
     if(!has)
         {
@@ -956,16 +972,19 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
         n = kCand + 1;
     if(n > kCand)
         {
+            if(trace) trace->path = KGMT_CHECK_SCENE;
             FullCheck<1> chk{sc};
             return propagate_with<MODEL>(qd, s, u, steps, chk);
         }
     if(!BOUND && n == 0)
         {
             // no box near any lane: the dry run is the run
+            if(trace) trace->path = KGMT_CHECK_DRY_RUN;
 #pragma unroll
             for(int d = 0; d < S; ++d) s[d] = s1[d];
             return dryOk;
         }
+    if(trace) trace->path = KGMT_CHECK_CANDIDATES;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -1117,6 +1117,35 @@ int kgmt_propagate_collide(const kgmt_config* cfg, const int32_t* d_parent_idx, 
     return KGMT_OK;
 }
 
+int kgmt_propagate_collide_planner(const kgmt_config* cfg, const int32_t* d_parent_idx,
+                                   const float* d_parent_states, uint32_t n_parents, const float* d_controls,
+                                   const int32_t* d_steps, uint32_t n_children, const float* d_obstacles,
+                                   uint32_t n_obs, float* d_child_samples, uint8_t* d_valid, int32_t* d_r1,
+                                   int32_t* d_r2, uint8_t* d_check, float* d_edge_box, void* stream)
+{
+    if(n_children && (!d_parent_idx || !d_parent_states || !d_controls || !d_steps || !d_child_samples || !d_valid ||
+                      !d_r1 || !d_r2))
+        return fail(KGMT_E_ARG, "null buffer");
+    if(n_obs > kMaxObs) return fail(KGMT_E_CAPACITY, "more than 1500 obstacles");
+    if(n_obs && !d_obstacles) return fail(KGMT_E_ARG, "obstacles pointer is null");
+    StageDesc sd((hipStream_t)stream);
+    int rc = sd.make(cfg);
+    if(rc) return rc;
+    if(n_obs && (rc = check_device_rows(d_obstacles))) return rc;
+    const uint32_t nGrp = scene_groups(n_obs);
+    float4* d_scene     = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&d_scene, (size_t)std::max<uint32_t>(scene_pairs(n_obs), 1) * 2 * sizeof(float4),
+                           sd.s));
+    hipError_t e = launch_build_scene(d_obstacles, n_obs, nGrp, d_scene, model_wdim(cfg->model), sd.s);
+    if(e == hipSuccess)
+        e = launch_stage_propagate_planner(cfg->model, sd.d, d_parent_idx, d_parent_states, n_parents, d_controls,
+                                           d_steps, n_children, d_scene, nGrp, d_child_samples, d_valid, d_r1, d_r2,
+                                           d_check, d_edge_box, sd.s);
+    (void)hipFreeAsync(d_scene, sd.s);
+    if(e != hipSuccess) return fail(KGMT_E_HIP, std::string("propagate: ") + hipGetErrorString(e));
+    return KGMT_OK;
+}
+
 int kgmt_region_update(const kgmt_config* cfg, const int32_t* d_r1, const int32_t* d_r2, const uint8_t* d_valid,
                        const float* d_ufront, uint32_t n, const float* d_score, const uint32_t* d_active_prev,
                        uint32_t* d_active_next, int32_t* d_total, int32_t* d_valid_count, uint8_t* d_fnext,

@@ -255,12 +255,17 @@ __device__ void score_regions_sum(const QueryDesc& qd, const ScoreInputs<NT>& in
         }
 }
 
+// fAccept (KGMT.cu:387-388): the host computes (itr * EPSILON) * pow(a, 5)
+// with a and the first factor in float and pow's double result, and stores
+// the product as float; here the fifth power is (a^2)^2 * a in double (a^2
+// is exact) and the product is rounded to float once
 __device__ __forceinline__ float faccept(const QueryDesc& qd, uint32_t itr, uint32_t tree, uint32_t next)
 {
-    const float a  = 1.0f - (float)(tree + next) / (float)qd.cap;
-    const float a2 = a * a;
-    const float a5 = (a2 * a2) * a;
-    return ((float)itr * qd.cfg.epsilon) * a5;
+    const float a   = 1.0f - (float)(tree + next) / (float)qd.cap;
+    const double ad = (double)a;
+    const double a2 = ad * ad;
+    const double a5 = (a2 * a2) * ad;
+    return (float)((double)((float)itr * qd.cfg.epsilon) * a5);
 }
 
 // children per expansion entry and child count (KGMT.cu:183-201, Q9)
@@ -1430,6 +1435,93 @@ __global__ __launch_bounds__(kBlock) void k_stage_propagate(const QueryDesc* __r
     r2o[i]       = subregion_of(qd, s[0], s[1], s[2], r1, ix, iy, iz);
 }
 
+// The planner's per-child path on caller buffers: children in waves of 64
+// consecutive indices, handed to the wave's lanes longest-first by the same
+// counting sort child_tile runs with one lane per child, then
+// propagate_two_phase called by every lane of the wave as child_tile calls it
+// (lanes without a child take part with an empty box), or the one-pass check
+// for the quadcopter. check / box (optional) receive the check that decided
+// each child and its edge box.
+template <int MODEL>
+__global__ __launch_bounds__(kBlock) void k_stage_propagate_planner(
+  const QueryDesc* __restrict__ qdp, const int32_t* __restrict__ parentIdx, const float* __restrict__ parentStates,
+  uint32_t nParents, const float* __restrict__ controls, const int32_t* __restrict__ steps, uint32_t n,
+  const float4* __restrict__ obs, uint32_t nGrp, float* child, uint8_t* valid, int32_t* r1o, int32_t* r2o,
+  uint8_t* checkOut, float* boxOut)
+{
+    constexpr int S = ModelDims<MODEL>::S, C = ModelDims<MODEL>::C, D = S + C + 1;
+    extern __shared__ float4 sObs[];
+    __shared__ float4 cand[kBlock / 64][2 * kCand];
+    __shared__ uint32_t hist[kBlock];
+    __shared__ uint16_t slot[kBlock];
+    const QueryDesc& qd = *qdp;
+    stage_scene(obs, 0, nGrp * (1 + kGroup), sObs);
+    __syncthreads();
+    const uint32_t t    = threadIdx.x;
+    const uint32_t base = blockIdx.x * kBlock;
+    // counting sort of the wave's 64 children by step count, longest first
+    // (bucket 63 of the wave: lanes past the end)
+    {
+        const uint32_t g0 = (t / 64u) * 64u;
+        uint32_t bucket   = g0 + 63u;
+        if(base + t < n)
+            {
+                int32_t k = steps[base + t] - qd.cfg.steps_min;
+                k         = k < 0 ? 0 : (k > 62 ? 62 : k);
+                bucket    = g0 + 62u - (uint32_t)k;
+            }
+        hist[t] = 0;
+        wave_sync();
+        const uint32_t rank = atomicAdd(&hist[bucket], 1u);
+        wave_sync();
+        const uint32_t h = hist[t];
+        hist[t]          = g0 + wave_incl_scan(h) - h;
+        wave_sync();
+        slot[hist[bucket] + rank] = (uint16_t)t;
+        wave_sync();
+    }
+    const uint32_t i = base + slot[t];
+    const int32_t p  = i < n ? parentIdx[i] : -1;
+    const bool has   = i < n && p >= 0 && (uint32_t)p < nParents;
+    float s[S], uc[C];
+    int32_t st = 0;
+#pragma unroll
+    for(int d = 0; d < S; ++d) s[d] = has ? parentStates[(size_t)d * nParents + p] : 0.0f;
+#pragma unroll
+    for(int k = 0; k < C; ++k) uc[k] = has ? controls[(size_t)k * n + i] : 0.0f;
+    if(has) st = steps[i];
+    const Scene sc{sObs, (int)nGrp, 0};
+    bool ok = false;
+    TwoPhaseTrace tr{KGMT_CHECK_ONE_PASS, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+    if constexpr(MODEL != KGMT_MODEL_QUADCOPTER)
+        ok = propagate_two_phase<MODEL>(qd, s, uc, st, sc, cand[t / 64], has, &tr);  // every lane
+    else if(has)
+        ok = propagate<MODEL>(qd, s, uc, st, sc);
+    if(i >= n) return;
+    if(!has)
+        {
+            // caller index out of range: no state is read; flagged by r1 = r2 = -1
+            for(int d = 0; d < D; ++d) child[(size_t)d * n + i] = 0.0f;
+            valid[i] = 0;
+            r1o[i]   = -1;
+            r2o[i]   = -1;
+            return;
+        }
+#pragma unroll
+    for(int d = 0; d < S; ++d) child[(size_t)d * n + i] = s[d];
+#pragma unroll
+    for(int k = 0; k < C; ++k) child[(size_t)(S + k) * n + i] = uc[k];
+    child[(size_t)(D - 1) * n + i] = qd.cfg.dt * (float)st;
+    int ix, iy, iz;
+    const int r1 = region_of(qd, s[0], s[1], s[2], ix, iy, iz);
+    valid[i]     = ok ? 1 : 0;
+    r1o[i]       = r1;
+    r2o[i]       = subregion_of(qd, s[0], s[1], s[2], r1, ix, iy, iz);
+    if(checkOut) checkOut[i] = (uint8_t)tr.path;
+    if(boxOut && MODEL != KGMT_MODEL_QUADCOPTER)
+        for(int d = 0; d < 3; ++d) boxOut[(size_t)d * n + i] = tr.lo[d], boxOut[(size_t)(3 + d) * n + i] = tr.hi[d];
+}
+
 __global__ __launch_bounds__(kBlock) void k_stage_region_update(const QueryDesc* __restrict__ qdp,
                                                                 const int32_t* __restrict__ r1,
                                                                 const int32_t* __restrict__ r2,
@@ -1676,6 +1768,31 @@ hipError_t launch_stage_propagate(int model, const QueryDesc* qd, const int32_t*
 #define KGMT_STAGE_PROP(M)                                                                                     \
     hipLaunchKernelGGL(k_stage_propagate<M>, g, b, lds, s, qd, parentIdx, parentStates, nParents, controls, steps, \
                        n, obs, nGrp, nObs, child, valid, r1, r2)
+    switch(model)
+        {
+            case KGMT_MODEL_DOUBLE_INTEGRATOR: KGMT_STAGE_PROP(KGMT_MODEL_DOUBLE_INTEGRATOR); break;
+            case KGMT_MODEL_DUBINS_AIRPLANE: KGMT_STAGE_PROP(KGMT_MODEL_DUBINS_AIRPLANE); break;
+            case KGMT_MODEL_DUBINS_5D: KGMT_STAGE_PROP(KGMT_MODEL_DUBINS_5D); break;
+            case KGMT_MODEL_QUADCOPTER: KGMT_STAGE_PROP(KGMT_MODEL_QUADCOPTER); break;
+            case KGMT_MODEL_UNICYCLE: KGMT_STAGE_PROP(KGMT_MODEL_UNICYCLE); break;
+            default: return hipErrorInvalidValue;
+        }
+#undef KGMT_STAGE_PROP
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_propagate_planner(int model, const QueryDesc* qd, const int32_t* parentIdx,
+                                          const float* parentStates, uint32_t nParents, const float* controls,
+                                          const int32_t* steps, uint32_t n, const float4* obs, uint32_t nGrp,
+                                          float* child, uint8_t* valid, int32_t* r1, int32_t* r2, uint8_t* check,
+                                          float* box, hipStream_t s)
+{
+    if(n == 0) return hipSuccess;
+    const dim3 g((n + kBlock - 1) / kBlock), b(kBlock);
+    const size_t lds = (size_t)(nGrp ? nGrp * (1 + kGroup) : 1) * 2 * sizeof(float4);
+#define KGMT_STAGE_PROP(M)                                                                                        \
+    hipLaunchKernelGGL(k_stage_propagate_planner<M>, g, b, lds, s, qd, parentIdx, parentStates, nParents, controls, \
+                       steps, n, obs, nGrp, child, valid, r1, r2, check, box)
     switch(model)
         {
             case KGMT_MODEL_DOUBLE_INTEGRATOR: KGMT_STAGE_PROP(KGMT_MODEL_DOUBLE_INTEGRATOR); break;

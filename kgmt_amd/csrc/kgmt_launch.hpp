@@ -53,6 +53,12 @@ hipError_t launch_stage_propagate(int model, const QueryDesc* qd, const int32_t*
                                   uint32_t nParents, const float* controls, const int32_t* steps, uint32_t n,
                                   const float4* obs, uint32_t nGrp, uint32_t nObs, float* child, uint8_t* valid,
                                   int32_t* r1, int32_t* r2, hipStream_t s);
+// the planner's per-child path (wave sort, two-phase or one-pass check)
+hipError_t launch_stage_propagate_planner(int model, const QueryDesc* qd, const int32_t* parentIdx,
+                                          const float* parentStates, uint32_t nParents, const float* controls,
+                                          const int32_t* steps, uint32_t n, const float4* obs, uint32_t nGrp,
+                                          float* child, uint8_t* valid, int32_t* r1, int32_t* r2, uint8_t* check,
+                                          float* box, hipStream_t s);
 hipError_t launch_stage_region_update(const QueryDesc* qd, const int32_t* r1, const int32_t* r2, const uint8_t* valid,
                                       const float* ufront, uint32_t n, const float* score, const uint32_t* actPrev,
                                       uint32_t* actNext, int32_t* total, int32_t* validc, uint8_t* fnext,

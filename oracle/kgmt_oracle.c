@@ -782,13 +782,18 @@ void orc_score(const kgmt_config* cfg, const int32_t* total, const int32_t* vali
     free(a);
 }
 
-/* KGMT.cu:387-388 with the clamped accepted count (Q9). */
+/* KGMT.cu:387-388 with the clamped accepted count (Q9). The reference's
+ * host computes (h_itr_ * EPSILON) in float times pow(treeAddSize, 5) in
+ * double (std::pow of a float promotes to double) and stores the product as
+ * float: here the fifth power is (a^2)^2 * a in double (a^2 is exact) and the
+ * product is rounded to float once. */
 float orc_faccept(const kgmt_config* cfg, uint32_t itr, uint32_t tree_size, uint32_t next)
 {
-    float a  = 1.0f - (float)(tree_size + next) / (float)cfg->capacity;
-    float a2 = a * a;
-    float a5 = (a2 * a2) * a;
-    return ((float)itr * cfg->epsilon) * a5;
+    float a   = 1.0f - (float)(tree_size + next) / (float)cfg->capacity;
+    double ad = (double)a;
+    double a2 = ad * ad;
+    double a5 = (a2 * a2) * ad;
+    return (float)((double)((float)itr * cfg->epsilon) * a5);
 }
 
 /* ------------------------------------------------------------------------ */

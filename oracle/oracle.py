@@ -72,17 +72,23 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def load(path: str):
+    """A build of the oracle at `path` with every function's signature set
+    (the tests also load deliberately altered builds this way)."""
+    h = C.CDLL(path)
+    for name, (res, args) in _SIG.items():
+        f = getattr(h, name)
+        f.restype = res
+        f.argtypes = args
+    return h
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        h = C.CDLL(LIB_PATH)
-        for name, (res, args) in _SIG.items():
-            f = getattr(h, name)
-            f.restype = res
-            f.argtypes = args
-        _lib = h
+        _lib = load(LIB_PATH)
     return _lib
 
 
@@ -227,8 +233,9 @@ def faccept(cfg, itr, tree, nxt) -> float:
     return lib().orc_faccept(C.byref(cfg), itr, tree, nxt)
 
 
-def plan(cfg: KgmtConfig, start, goal, obs, log_iters: bool = False):
-    """Full planner on the CPU. Returns dict with stats, tree, path, regions."""
+def plan(cfg: KgmtConfig, start, goal, obs, log_iters: bool = False, lib_=None):
+    """Full planner on the CPU. Returns dict with stats, tree, path, regions.
+    lib_: another build of the oracle (from load()) to run instead."""
     S, Cd, D = dims(cfg.model)
     N = cfg.capacity
     samples = np.zeros((N, D), np.float32)
@@ -246,7 +253,7 @@ def plan(cfg: KgmtConfig, start, goal, obs, log_iters: bool = False):
     g = np.zeros(KGMT_MAX_SAMPLE, np.float32)
     g[:D] = np.asarray(goal, np.float32)[:D]
     ob = _c(obs, np.float32).reshape(-1)
-    rc = lib().orc_plan(C.byref(cfg), _p(s), _p(g), _p(ob) if ob.size else None, ob.size // (2 * cfg.wdim), _p(samples),
+    rc = (lib_ or lib()).orc_plan(C.byref(cfg), _p(s), _p(g), _p(ob) if ob.size else None, ob.size // (2 * cfg.wdim), _p(samples),
                         _p(parents), _p(costs), _p(path), _p(total), _p(valid), _p(score_),
                         _p(log) if log is not None else None, C.byref(st))
     assert rc == 0, rc

@@ -385,11 +385,29 @@ def test_score_closed_form(orc):
         assert np.allclose(got, want, rtol=2e-6, atol=0)
 
 
+def reference_faccept(cfg, itr, tree, nxt):
+    """KGMT.cu:387-388 as the reference's host evaluates it: treeAddSize and
+    h_itr_ * EPSILON in float, pow(treeAddSize, 5) in double (std::pow of a
+    float promotes), the product in double, stored into a float."""
+    f32 = np.float32
+    a = f32(1) - f32(tree + nxt) / f32(cfg.capacity)
+    first = f32(itr) * f32(cfg.epsilon)
+    return f32(np.float64(first) * np.power(np.float64(a), 5.0))
+
+
 def test_faccept_closed_form(orc):
     cfg = W.config("di6d")
-    for itr, tree, nxt in [(1, 1, 100), (10, 5000, 300), (300, 100000, 31072), (7, 0, 0)]:
+    rng = np.random.default_rng(11)
+    cases = [(1, 1, 100), (10, 5000, 300), (300, 100000, 31072), (7, 0, 0), (1, cfg.capacity - 1, 1)]
+    cases += [(int(i), int(t), int(n)) for i, t, n in zip(rng.integers(1, 301, 5000),
+                                                          rng.integers(0, cfg.capacity // 2, 5000),
+                                                          rng.integers(0, cfg.capacity // 2, 5000))]
+    for itr, tree, nxt in cases:
+        got = np.float32(orc.faccept(cfg, itr, tree, nxt))
         want = itr * cfg.epsilon * (1 - (tree + nxt) / cfg.capacity) ** 5
-        assert abs(orc.faccept(cfg, itr, tree, nxt) - want) <= 1e-6 * max(1.0, abs(want))
+        assert abs(got - want) <= 1e-6 * max(1.0, abs(want))
+        # bit for bit the reference's float/double evaluation
+        assert got.view(np.uint32) == reference_faccept(cfg, itr, tree, nxt).view(np.uint32), (itr, tree, nxt)
 
 
 def test_region_update_rules(orc):
