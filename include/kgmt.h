@@ -31,6 +31,13 @@
  *                          383-385; src/planners/Planner.cu:52-58
  *   kgmt_repeat            thrust::exclusive_scan + repeatInd src/planners/KGMT.cu:176-181;
  *                          src/planners/Planner.cu:68-80
+ *   kgmt_frontier_update   KGMT::updateFrontier + updateFrontier_kernel
+ *                          src/planners/KGMT.cu:305-403
+ *   kgmt_propagate_collide_planner  as kgmt_propagate_collide, through the
+ *                          planner's own per-wave check
+ *   kgmt_get_tree_device   public device members d_treeSamples_ptr_,
+ *                          d_treeSamplesParentIdxs_ptr_, d_treeSampleCosts_ptr_
+ *                          include/planners/Planner.cuh:37-43
  *   kgmt_batch_*           no single counterpart: many KGMT::plan calls served by
  *                          one launch sequence per iteration.
  */
@@ -225,6 +232,23 @@ int kgmt_plan(kgmt_ctx* ctx, const float* start, const float* goal, const float*
  * cap: rows available; *n receives the tree size. */
 int kgmt_get_tree(kgmt_ctx* ctx, float* samples, int32_t* parents, float* costs,
                   uint32_t cap, uint32_t* n);
+/* The tree of the last plan in device memory, in the reference's layout
+ * (public members d_treeSamples_ (rows of D floats), d_treeSamplesParentIdxs_
+ * and d_treeSampleCosts_, include/planners/Planner.cuh:37-43): library-owned
+ * buffers of `capacity` entries, entries past tree_size 0, -1 and 0 as the
+ * reference's fills leave them (KGMT.cu:62-64). The rows are written by a
+ * device kernel on the first call after a plan; the call returns when they are
+ * complete, and they stay valid until the next plan or kgmt_destroy. */
+typedef struct kgmt_device_tree {
+    float*   samples;    /* [capacity][D] */
+    int32_t* parents;    /* [capacity], -1 for the root */
+    float*   costs;      /* [capacity], distance of the node position to the goal */
+    uint32_t tree_size;
+    uint32_t capacity;
+    int32_t  sample_dim; /* D */
+    uint32_t reserved;
+} kgmt_device_tree;
+int kgmt_get_tree_device(kgmt_ctx* ctx, kgmt_device_tree* out);
 /* Rows goal->root of the last successful plan (D floats each). */
 int kgmt_get_path(kgmt_ctx* ctx, float* rows, uint32_t cap_rows, uint32_t* n_rows);
 /* Per-region counters of the last plan: total[R1], valid[R1], score[R1]. */
@@ -317,6 +341,28 @@ int kgmt_propagate_collide_planner(const kgmt_config* cfg, const int32_t* d_pare
                                    const float* d_obstacles, uint32_t n_obs, float* d_child_samples,
                                    uint8_t* d_valid, int32_t* d_r1, int32_t* d_r2, uint8_t* d_check,
                                    float* d_edge_box, void* stream);
+/* Frontier update of one iteration (updateFrontier + updateFrontier_kernel,
+ * KGMT.cu:305-403) through the planner's own append and expand kernels, on
+ * caller buffers (device memory, capacity = cfg->capacity):
+ *   tree: samples [D][capacity] (SoA), parents, costs [capacity], frontier
+ *   flags [capacity] (u8); the first tree_size entries are the tree;
+ *   the iteration's children: samples [D][n_children], parent (tree index),
+ *   next-frontier flag (u8) [n_children];
+ *   region valid counts and scores [R1] after the iteration's propagate;
+ *   goal: host array of W position floats; itr: the iteration (1-based).
+ * Writes: the flagged children in child order, clamped to the free capacity,
+ * appended at tree_size.. (sample, parent, cost = distance to the goal,
+ * frontier 1); frontier 1 for every old node with frontier 0 whose draw U <=
+ * score[region] + fAccept; rep_idx [capacity]: the next expansion multiset
+ * (re-activated nodes ascending, then each new node repeated 15 times when
+ * its region's valid count < 10, else once), its first capacity entries;
+ * info [4] (int32): appended count, multiset size, the smallest new index
+ * within goal_thresh or -1, re-activated count. Synchronous on `stream`. */
+int kgmt_frontier_update(const kgmt_config* cfg, uint32_t itr, uint32_t tree_size, float* d_tree_samples,
+                         int32_t* d_parents, float* d_costs, uint8_t* d_frontier, const float* d_child_samples,
+                         const int32_t* d_child_parent, const uint8_t* d_fnext, uint32_t n_children,
+                         const int32_t* d_valid_count, const float* d_score, const float* goal, uint32_t* d_rep_idx,
+                         int32_t* d_info, void* stream);
 /* Region bookkeeping of one iteration: total[r1]++ for every child,
  * valid[r1]++ and active_next[r2]=1 for valid ones, and
  * fnext[i] = valid && (ufront[i] < score[r1] || !active_prev[r2]).

@@ -4,22 +4,70 @@
 // Mirrors include/planners/Planner.cuh:19-44 and include/planners/KGMT.cuh:5-36:
 // same class names, same plan(float*, float*, float*, uint) signature taking a
 // DEVICE pointer to the obstacle rows, same public result members
-// (h_treeSize_, h_itr_, h_pathToGoal_, h_controlPathToGoal_, h_costToGoal_).
+// (h_treeSize_, h_itr_, h_pathToGoal_, h_costToGoal_, h_controlPathToGoal_
+// as a float* the object owns, d_treeSamples_ptr_, d_treeSamplesParentIdxs_ptr_,
+// d_treeSampleCosts_ptr_ in device memory), plus the helpers a reference
+// caller uses: readObstaclesFromCSV (src/helper/helper.cu:13-39) and the
+// dimension macros of config.h (STATE_DIM, CONTROL_DIM, SAMPLE_DIM, W_DIM).
 // Differences a caller can see:
 //   * the configuration is a runtime kgmt_config (config.h's macros); the
-//     default constructor uses config.h's quadcopter block;
-//   * failures throw kgmt::Error instead of continuing silently;
+//     default constructor uses the preset named by KGMT_DROPIN_PRESET (0, the
+//     reference's quadcopter block of config.h:209-307, unless defined before
+//     this header), and the dimension macros follow the same preset;
+//   * failures throw kgmt::Error instead of continuing silently (the
+//     reference's CSV reader calls exit(1); an uncaught kgmt::Error also ends
+//     the program with an error);
 //   * h_controlPathToGoal_ holds h_pathRows_ rows (the exact path depth);
-//   * tree arrays are copied to the host on request (treeSamples() ...),
-//     the device buffers are owned by the library.
+//   * the device tree members are raw pointers into library-owned buffers
+//     (there are no thrust vectors), valid after plan() until the next plan;
+//     setDeviceTree(false) skips writing them.
 #pragma once
 
 #include <cstdint>
+#include <fstream>
+#include <iostream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "kgmt.h"
+
+// config.h dimension macros of the drop-in's preset (KGMT_PRESET_* number)
+#ifndef KGMT_DROPIN_PRESET
+#define KGMT_DROPIN_PRESET 0
+#endif
+#if KGMT_DROPIN_PRESET == 0 || KGMT_DROPIN_PRESET == 3
+#define KGMT_DROPIN_STATE_DIM 12
+#define KGMT_DROPIN_CONTROL_DIM 4
+#define KGMT_DROPIN_W_DIM 3
+#elif KGMT_DROPIN_PRESET == 1
+#define KGMT_DROPIN_STATE_DIM 6
+#define KGMT_DROPIN_CONTROL_DIM 3
+#define KGMT_DROPIN_W_DIM 3
+#elif KGMT_DROPIN_PRESET == 2
+#define KGMT_DROPIN_STATE_DIM 5
+#define KGMT_DROPIN_CONTROL_DIM 3
+#define KGMT_DROPIN_W_DIM 3
+#elif KGMT_DROPIN_PRESET == 4
+#define KGMT_DROPIN_STATE_DIM 4
+#define KGMT_DROPIN_CONTROL_DIM 2
+#define KGMT_DROPIN_W_DIM 2
+#else
+#error "KGMT_DROPIN_PRESET must be a KGMT_PRESET_* number (0..4)"
+#endif
+#ifndef STATE_DIM
+#define STATE_DIM KGMT_DROPIN_STATE_DIM
+#endif
+#ifndef CONTROL_DIM
+#define CONTROL_DIM KGMT_DROPIN_CONTROL_DIM
+#endif
+#ifndef SAMPLE_DIM
+#define SAMPLE_DIM (STATE_DIM + CONTROL_DIM + 1)
+#endif
+#ifndef W_DIM
+#define W_DIM KGMT_DROPIN_W_DIM
+#endif
 
 namespace kgmt
 {
@@ -37,33 +85,70 @@ inline void check(int rc)
 
 typedef unsigned int uint;
 
+// src/helper/helper.cu:13-39: every number of the file in order (a comma
+// after a number is skipped; a line stops at the first token that is not a
+// number), numObstacles = values / (2 * workspaceDim). A file that cannot be
+// opened throws kgmt::Error (the reference prints and calls exit(1)).
+inline std::vector<float> readObstaclesFromCSV(const std::string& filename, int& numObstacles, int workspaceDim)
+{
+    std::vector<float> obstacles;
+    std::ifstream file(filename);
+    if(!file.is_open())
+        {
+            std::cerr << "Error opening file: " << filename << std::endl;
+            throw kgmt::Error(KGMT_E_ARG, "cannot open obstacle file " + filename);
+        }
+    std::string line;
+    while(std::getline(file, line))
+        {
+            std::istringstream ss(line);
+            float value;
+            while(ss >> value)
+                {
+                    obstacles.push_back(value);
+                    if(ss.peek() == ',') ss.ignore();
+                }
+        }
+    numObstacles = (int)(obstacles.size() / (size_t)(2 * workspaceDim));
+    return obstacles;
+}
+
 // include/planners/Planner.cuh:19-44
 class Planner
 {
 public:
-    virtual ~Planner() = default;
+    Planner()                          = default;
+    Planner(const Planner&)            = delete;
+    Planner& operator=(const Planner&) = delete;
+    virtual ~Planner() { delete[] h_controlPathToGoal_; }
     virtual void plan(float* h_initial, float* h_goal, float* d_obstacles_ptr, uint h_obstaclesCount) = 0;
 
+    // --- host fields ---
     uint h_treeSize_ = 0, h_itr_ = 0;
     float h_costToGoal_ = 0.0f;
     int h_pathToGoal_ = 0;
-    std::vector<float> h_controlPathToGoal_;  // rows goal -> root, SAMPLE_DIM floats each
+    float* h_controlPathToGoal_ = nullptr;  // rows goal -> root, SAMPLE_DIM floats each (h_pathRows_ of them)
     uint h_pathRows_ = 0;
+
+    // --- device fields (rows of SAMPLE_DIM floats, parent indices, costs) ---
+    float* d_treeSamples_ptr_ = nullptr;
+    int* d_treeSamplesParentIdxs_ptr_ = nullptr;
+    float* d_treeSampleCosts_ptr_ = nullptr;
 };
 
 // include/planners/KGMT.cuh:5-36
 class KGMT : public Planner
 {
 public:
-    KGMT() : KGMT(preset(KGMT_PRESET_REFERENCE_QUAD)) {}
+    KGMT() : KGMT(preset(KGMT_DROPIN_PRESET)) {}
     explicit KGMT(const kgmt_config& cfg) : cfg_(cfg)
     {
         kgmt::check(kgmt_model_dims(cfg.model, &S_, &C_, &D_));
         kgmt::check(kgmt_create(&cfg_, &ctx_));
+        // a path has at most one row per iteration plus the root
+        h_controlPathToGoal_ = new float[(size_t)D_ * (cfg_.max_iter + 1)]();
     }
     ~KGMT() override { kgmt_destroy(ctx_); }
-    KGMT(const KGMT&)            = delete;
-    KGMT& operator=(const KGMT&) = delete;
 
     static kgmt_config preset(int32_t p)
     {
@@ -74,7 +159,8 @@ public:
 
     // KGMT::plan, src/planners/KGMT.cu:41-105. d_obstacles_ptr: device memory,
     // 2 * W_DIM floats per box (xmin ymin zmin xmax ymax zmax; xmin ymin xmax
-    // ymax for the planar unicycle).
+    // ymax for the planar unicycle), complete before the call (it is read on
+    // the planner's own stream).
     void plan(float* h_initial, float* h_goal, float* d_obstacles_ptr, uint h_obstaclesCount) override
     {
         kgmt::check(kgmt_plan(ctx_, h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount, 1, &result_));
@@ -83,28 +169,40 @@ public:
         h_pathToGoal_ = result_.success ? result_.goal_index : 0;
         h_costToGoal_ = result_.success ? result_.path_cost : 0.0f;
         h_pathRows_   = result_.path_rows;
-        h_controlPathToGoal_.assign((size_t)h_pathRows_ * D_, 0.0f);
         if(h_pathRows_)
             {
                 uint32_t n = 0;
-                kgmt::check(kgmt_get_path(ctx_, h_controlPathToGoal_.data(), h_pathRows_, &n));
+                kgmt::check(kgmt_get_path(ctx_, h_controlPathToGoal_, cfg_.max_iter + 1, &n));
+            }
+        if(deviceTree_)
+            {
+                kgmt_device_tree t;
+                kgmt::check(kgmt_get_tree_device(ctx_, &t));
+                d_treeSamples_ptr_           = t.samples;
+                d_treeSamplesParentIdxs_ptr_ = t.parents;
+                d_treeSampleCosts_ptr_       = t.costs;
             }
     }
 
-    // KGMT::planBench, src/planners/KGMT.cu:107-165 (without the per-iteration
-    // CSV dump; use treeSamples()/treeParents() after the plan instead)
+    // KGMT::planBench, src/planners/KGMT.cu:107-165: plan() (the
+    // per-iteration CSV dump of the reference is kgmt_amd.export in Python)
     void planBench(float* h_initial, float* h_goal, float* d_obstacles_ptr, uint h_obstaclesCount, int /*benchItr*/)
     {
         plan(h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount);
     }
 
     void setSeed(uint64_t seed) { kgmt::check(kgmt_set_seed(ctx_, seed)); }
+    // write the device tree members after every plan (default) or not
+    void setDeviceTree(bool on)
+    {
+        deviceTree_ = on;
+        if(!on) d_treeSamples_ptr_ = nullptr, d_treeSamplesParentIdxs_ptr_ = nullptr, d_treeSampleCosts_ptr_ = nullptr;
+    }
     // plan() returns with its results complete; this also waits for the few
     // launches it leaves to drain on the planner's stream
     void synchronize() { kgmt::check(kgmt_synchronize(ctx_)); }
 
-    // host copies of d_treeSamples_ (row-major, D floats per node),
-    // d_treeSamplesParentIdxs_, d_treeSampleCosts_
+    // host copies of the tree (row-major, D floats per node), parents, costs
     std::vector<float> treeSamples() const
     {
         std::vector<float> s((size_t)h_treeSize_ * D_);
@@ -130,10 +228,12 @@ public:
     const kgmt_result& result() const { return result_; }
     const kgmt_config& config() const { return cfg_; }
     int sampleDim() const { return D_; }
+    kgmt_ctx* context() const { return ctx_; }
 
 private:
     kgmt_config cfg_;
     kgmt_ctx* ctx_ = nullptr;
     kgmt_result result_{};
     int32_t S_ = 0, C_ = 0, D_ = 0;
+    bool deviceTree_ = true;
 };

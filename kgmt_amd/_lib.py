@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-from .abi import KGMT_ABI_VERSION, KgmtAdjointParams, KgmtConfig, KgmtProfile, KgmtResult
+from .abi import KGMT_ABI_VERSION, KgmtAdjointParams, KgmtConfig, KgmtDeviceTree, KgmtProfile, KgmtResult
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libkgmt.so")
 
@@ -45,6 +45,7 @@ SIGNATURES = {
     "kgmt_synchronize": (C.c_int, [_P]),
     "kgmt_plan": (C.c_int, [_P, _P, _P, _P, _u32, _i32, _resp]),
     "kgmt_get_tree": (C.c_int, [_P, _P, _P, _P, _u32, C.POINTER(_u32)]),
+    "kgmt_get_tree_device": (C.c_int, [_P, C.POINTER(KgmtDeviceTree)]),
     "kgmt_get_path": (C.c_int, [_P, _P, _u32, C.POINTER(_u32)]),
     "kgmt_get_regions": (C.c_int, [_P, _P, _P, _P, _u32, C.POINTER(_u32)]),
     "kgmt_set_profiling": (C.c_int, [_P, _i32]),
@@ -62,6 +63,8 @@ SIGNATURES = {
     "kgmt_propagate_collide": (C.c_int, [_cfgp, _P, _P, _u32, _P, _P, _u32, _P, _u32, _P, _P, _P, _P, _P]),
     "kgmt_propagate_collide_planner": (C.c_int, [_cfgp, _P, _P, _u32, _P, _P, _u32, _P, _u32, _P, _P, _P, _P, _P,
                                                  _P, _P]),
+    "kgmt_frontier_update": (C.c_int, [_cfgp, _u32, _u32, _P, _P, _P, _P, _P, _P, _P, _u32, _P, _P, _P, _P, _P,
+                                       _P]),
     "kgmt_region_update": (C.c_int, [_cfgp, _P, _P, _P, _P, _u32, _P, _P, _P, _P, _P, _P, _P]),
     "kgmt_score": (C.c_int, [_cfgp, _P, _P, _P, _P, _P]),
     "kgmt_compact": (C.c_int, [_P, _u32, _P, _P, _P]),

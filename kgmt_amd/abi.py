@@ -171,6 +171,18 @@ class KgmtAdjointParams(C.Structure):
     ]
 
 
+class KgmtDeviceTree(C.Structure):
+    _fields_ = [
+        ("samples", C.c_void_p),
+        ("parents", C.c_void_p),
+        ("costs", C.c_void_p),
+        ("tree_size", C.c_uint32),
+        ("capacity", C.c_uint32),
+        ("sample_dim", C.c_int32),
+        ("reserved", C.c_uint32),
+    ]
+
+
 KGMT_PROFILE_KERNELS = 5
 PROFILE_KERNEL_NAMES = ("propagate", "score", "append", "expand", "finish")
 
@@ -198,3 +210,4 @@ assert C.sizeof(KgmtConfig) == 168, C.sizeof(KgmtConfig)
 assert C.sizeof(KgmtResult) == 40, C.sizeof(KgmtResult)
 assert C.sizeof(KgmtAdjointParams) == 60, C.sizeof(KgmtAdjointParams)
 assert C.sizeof(KgmtProfile) == 104, C.sizeof(KgmtProfile)
+assert C.sizeof(KgmtDeviceTree) == 40, C.sizeof(KgmtDeviceTree)

@@ -278,6 +278,13 @@ struct Scene
     int sub = 0;        // lane within the child's group of K lanes
 };
 
+// copy `pairs` (lo, hi) float4 pairs of a scene into LDS
+__device__ __forceinline__ void stage_scene(const float4* __restrict__ pool, uint32_t off, uint32_t pairs,
+                                            float4* sObs)
+{
+    for(uint32_t i = threadIdx.x; i < 2 * pairs; i += blockDim.x) sObs[i] = pool[2 * (size_t)off + i];
+}
+
 // OR of a flag over each group of K consecutive lanes (K = 1, 2, 4), by DPP
 // quad permutations; every lane of a group must be active.
 template <int K>
@@ -1230,6 +1237,49 @@ __device__ __forceinline__ uint32_t locate(const uint32_t* prefix, uint32_t nq, 
                 hi = mid;
         }
     return lo;
+}
+
+// one child per thread, the one-pass check against every box of the scene
+// (staged in LDS by the caller kernel)
+template <int MODEL>
+__device__ __forceinline__ void stage_propagate_child(const QueryDesc& qd, const int32_t* __restrict__ parentIdx,
+                                                      const float* __restrict__ parentStates, uint32_t nParents,
+                                                      const float* __restrict__ controls,
+                                                      const int32_t* __restrict__ steps, uint32_t n,
+                                                      const float4* sObs, uint32_t nGrp, float* child,
+                                                      uint8_t* valid, int32_t* r1o, int32_t* r2o)
+{
+    constexpr int S = ModelDims<MODEL>::S, C = ModelDims<MODEL>::C, D = S + C + 1;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if(i >= n) return;
+    const int32_t p = parentIdx[i];
+    if(p < 0 || (uint32_t)p >= nParents)
+        {
+            // caller index out of range: no state is read; flagged by r1 = r2 = -1
+            for(int d = 0; d < D; ++d) child[(size_t)d * n + i] = 0.0f;
+            valid[i] = 0;
+            r1o[i]   = -1;
+            r2o[i]   = -1;
+            return;
+        }
+    float s[S], uc[C];
+#pragma unroll
+    for(int d = 0; d < S; ++d) s[d] = parentStates[(size_t)d * nParents + p];
+#pragma unroll
+    for(int k = 0; k < C; ++k) uc[k] = controls[(size_t)k * n + i];
+    const int32_t st = steps[i];
+    const Scene sc{sObs, (int)nGrp};
+    const bool ok    = propagate<MODEL>(qd, s, uc, st, sc);
+#pragma unroll
+    for(int d = 0; d < S; ++d) child[(size_t)d * n + i] = s[d];
+#pragma unroll
+    for(int k = 0; k < C; ++k) child[(size_t)(S + k) * n + i] = uc[k];
+    child[(size_t)(D - 1) * n + i] = qd.cfg.dt * (float)st;
+    int ix, iy, iz;
+    const int r1 = region_of(qd, s[0], s[1], s[2], ix, iy, iz);
+    valid[i]     = ok ? 1 : 0;
+    r1o[i]       = r1;
+    r2o[i]       = subregion_of(qd, s[0], s[1], s[2], r1, ix, iy, iz);
 }
 
 }  // namespace kgmt

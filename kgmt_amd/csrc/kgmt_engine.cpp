@@ -229,6 +229,12 @@ struct Engine
     int profiling = 0;
     std::vector<hipEvent_t> profEv;
     kgmt_profile prof{};
+    // device copy of query 0's tree in the reference's layout (rows of D
+    // floats, parents, costs), written on request after a plan
+    float* mirRows    = nullptr;
+    int32_t* mirPar   = nullptr;
+    float* mirCost    = nullptr;
+    bool mirValid     = false;
 
     ~Engine()
     {
@@ -477,6 +483,7 @@ struct Engine
     int run(bool useGraph)
     {
         const auto t0 = std::chrono::steady_clock::now();
+        mirValid      = false;
         HIP_TRY(hipMemcpyAsync(dq, hq.data(), sizeof(QueryDesc) * nq, hipMemcpyHostToDevice, stream));
         // k_init's last workgroup leaves the arrival counter at zero; after a
         // plan that did not complete it is cleared here
@@ -602,6 +609,33 @@ struct Engine
         if(parents)
             HIP_TRY(hipMemcpy(parents, P.parent + qd.nodeOff, sizeof(int32_t) * ts, hipMemcpyDeviceToHost));
         if(costs) HIP_TRY(hipMemcpy(costs, P.cost + qd.nodeOff, sizeof(float) * ts, hipMemcpyDeviceToHost));
+        return KGMT_OK;
+    }
+
+    // query 0's tree in the reference's layout in library-owned device
+    // memory, complete when this returns
+    int tree_device(kgmt_device_tree* out)
+    {
+        const QueryDesc& qd = hq[0];
+        int rc              = 0;
+        if(!mirRows && ((rc = dev_alloc(mirRows, (size_t)qd.cap * qd.D, owned)) ||
+                        (rc = dev_alloc(mirPar, qd.cap, owned)) || (rc = dev_alloc(mirCost, qd.cap, owned))))
+            return rc;
+        const uint32_t ts = hs[0].treeSize;
+        if(!mirValid)
+            {
+                HIP_TRY(launch_tree_rows(P.treeS + qd.sampleOff, P.parent + qd.nodeOff, P.cost + qd.nodeOff, qd.cap,
+                                         qd.D, ts, mirRows, mirPar, mirCost, stream));
+                HIP_TRY(hipStreamSynchronize(stream));
+                mirValid = true;
+            }
+        out->samples    = mirRows;
+        out->parents    = mirPar;
+        out->costs      = mirCost;
+        out->tree_size  = ts;
+        out->capacity   = qd.cap;
+        out->sample_dim = qd.D;
+        out->reserved   = 0;
         return KGMT_OK;
     }
 
@@ -927,6 +961,12 @@ int kgmt_get_tree(kgmt_ctx* ctx, float* samples, int32_t* parents, float* costs,
     return ctx->eng.get_tree(0, samples, parents, costs, cap, n);
 }
 
+int kgmt_get_tree_device(kgmt_ctx* ctx, kgmt_device_tree* out)
+{
+    if(!ctx || !out) return fail(KGMT_E_ARG, "null argument");
+    return ctx->eng.tree_device(out);
+}
+
 int kgmt_get_path(kgmt_ctx* ctx, float* rows, uint32_t cap_rows, uint32_t* n_rows)
 {
     if(!ctx) return fail(KGMT_E_ARG, "null context");
@@ -1036,6 +1076,7 @@ int kgmt_batch_get_tree(kgmt_batch* b, uint32_t q, float* samples, int32_t* pare
 }
 
 // ---- stage-level ----------------------------------------------------------
+extern "C++" {
 namespace
 {
 struct StageDesc
@@ -1059,7 +1100,37 @@ struct StageDesc
         if(d) (void)hipFreeAsync(d, s);
     }
 };
+
+// Stream-ordered scratch buffers of one stage call, released on every path
+// (hipFreeAsync on the call's stream after the work enqueued on it).
+struct AsyncBuffers
+{
+    hipStream_t s;
+    std::vector<void*> ptrs;
+    explicit AsyncBuffers(hipStream_t st) : s(st) {}
+    AsyncBuffers(const AsyncBuffers&)            = delete;
+    AsyncBuffers& operator=(const AsyncBuffers&) = delete;
+    ~AsyncBuffers()
+    {
+        for(void* p : ptrs) (void)hipFreeAsync(p, s);
+    }
+    template <typename T>
+    int get(T*& p, size_t count)
+    {
+        void* q            = nullptr;
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        if(hipMallocAsync(&q, bytes, s) != hipSuccess)
+            {
+                (void)hipGetLastError();
+                return fail(KGMT_E_NOMEM, "hipMallocAsync of " + std::to_string(bytes) + " bytes failed");
+            }
+        ptrs.push_back(q);
+        p = static_cast<T*>(q);
+        return KGMT_OK;
+    }
+};
 }  // namespace
+}  // extern "C++"
 
 int kgmt_draw_controls(const kgmt_config* cfg, uint32_t query, uint32_t itr, uint32_t u0, uint32_t n,
                        float* d_controls, int32_t* d_steps, float* d_ufront, void* stream)
@@ -1146,6 +1217,67 @@ int kgmt_propagate_collide_planner(const kgmt_config* cfg, const int32_t* d_pare
     return KGMT_OK;
 }
 
+int kgmt_frontier_update(const kgmt_config* cfg, uint32_t itr, uint32_t tree_size, float* d_tree_samples,
+                         int32_t* d_parents, float* d_costs, uint8_t* d_frontier, const float* d_child_samples,
+                         const int32_t* d_child_parent, const uint8_t* d_fnext, uint32_t n_children,
+                         const int32_t* d_valid_count, const float* d_score, const float* goal, uint32_t* d_rep_idx,
+                         int32_t* d_info, void* stream)
+{
+    if(!cfg || !goal) return fail(KGMT_E_ARG, "null argument");
+    if(!d_tree_samples || !d_parents || !d_costs || !d_frontier || !d_valid_count || !d_score || !d_rep_idx ||
+       !d_info || (n_children && (!d_child_samples || !d_child_parent || !d_fnext)))
+        return fail(KGMT_E_ARG, "null buffer");
+    int rc = check_config(cfg);
+    if(rc) return rc;
+    if(itr < 1) return fail(KGMT_E_ARG, "itr must be >= 1");
+    if(tree_size < 1 || tree_size > cfg->capacity) return fail(KGMT_E_ARG, "tree_size must be in [1, capacity]");
+    if(n_children > cfg->capacity) return fail(KGMT_E_CAPACITY, "more children than capacity");
+    if(n_children > 1024u * kChunk) return fail(KGMT_E_CAPACITY, "at most 2^20 children");
+    hipStream_t s = (hipStream_t)stream;
+    QueryDesc h;
+    fill_desc(h, *cfg);
+    h.goal[0] = goal[0], h.goal[1] = goal[1], h.goal[2] = h.wdim == 3 ? goal[2] : 0.0f;
+    const uint32_t cap = h.cap;
+    const size_t nodes = ((size_t)cap + kNodeAlign - 1) / kNodeAlign * kNodeAlign + kNodeAlign;
+    AsyncBuffers a(s);
+    QueryDesc* dq  = nullptr;
+    QueryState* ds = nullptr;
+    int32_t* info  = nullptr;
+    Pools P{};
+    WorkLists W{};
+    if((rc = a.get(dq, 1)) || (rc = a.get(ds, 1)) || (rc = a.get(info, 4)) ||
+       (rc = a.get(P.treeS, (size_t)h.D * cap)) || (rc = a.get(P.childR, (size_t)row_words(h.D) * cap)) ||
+       (rc = a.get(P.parent, nodes)) || (rc = a.get(P.grand, nodes)) || (rc = a.get(P.cost, nodes)) ||
+       (rc = a.get(P.frontier, nodes)) || (rc = a.get(P.r1node, nodes)) || (rc = a.get(P.r1child, nodes)) ||
+       (rc = a.get(P.fnext, nodes)) || (rc = a.get(P.react, nodes)) || (rc = a.get(P.repU, nodes)) ||
+       (rc = a.get(P.repIdx, nodes)) || (rc = a.get(P.partA, h.nChunks + 1)) ||
+       (rc = a.get(P.partB, h.nChunks + 1)) || (rc = a.get(P.partC, h.nChunks + 1)) ||
+       (rc = a.get(P.validc, h.R1)) || (rc = a.get(P.score, h.R1)) || (rc = a.get(W.tileStart, 4)) ||
+       (rc = a.get(W.jobStart, 4)) || (rc = a.get(W.active, 2)) || (rc = a.get(W.counter, 1)) ||
+       (rc = a.get(W.allDone, 1)))
+        return rc;
+    W.nq   = 1;
+    W.tile = kTile;
+    HIP_TRY(hipMemcpyAsync(dq, &h, sizeof(QueryDesc), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(P.treeS, d_tree_samples, sizeof(float) * h.D * cap, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(P.parent, d_parents, sizeof(int32_t) * cap, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(P.cost, d_costs, sizeof(float) * cap, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(P.frontier, d_frontier, cap, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(P.validc, d_valid_count, sizeof(int32_t) * h.R1, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(P.score, d_score, sizeof(float) * h.R1, hipMemcpyDeviceToDevice, s));
+    hipError_t e = launch_stage_frontier_update(dq, ds, P, W, itr, tree_size, d_child_samples, d_child_parent,
+                                                d_fnext, n_children, info, s);
+    if(e != hipSuccess) return fail(KGMT_E_HIP, std::string("frontier update: ") + hipGetErrorString(e));
+    HIP_TRY(hipMemcpyAsync(d_tree_samples, P.treeS, sizeof(float) * h.D * cap, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_parents, P.parent, sizeof(int32_t) * cap, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_costs, P.cost, sizeof(float) * cap, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_frontier, P.frontier, cap, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_rep_idx, P.repIdx, sizeof(uint32_t) * cap, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_info, info, sizeof(int32_t) * 4, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));  // h is released on return
+    return KGMT_OK;
+}
+
 int kgmt_region_update(const kgmt_config* cfg, const int32_t* d_r1, const int32_t* d_r2, const uint8_t* d_valid,
                        const float* d_ufront, uint32_t n, const float* d_score, const uint32_t* d_active_prev,
                        uint32_t* d_active_next, int32_t* d_total, int32_t* d_valid_count, uint8_t* d_fnext,
@@ -1218,12 +1350,11 @@ static int adjoint_common(bool refine, const kgmt_config* cfg, const kgmt_adjoin
     const size_t ck = ((size_t)max_steps + 1) * 12 * n, mv = (size_t)max_steps * 4 * n;
     float *ckpt = nullptr, *mom = nullptr, *vel = nullptr;
     float4* boxes = nullptr;
-    HIP_TRY(hipMallocAsync((void**)&ckpt, ck * sizeof(float), sd.s));
-    HIP_TRY(hipMallocAsync((void**)&boxes, sizeof(float4) * 2 * std::max<uint32_t>(n_obs, 1), sd.s));
+    AsyncBuffers a(sd.s);  // released on every return path
+    if((rc = a.get(ckpt, ck)) || (rc = a.get(boxes, 2 * (size_t)std::max<uint32_t>(n_obs, 1)))) return rc;
     if(refine)
         {
-            HIP_TRY(hipMallocAsync((void**)&mom, std::max<size_t>(mv, 1) * sizeof(float), sd.s));
-            HIP_TRY(hipMallocAsync((void**)&vel, std::max<size_t>(mv, 1) * sizeof(float), sd.s));
+            if((rc = a.get(mom, mv)) || (rc = a.get(vel, mv))) return rc;
             HIP_TRY(hipMemsetAsync(mom, 0, mv * sizeof(float), sd.s));
             HIP_TRY(hipMemsetAsync(vel, 0, mv * sizeof(float), sd.s));
         }
@@ -1231,10 +1362,6 @@ static int adjoint_common(bool refine, const kgmt_config* cfg, const kgmt_adjoin
     if(e == hipSuccess)
         e = launch_adjoint(refine, sd.d, *ap, n, max_steps, d_steps, d_x0, d_goal, d_u, d_grad, ckpt, mom, vel, boxes,
                            n_obs, d_cost, sd.s);
-    (void)hipFreeAsync(ckpt, sd.s);
-    (void)hipFreeAsync(boxes, sd.s);
-    if(mom) (void)hipFreeAsync(mom, sd.s);
-    if(vel) (void)hipFreeAsync(vel, sd.s);
     if(e != hipSuccess) return fail(KGMT_E_HIP, std::string("adjoint: ") + hipGetErrorString(e));
     return KGMT_OK;
 }

@@ -35,13 +35,6 @@ namespace kgmt
 // ---------------------------------------------------------------------------
 // shared helpers
 // ---------------------------------------------------------------------------
-// copy `pairs` (lo, hi) float4 pairs of a scene into LDS
-__device__ __forceinline__ void stage_scene(const float4* __restrict__ pool, uint32_t off, uint32_t pairs,
-                                            float4* sObs)
-{
-    for(uint32_t i = threadIdx.x; i < 2 * pairs; i += blockDim.x) sObs[i] = pool[2 * (size_t)off + i];
-}
-
 // total[key]++ for lanes with `has`, valid[key]++ for lanes with `has && ok`:
 // one atomic per distinct key held by several lanes (leader loop over the
 // ballot; children of one parent share regions, and atomics of every lane on
@@ -1368,6 +1361,25 @@ __global__ __launch_bounds__(kBlock) void k_build_scene(const float* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// a query's tree in the reference's layout: rows of D floats
+// (d_treeSamples_), parents and costs; entries past the tree are 0, -1, 0 as
+// the reference's fills leave them (KGMT.cu:62-64)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_tree_rows(const float* __restrict__ treeS,
+                                                      const int32_t* __restrict__ parent,
+                                                      const float* __restrict__ cost, uint32_t cap, int D,
+                                                      uint32_t treeSize, float* __restrict__ rows,
+                                                      int32_t* __restrict__ parents, float* __restrict__ costs)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if(i >= cap) return;
+    const bool in = i < treeSize;
+    for(int d = 0; d < D; ++d) rows[(size_t)i * D + d] = in ? treeS[(size_t)d * cap + i] : 0.0f;
+    parents[i] = in ? parent[i] : -1;
+    costs[i]   = in ? cost[i] : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
 // stage-level kernels (single query, caller buffers)
 // ---------------------------------------------------------------------------
 template <int C>
@@ -1394,45 +1406,13 @@ __global__ __launch_bounds__(kBlock) void k_stage_propagate(const QueryDesc* __r
                                                             uint32_t nParents, const float* __restrict__ controls,
                                                             const int32_t* __restrict__ steps, uint32_t n,
                                                             const float4* __restrict__ obs, uint32_t nGrp,
-                                                            uint32_t nObs,
                                                             float* child, uint8_t* valid, int32_t* r1o, int32_t* r2o)
 {
-    constexpr int S = ModelDims<MODEL>::S, C = ModelDims<MODEL>::C, D = S + C + 1;
     extern __shared__ float4 sObs[];
-    const QueryDesc& qd = *qdp;
     stage_scene(obs, 0, nGrp * (1 + kGroup), sObs);
     __syncthreads();
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if(i >= n) return;
-    const int32_t p = parentIdx[i];
-    if(p < 0 || (uint32_t)p >= nParents)
-        {
-            // caller index out of range: no state is read; flagged by r1 = r2 = -1
-            for(int d = 0; d < D; ++d) child[(size_t)d * n + i] = 0.0f;
-            valid[i] = 0;
-            r1o[i]   = -1;
-            r2o[i]   = -1;
-            return;
-        }
-    float s[S], uc[C];
-#pragma unroll
-    for(int d = 0; d < S; ++d) s[d] = parentStates[(size_t)d * nParents + p];
-#pragma unroll
-    for(int k = 0; k < C; ++k) uc[k] = controls[(size_t)k * n + i];
-    const int32_t st = steps[i];
-    const Scene sc{sObs, (int)nGrp};
-    (void)nObs;
-    const bool ok    = propagate<MODEL>(qd, s, uc, st, sc);
-#pragma unroll
-    for(int d = 0; d < S; ++d) child[(size_t)d * n + i] = s[d];
-#pragma unroll
-    for(int k = 0; k < C; ++k) child[(size_t)(S + k) * n + i] = uc[k];
-    child[(size_t)(D - 1) * n + i] = qd.cfg.dt * (float)st;
-    int ix, iy, iz;
-    const int r1 = region_of(qd, s[0], s[1], s[2], ix, iy, iz);
-    valid[i]     = ok ? 1 : 0;
-    r1o[i]       = r1;
-    r2o[i]       = subregion_of(qd, s[0], s[1], s[2], r1, ix, iy, iz);
+    stage_propagate_child<MODEL>(*qdp, parentIdx, parentStates, nParents, controls, steps, n, sObs, nGrp, child,
+                                 valid, r1o, r2o);
 }
 
 // The planner's per-child path on caller buffers: children in waves of 64
@@ -1520,6 +1500,99 @@ __global__ __launch_bounds__(kBlock) void k_stage_propagate_planner(
     if(checkOut) checkOut[i] = (uint8_t)tr.path;
     if(boxOut && MODEL != KGMT_MODEL_QUADCOPTER)
         for(int d = 0; d < 3; ++d) boxOut[(size_t)d * n + i] = tr.lo[d], boxOut[(size_t)(3 + d) * n + i] = tr.hi[d];
+}
+
+// Frontier update stage: the caller's tree and children staged into a
+// one-query pool the way the iteration's propagate and score kernels leave
+// it, so that the planner's own k_append and k_expand run on it. Per tree
+// node its region (the reference's getRegion of the node, KGMT.cu:369); per
+// child its region, its flag and, when flagged, its row (parent, parent's
+// parent, sample) at the next free slot of its 32-child segment.
+__global__ __launch_bounds__(kBlock) void k_stage_fu_nodes(const QueryDesc* __restrict__ qdp, Pools P, uint32_t tree,
+                                                           const float* __restrict__ child,
+                                                           const int32_t* __restrict__ childParent,
+                                                           const uint8_t* __restrict__ fnext, uint32_t nC)
+{
+    const QueryDesc& qd = *qdp;
+    const uint32_t i    = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t cap  = qd.cap;
+    int ix, iy, iz;
+    if(i < tree)
+        P.r1node[i] = region_of(qd, P.treeS[i], P.treeS[(size_t)cap + i], P.treeS[2 * (size_t)cap + i], ix, iy, iz);
+    if(i >= nC) return;
+    const bool f = fnext[i] != 0;
+    P.fnext[i]   = f ? 1 : 0;
+    P.r1child[i] = region_of(qd, child[i], child[(size_t)nC + i], child[2 * (size_t)nC + i], ix, iy, iz);
+    if(!f) return;
+    const uint32_t s0 = i & ~31u;
+    uint32_t before   = 0;
+    for(uint32_t k = s0; k < i; ++k) before += fnext[k] ? 1u : 0u;
+    const int32_t p  = childParent[i];
+    const int32_t gp = p >= 0 && (uint32_t)p < tree ? P.parent[p] : -1;
+    const int D      = qd.D, RW = row_words(D);
+    float* row       = P.childR + (size_t)(s0 + before) * RW;
+    row[0]           = __int_as_float(p);
+    row[1]           = __int_as_float(gp);
+    for(int d = 0; d < D; ++d) row[2 + d] = child[(size_t)d * nC + i];
+    for(int e = D + 2; e < RW; ++e) row[e] = 0.0f;
+}
+
+// flagged children per chunk, exclusive offsets, and the state the score
+// kernel leaves (accepted count clamped to the free capacity, fAccept); the
+// work lists of the append / expand jobs (one workgroup)
+__global__ __launch_bounds__(kBlock) void k_stage_fu_state(const QueryDesc* __restrict__ qdp, QueryState* st,
+                                                           Pools P, WorkLists W, uint32_t itr, uint32_t tree,
+                                                           uint32_t nC)
+{
+    __shared__ uint32_t cnt[1024];
+    const QueryDesc& qd = *qdp;
+    const uint32_t nCC  = (nC + kChunk - 1) / kChunk;
+    for(uint32_t c = threadIdx.x; c < nCC; c += kBlock)
+        {
+            uint32_t n = 0;
+            for(uint32_t u = c * kChunk; u < nC && u < (c + 1) * kChunk; ++u) n += P.fnext[u];
+            cnt[c] = n;
+        }
+    __syncthreads();
+    if(threadIdx.x != 0) return;
+    uint32_t acc = 0;
+    for(uint32_t c = 0; c < nCC; ++c)
+        {
+            P.partA[c] = acc;
+            acc += cnt[c];
+        }
+    const uint32_t avail = qd.cap - tree;
+    const uint32_t next  = acc < avail ? acc : avail;
+    QueryState s{};
+    s.itr = s.itrN = itr;
+    s.treeSize = s.treeSizeN = tree;
+    s.nChildren = s.nChildrenN = nC;
+    s.nAcc      = acc;
+    s.next      = next;
+    s.fAccept   = faccept(qd, itr, tree, next);
+    s.goalIdx   = kNoGoal;
+    *st         = s;
+    const uint32_t jobs = nCC + (tree + kChunk - 1) / kChunk;
+    for(uint32_t par = 0; par < 2; ++par)
+        {
+            W.jobStart[2 * par] = 0, W.jobStart[2 * par + 1] = jobs;
+            W.tileStart[2 * par] = 0, W.tileStart[2 * par + 1] = 0;
+            W.active[par] = 1;
+        }
+}
+
+// info[4] = {appended, expansion multiset size, goal node or -1, re-activated}
+__global__ void k_stage_fu_info(const QueryDesc* __restrict__ qdp, const QueryState* st, Pools P, uint32_t tree,
+                                int32_t* info)
+{
+    if(threadIdx.x != 0) return;
+    const uint32_t nOC = (tree + kChunk - 1) / kChunk;
+    uint32_t react     = 0;
+    for(uint32_t o = 0; o < nOC; ++o) react += P.partB[o];
+    info[0] = (int32_t)st->next;
+    info[1] = (int32_t)st->R;
+    info[2] = st->goalIdx == kNoGoal ? -1 : st->goalIdx;
+    info[3] = (int32_t)react;
 }
 
 __global__ __launch_bounds__(kBlock) void k_stage_region_update(const QueryDesc* __restrict__ qdp,
@@ -1742,6 +1815,15 @@ hipError_t launch_expand(const QueryDesc* qds, QueryState* sts, const Pools& P, 
     return hipGetLastError();
 }
 
+hipError_t launch_tree_rows(const float* treeS, const int32_t* parent, const float* cost, uint32_t cap, int D,
+                            uint32_t treeSize, float* rows, int32_t* parents, float* costs, hipStream_t s)
+{
+    if(cap == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tree_rows, dim3((cap + kBlock - 1) / kBlock), dim3(kBlock), 0, s, treeS, parent, cost, cap,
+                       D, treeSize, rows, parents, costs);
+    return hipGetLastError();
+}
+
 hipError_t launch_stage_draw(const QueryDesc* qd, int C, uint32_t query, uint32_t itr, uint32_t u0, uint32_t n,
                              float* controls, int32_t* steps, float* ufront, hipStream_t s)
 {
@@ -1765,9 +1847,10 @@ hipError_t launch_stage_propagate(int model, const QueryDesc* qd, const int32_t*
     if(n == 0) return hipSuccess;
     const dim3 g((n + kBlock - 1) / kBlock), b(kBlock);
     const size_t lds = (size_t)(nGrp ? nGrp * (1 + kGroup) : 1) * 2 * sizeof(float4);
+    (void)nObs;
 #define KGMT_STAGE_PROP(M)                                                                                     \
     hipLaunchKernelGGL(k_stage_propagate<M>, g, b, lds, s, qd, parentIdx, parentStates, nParents, controls, steps, \
-                       n, obs, nGrp, nObs, child, valid, r1, r2)
+                       n, obs, nGrp, child, valid, r1, r2)
     switch(model)
         {
             case KGMT_MODEL_DOUBLE_INTEGRATOR: KGMT_STAGE_PROP(KGMT_MODEL_DOUBLE_INTEGRATOR); break;
@@ -1803,6 +1886,23 @@ hipError_t launch_stage_propagate_planner(int model, const QueryDesc* qd, const 
             default: return hipErrorInvalidValue;
         }
 #undef KGMT_STAGE_PROP
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_frontier_update(const QueryDesc* qd, QueryState* st, const Pools& P, const WorkLists& W,
+                                        uint32_t itr, uint32_t tree, const float* child, const int32_t* childParent,
+                                        const uint8_t* fnext, uint32_t nC, int32_t* info, hipStream_t s)
+{
+    const uint32_t n = tree > nC ? tree : nC;
+    if(n) hipLaunchKernelGGL(k_stage_fu_nodes, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, qd, P, tree, child,
+                             childParent, fnext, nC);
+    hipLaunchKernelGGL(k_stage_fu_state, dim3(1), dim3(kBlock), 0, s, qd, st, P, W, itr, tree, nC);
+    Launch L;
+    L.chunkGrid = 256;
+    hipError_t e = launch_append(qd, st, P, W, 0, L, s);
+    if(e == hipSuccess) e = launch_expand(qd, st, P, W, 0, L, s);
+    if(e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_stage_fu_info, dim3(1), dim3(64), 0, s, qd, st, P, tree, info);
     return hipGetLastError();
 }
 

@@ -46,6 +46,11 @@ hipError_t launch_finish(const QueryDesc* qds, QueryState* sts, const WorkLists&
 // goal paths of the solved queries, once a plan has ended
 hipError_t launch_paths(const QueryDesc* qds, QueryState* sts, const Pools& P, uint32_t nq, hipStream_t s);
 
+// a query's tree as rows of D floats, parents and costs (entries past the
+// tree: 0, -1, 0)
+hipError_t launch_tree_rows(const float* treeS, const int32_t* parent, const float* cost, uint32_t cap, int D,
+                            uint32_t treeSize, float* rows, int32_t* parents, float* costs, hipStream_t s);
+
 // stage-level kernels (one query, caller buffers)
 hipError_t launch_stage_draw(const QueryDesc* qd, int C, uint32_t query, uint32_t itr, uint32_t u0, uint32_t n,
                              float* controls, int32_t* steps, float* ufront, hipStream_t s);
@@ -59,6 +64,13 @@ hipError_t launch_stage_propagate_planner(int model, const QueryDesc* qd, const 
                                           const int32_t* steps, uint32_t n, const float4* obs, uint32_t nGrp,
                                           float* child, uint8_t* valid, int32_t* r1, int32_t* r2, uint8_t* check,
                                           float* box, hipStream_t s);
+// frontier update of one iteration on a one-query pool (the caller's tree
+// already in P.treeS / parent / cost / frontier, region counts and scores in
+// P.validc / P.score): staging kernels, then the planner's k_append and
+// k_expand; info[4] = {appended, multiset size, goal node or -1, re-activated}
+hipError_t launch_stage_frontier_update(const QueryDesc* qd, QueryState* st, const Pools& P, const WorkLists& W,
+                                        uint32_t itr, uint32_t tree, const float* child, const int32_t* childParent,
+                                        const uint8_t* fnext, uint32_t nC, int32_t* info, hipStream_t s);
 hipError_t launch_stage_region_update(const QueryDesc* qd, const int32_t* r1, const int32_t* r2, const uint8_t* valid,
                                       const float* ufront, uint32_t n, const float* score, const uint32_t* actPrev,
                                       uint32_t* actNext, int32_t* total, int32_t* validc, uint8_t* fnext,

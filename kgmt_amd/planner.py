@@ -16,7 +16,7 @@ import os
 import numpy as np
 
 from . import _lib
-from .abi import KGMT_MAX_SAMPLE, KgmtConfig, KgmtProfile, KgmtResult, dims, wdim
+from .abi import KGMT_MAX_SAMPLE, KgmtConfig, KgmtDeviceTree, KgmtProfile, KgmtResult, dims, wdim
 from .workloads import config as workload_config
 
 
@@ -162,6 +162,15 @@ class KGMT(Planner):
         c = np.zeros(k, np.float32)
         _lib.check(L.kgmt_get_tree(self._h, _ptr(s), _ptr(p), _ptr(c), k, C.byref(n)))
         return s, p, c
+
+    def tree_device(self):
+        """The tree in the reference's device layout (d_treeSamples_ rows of D
+        floats, d_treeSamplesParentIdxs_, d_treeSampleCosts_): a KgmtDeviceTree
+        of device addresses into library-owned buffers of `capacity` entries,
+        valid until the next plan."""
+        t = KgmtDeviceTree()
+        _lib.check(_lib.lib().kgmt_get_tree_device(self._h, C.byref(t)))
+        return t
 
     def path(self) -> np.ndarray:
         L = _lib.lib()

@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import math
 import os
+import re
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -254,19 +255,31 @@ def pack_batch(queries):
 # ---------------------------------------------------------------------------
 # obstacle CSV files (include/config/obstacles/*/obstacles.csv format)
 # ---------------------------------------------------------------------------
+_NUMBER = re.compile(r"\s*([+-]?(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?)")
+
+
 def read_obstacles_csv(path: str, workspace_dim: int = 3) -> np.ndarray:
-    """Boxes of an obstacle CSV: every number of the file in order (commas
-    and whitespace separate), grouped 2*workspace_dim per box; a trailing
-    partial group is dropped. Mirrors readObstaclesFromCSV
-    (src/helper/helper.cu:13-39), which also reads a last line without a
-    newline."""
+    """Boxes of an obstacle CSV, read as readObstaclesFromCSV
+    (src/helper/helper.cu:13-39) reads them: each line is parsed number by
+    number (leading whitespace skipped, one comma right after a number
+    skipped) until a token is not a number; every value of the file in order,
+    grouped 2*workspace_dim per box (a trailing partial group is dropped). A
+    last line without a newline is read too."""
     if not os.path.exists(path):
         raise FileNotFoundError(path)
     vals = []
     with open(path, "r") as f:
         for line in f:
-            for tok in line.replace(",", " ").split():
-                vals.append(float(tok))
+            line = line.rstrip("\n")
+            pos = 0
+            while True:
+                m = _NUMBER.match(line, pos)
+                if not m:
+                    break
+                vals.append(float(np.float32(float(m.group(1)))))
+                pos = m.end()
+                if line.startswith(",", pos):
+                    pos += 1
     k = 2 * workspace_dim
     n = len(vals) // k
     return np.asarray(vals[: n * k], np.float32).reshape(n, k)

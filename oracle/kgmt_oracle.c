@@ -811,6 +811,84 @@ static float pos_dist(const kgmt_config* cfg, const float* a, const float* b)
     return sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
+/* updateFrontier (KGMT.cu:380-403) and updateFrontier_kernel (:305-378) of
+ * iteration itr, after the iteration's propagate and score update:
+ * compaction of the flagged children in child order, clamped to the free
+ * capacity (Q9); fAccept; every old node with frontier 0 re-activated when
+ * its draw U <= score[region] + fAccept (repeat 1); accepted child j appended
+ * as node tree_size + j (parent, cost = goal distance, frontier 1, repeat 15
+ * while its region's valid count < 10, else 1); the smallest new index
+ * within the goal threshold; the next expansion multiset (re-activated nodes
+ * ascending, then each new node repeated, Q5), its first capacity entries in
+ * M. Layouts: samples [capacity][D], child [n_children][D]. */
+int orc_frontier_update(const kgmt_config* cfg, uint32_t itr, uint32_t tree_size, float* samples, int32_t* parents,
+                        float* costs, uint8_t* frontier, const float* child, const int32_t* parentU,
+                        const uint8_t* fnext, uint32_t nC, const int32_t* validc, const float* score,
+                        const float* goal, uint32_t* M, uint32_t* nM_out, uint64_t* R_out, uint32_t* next_out,
+                        int32_t* goal_out, uint32_t* nreact_out)
+{
+    int S, C;
+    if(dims(cfg->model, &S, &C)) return KGMT_E_UNSUPPORTED;
+    const int D          = S + C + 1;
+    const uint32_t N     = cfg->capacity;
+    const uint32_t query = cfg->query_id;
+    if(tree_size > N) return KGMT_E_ARG;
+    uint32_t* accept = (uint32_t*)malloc(sizeof(uint32_t) * (nC ? nC : 1));
+    uint32_t* react  = (uint32_t*)malloc(sizeof(uint32_t) * (tree_size ? tree_size : 1));
+    uint32_t* newrep = (uint32_t*)malloc(sizeof(uint32_t) * (nC ? nC : 1));
+    if(!accept || !react || !newrep)
+        {
+            free(accept), free(react), free(newrep);
+            return KGMT_E_NOMEM;
+        }
+    const uint32_t avail = N - tree_size;
+    uint32_t nAcc        = 0;
+    for(uint32_t u = 0; u < nC; ++u)
+        if(fnext[u]) accept[nAcc++] = u;
+    const uint32_t next = nAcc < avail ? nAcc : avail;
+    const float fA      = orc_faccept(cfg, itr, tree_size, next);
+    uint32_t nReact     = 0;
+    for(uint32_t i = 0; i < tree_size; ++i)
+        {
+            if(frontier[i] == 0)
+                {
+                    float U = orc_draw_react(cfg, query, itr, i);
+                    if(U <= score[orc_region(cfg, samples + (size_t)i * D)] + fA)
+                        {
+                            frontier[i]     = 1;
+                            react[nReact++] = i;
+                        }
+                }
+        }
+    int32_t best    = -1;
+    uint64_t sumRep = 0;
+    for(uint32_t j = 0; j < next; ++j)
+        {
+            uint32_t t = tree_size + j, u = accept[j];
+            const float* x1 = child + (size_t)u * D;
+            memcpy(samples + (size_t)t * D, x1, sizeof(float) * D);
+            parents[t]  = parentU[u];
+            float dg    = pos_dist(cfg, x1, goal);
+            costs[t]    = dg;
+            frontier[t] = 1;
+            int r1      = orc_region(cfg, x1);
+            newrep[j]   = validc[r1] < cfg->sparse_valid_below ? cfg->repeat_sparse : cfg->repeat_dense;
+            sumRep += newrep[j];
+            if(dg < cfg->goal_thresh && best < 0) best = (int32_t)t;
+        }
+    uint32_t nM = 0;
+    for(uint32_t k = 0; k < nReact && nM < N; ++k) M[nM++] = react[k];
+    for(uint32_t j = 0; j < next && nM < N; ++j)
+        for(uint32_t k = 0; k < newrep[j] && nM < N; ++k) M[nM++] = tree_size + j;
+    *nM_out     = nM;
+    *R_out      = (uint64_t)nReact + sumRep;
+    *next_out   = next;
+    *goal_out   = best;
+    *nreact_out = nReact;
+    free(accept), free(react), free(newrep);
+    return KGMT_OK;
+}
+
 int orc_plan(const kgmt_config* cfg, const float* start, const float* goal, const float* obs,
              uint32_t n_obs, float* samples, int32_t* parents, float* costs, float* path, int32_t* total_out,
              int32_t* valid_out, float* score_out, uint32_t* iter_log, orc_stats* st)
@@ -827,19 +905,14 @@ int orc_plan(const kgmt_config* cfg, const float* start, const float* goal, cons
     int32_t* parentU  = (int32_t*)malloc(sizeof(int32_t) * N);
     uint8_t* fnext    = (uint8_t*)malloc(N);
     uint8_t* frontier = (uint8_t*)calloc(N, 1);
-    int32_t* r1node   = (int32_t*)malloc(sizeof(int32_t) * N);
     uint32_t* M       = (uint32_t*)malloc(sizeof(uint32_t) * N); /* expansion list, first N entries */
-    uint32_t* accept  = (uint32_t*)malloc(sizeof(uint32_t) * N);
-    uint32_t* react   = (uint32_t*)malloc(sizeof(uint32_t) * N);
-    uint32_t* newrep  = (uint32_t*)malloc(sizeof(uint32_t) * N);
     int32_t* total    = (int32_t*)calloc((size_t)R1, sizeof(int32_t));
     int32_t* validc   = (int32_t*)calloc((size_t)R1, sizeof(int32_t));
     float* score      = (float*)calloc((size_t)R1, sizeof(float));
     uint8_t* actPrev  = (uint8_t*)calloc((size_t)R2, 1);
     uint8_t* actNext  = (uint8_t*)calloc((size_t)R2, 1);
     int rc            = KGMT_OK;
-    if(!child || !parentU || !fnext || !frontier || !r1node || !M || !accept || !react || !newrep || !total ||
-       !validc || !score || !actPrev || !actNext)
+    if(!child || !parentU || !fnext || !frontier || !M || !total || !validc || !score || !actPrev || !actNext)
         {
             rc = KGMT_E_NOMEM;
             goto done;
@@ -850,12 +923,9 @@ int orc_plan(const kgmt_config* cfg, const float* start, const float* goal, cons
     parents[0]   = -1;
     costs[0]     = 0.0f;
     frontier[0]  = 1;
-    r1node[0]    = orc_region(cfg, start);
-    uint32_t nM  = 1 < N ? 1 : N; /* materialised entries of the multiset (<= N) */
     uint64_t R   = cfg->root_repeat;
     uint32_t mroot;
     for(mroot = 0; mroot < cfg->root_repeat && mroot < N; ++mroot) M[mroot] = 0;
-    nM = mroot;
     uint32_t treeSize = 1, itr = 0;
     int32_t goalIdx   = -1;
     uint64_t expanded = 0;
@@ -904,48 +974,13 @@ int orc_plan(const kgmt_config* cfg, const float* start, const float* goal, cons
             orc_score(cfg, total, validc, actNext, score);
             memcpy(actPrev, actNext, (size_t)R2);
 
-            /* -- updateFrontier: compaction, fAccept, append, re-activation -- */
-            uint32_t nAcc = 0;
-            for(uint32_t u = 0; u < nC; ++u)
-                if(fnext[u]) accept[nAcc++] = u;
-            uint32_t next = nAcc < avail ? nAcc : avail;
-            float fA      = orc_faccept(cfg, itr, treeSize, next);
-            uint32_t nReact = 0;
-            for(uint32_t i = 0; i < treeSize; ++i)
-                {
-                    if(frontier[i] == 0)
-                        {
-                            float U = orc_draw_react(cfg, query, itr, i);
-                            if(U <= score[r1node[i]] + fA)
-                                {
-                                    frontier[i]     = 1;
-                                    react[nReact++] = i;
-                                }
-                        }
-                }
-            int32_t best = -1;
-            uint64_t sumRep = 0;
-            for(uint32_t j = 0; j < next; ++j)
-                {
-                    uint32_t t = treeSize + j, u = accept[j];
-                    const float* x1 = child + (size_t)u * D;
-                    memcpy(samples + (size_t)t * D, x1, sizeof(float) * D);
-                    parents[t]  = parentU[u];
-                    float dg    = pos_dist(cfg, x1, goal);
-                    costs[t]    = dg;
-                    frontier[t] = 1;
-                    int r1      = orc_region(cfg, x1);
-                    r1node[t]   = r1;
-                    newrep[j]   = validc[r1] < cfg->sparse_valid_below ? cfg->repeat_sparse : cfg->repeat_dense;
-                    sumRep += newrep[j];
-                    if(dg < cfg->goal_thresh && best < 0) best = (int32_t)t;
-                }
-            /* next expansion multiset (Q5): re-activated (ascending) then new */
-            R  = (uint64_t)nReact + sumRep;
-            nM = 0;
-            for(uint32_t k = 0; k < nReact && nM < N; ++k) M[nM++] = react[k];
-            for(uint32_t j = 0; j < next && nM < N; ++j)
-                for(uint32_t k = 0; k < newrep[j] && nM < N; ++k) M[nM++] = treeSize + j;
+            /* -- updateFrontier: compaction, fAccept, re-activation, append -- */
+            uint32_t next = 0, nReact = 0, nM32 = 0;
+            int32_t best  = -1;
+            rc = orc_frontier_update(cfg, itr, treeSize, samples, parents, costs, frontier, child, parentU, fnext, nC,
+                                     validc, score, goal, M, &nM32, &R, &next, &best, &nReact);
+            if(rc) goto done;
+            (void)nM32;
             if(iter_log)
                 {
                     iter_log[4 * (itr - 1) + 0] = (uint32_t)Rnow;
@@ -993,8 +1028,8 @@ int orc_plan(const kgmt_config* cfg, const float* start, const float* goal, cons
     if(score_out) memcpy(score_out, score, sizeof(float) * (size_t)R1);
 
 done:
-    free(child), free(parentU), free(fnext), free(frontier), free(r1node), free(M), free(accept), free(react);
-    free(newrep), free(total), free(validc), free(score), free(actPrev), free(actNext);
+    free(child), free(parentU), free(fnext), free(frontier), free(M), free(total), free(validc), free(score);
+    free(actPrev), free(actNext);
     return rc;
 }
 

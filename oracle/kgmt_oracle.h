@@ -52,6 +52,22 @@ void orc_score(const kgmt_config* cfg, const int32_t* total, const int32_t* vali
                const uint8_t* active, float* score);
 float orc_faccept(const kgmt_config* cfg, uint32_t itr, uint32_t tree_size, uint32_t next);
 
+/* frontier update of one iteration (updateFrontier + updateFrontier_kernel,
+ * KGMT.cu:305-403) on the planner's state after propagate and score:
+ * samples [capacity][D], parents, costs, frontier [capacity] (the first
+ * tree_size entries are the tree; accepted children are appended after
+ * them), children child [n_children][D], parentU, fnext [n_children], region
+ * valid counts and scores [R1]. Outputs: the next expansion multiset's first
+ * capacity entries in M (re-activated nodes ascending, then every new node
+ * repeated 15 or 1 times) and its length nM, the full multiset size R, the
+ * appended count, the smallest new node index within the goal threshold (or
+ * -1) and the number of re-activated nodes. */
+int orc_frontier_update(const kgmt_config* cfg, uint32_t itr, uint32_t tree_size, float* samples, int32_t* parents,
+                        float* costs, uint8_t* frontier, const float* child, const int32_t* parentU,
+                        const uint8_t* fnext, uint32_t n_children, const int32_t* valid_count, const float* score,
+                        const float* goal, uint32_t* M, uint32_t* nM, uint64_t* R, uint32_t* next,
+                        int32_t* goal_index, uint32_t* n_react);
+
 /* full planner ------------------------------------------------------------ */
 typedef struct orc_stats {
     uint32_t iterations;

@@ -51,6 +51,8 @@ _SIG = {
     "orc_region_update": (None, [_cfgp, _P, _P, _P, _P, _u32, _P, _P, _P, _P, _P, _P]),
     "orc_score": (None, [_cfgp, _P, _P, _P, _P]),
     "orc_faccept": (C.c_float, [_cfgp, _u32, _u32, _u32]),
+    "orc_frontier_update": (C.c_int, [_cfgp, _u32, _u32, _P, _P, _P, _P, _P, _P, _P, _u32, _P, _P, _P, _P, _P, _P, _P,
+                                      _P, _P]),
     "orc_plan": (C.c_int, [_cfgp, _P, _P, _P, _u32, _P, _P, _P, _P, _P, _P, _P, _P, C.POINTER(OrcStats)]),
     "orc_plan_many": (C.c_int, [_cfgp, _u32, _P, _P, _P, _P, _P, C.c_int, C.POINTER(OrcStats), _P]),
     "orc_sincosf_n": (None, [_P, _u32, _P, _P]),
@@ -231,6 +233,38 @@ def score(cfg, total, valid_count, active_bytes):
 
 def faccept(cfg, itr, tree, nxt) -> float:
     return lib().orc_faccept(C.byref(cfg), itr, tree, nxt)
+
+
+def frontier_update(cfg, itr, tree, samples, parents, costs, frontier, child, parent_u, fnext, valid, score, goal):
+    """orc_frontier_update on copies: samples (capacity, D), parents, costs,
+    frontier (capacity), child (D, n) as the stage entries take children,
+    parent_u, fnext (n), region valid counts and scores, goal (>= W floats).
+    Returns the updated arrays, the next expansion list (its first capacity
+    entries), its size R, the appended count, the goal node (-1) and the
+    re-activated count."""
+    S, Cd, D = dims(cfg.model)
+    N = cfg.capacity
+    smp = _c(samples, np.float32).reshape(N, D).copy()
+    par = _c(parents, np.int32).copy()
+    cst = _c(costs, np.float32).copy()
+    fr = _c(frontier, np.uint8).copy()
+    ch = np.ascontiguousarray(np.asarray(child, np.float32).reshape(D, -1).T)
+    pu = _c(parent_u, np.int32)
+    fn = _c(fnext, np.uint8)
+    g = np.zeros(KGMT_MAX_SAMPLE, np.float32)
+    g[: min(len(goal), KGMT_MAX_SAMPLE)] = np.asarray(goal, np.float32)[:KGMT_MAX_SAMPLE]
+    M = np.zeros(N, np.uint32)
+    nM, nxt, nr = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    R = C.c_uint64()
+    gi = C.c_int32()
+    n = len(pu)
+    rc = lib().orc_frontier_update(C.byref(cfg), itr, tree, _p(smp), _p(par), _p(cst), _p(fr), _p(ch) if n else None,
+                                   _p(pu) if n else None, _p(fn) if n else None, n, _p(_c(valid, np.int32)),
+                                   _p(_c(score, np.float32)), _p(g), _p(M), C.byref(nM), C.byref(R), C.byref(nxt),
+                                   C.byref(gi), C.byref(nr))
+    assert rc == 0, rc
+    return {"samples": smp, "parents": par, "costs": cst, "frontier": fr, "M": M[: nM.value], "R": R.value,
+            "next": nxt.value, "goal": gi.value, "react": nr.value}
 
 
 def plan(cfg: KgmtConfig, start, goal, obs, log_iters: bool = False, lib_=None):

@@ -1,7 +1,10 @@
 // Drop-in check of include/kgmt_planner.hpp, written the way the reference's
-// examples/gpu/executionTimeMain.cu drives its planner: read an obstacle CSV,
-// copy the boxes to device memory, construct KGMT, call plan() with the device
-// pointer, read the public result members.
+// examples/gpu/executionTimeMain.cu drives its planner: readObstaclesFromCSV
+// with W_DIM, start and goal arrays of SAMPLE_DIM floats, the obstacles copied
+// to device memory, plan() with the device pointer, then the public result
+// members, including h_controlPathToGoal_ (float*) and the device tree
+// pointers d_treeSamples_ptr_ / d_treeSamplesParentIdxs_ptr_ /
+// d_treeSampleCosts_ptr_.
 //
 // usage: dropin_main <obstacles.csv> <capacity> <max_iter> <seed> <runs>
 // prints one JSON object per plan.
@@ -9,28 +12,22 @@
 
 #include <cstdio>
 #include <cstdlib>
-#include <fstream>
-#include <sstream>
+#include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/kgmt_planner.hpp"
 
-static std::vector<float> read_csv(const char* path)
+static uint64_t bit_sum(const float* v, size_t n)
 {
-    std::ifstream f(path);
-    std::vector<float> v;
-    std::string line;
-    while(std::getline(f, line))
+    uint64_t s = 0;
+    for(size_t i = 0; i < n; ++i)
         {
-            for(char& ch : line)
-                if(ch == ',') ch = ' ';
-            std::stringstream ss(line);
-            float x;
-            while(ss >> x) v.push_back(x);
+            uint32_t b;
+            std::memcpy(&b, &v[i], 4);
+            s += b;
         }
-    v.resize(v.size() / 6 * 6);
-    return v;
+    return s;
 }
 
 int main(int argc, char** argv)
@@ -40,35 +37,70 @@ int main(int argc, char** argv)
             std::fprintf(stderr, "usage: %s obstacles.csv capacity max_iter seed runs\n", argv[0]);
             return 2;
         }
-    std::vector<float> obs = read_csv(argv[1]);
-    const uint n_obs       = (uint)(obs.size() / 6);
-    kgmt_config cfg        = KGMT::preset(KGMT_PRESET_REFERENCE_QUAD);
-    cfg.capacity           = (uint32_t)std::atoi(argv[2]);
-    cfg.max_iter           = (uint32_t)std::atoi(argv[3]);
-    cfg.seed               = (uint64_t)std::atoll(argv[4]);
-    const int runs         = std::atoi(argv[5]);
-
-    float* d_obstacles = nullptr;
-    if(hipMalloc(&d_obstacles, std::max<size_t>(obs.size(), 1) * sizeof(float)) != hipSuccess) return 3;
-    if(n_obs && hipMemcpy(d_obstacles, obs.data(), obs.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    int numObstacles = 0;
+    float* d_obstacles;
+    // --- Load Workspace Obstacles ---
+    std::vector<float> obstacles = readObstaclesFromCSV(argv[1], numObstacles, W_DIM);
+    // --- Transfer Obstacles to device ---
+    if(hipMalloc(&d_obstacles, std::max(numObstacles, 1) * 2 * W_DIM * sizeof(float)) != hipSuccess) return 3;
+    if(numObstacles && hipMemcpy(d_obstacles, obstacles.data(), numObstacles * 2 * W_DIM * sizeof(float),
+                                 hipMemcpyHostToDevice) != hipSuccess)
         return 3;
 
-    // executionTimeMain.cu:43-44: start and goal in the 100-unit workspace
-    float h_initial[KGMT_MAX_SAMPLE] = {10.0f, 8.0f, 5.0f};
-    float h_goal[KGMT_MAX_SAMPLE]    = {80.0f, 95.0f, 90.0f};
+    // executionTimeMain.cu: start and goal in the 100-unit workspace
+    float h_initial[SAMPLE_DIM] = {10.0, 8, 5, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+          h_goal[SAMPLE_DIM]    = {80, 95.0, 90.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     try
         {
+            // the default planner is the reference's configuration (config.h's
+            // quadcopter block); the parity runs take smaller sizes
+            KGMT reference_default;
+            if(reference_default.sampleDim() != SAMPLE_DIM || reference_default.config().capacity != 400000)
+                {
+                    std::fprintf(stderr, "default configuration is not config.h's\n");
+                    return 4;
+                }
+            kgmt_config cfg = KGMT::preset(KGMT_PRESET_REFERENCE_QUAD);
+            cfg.capacity    = (uint32_t)std::atoi(argv[2]);
+            cfg.max_iter    = (uint32_t)std::atoi(argv[3]);
+            cfg.seed        = (uint64_t)std::atoll(argv[4]);
+            const int runs  = std::atoi(argv[5]);
             KGMT kgmt(cfg);
             for(int i = 0; i < runs; ++i)
                 {
-                    kgmt.plan(h_initial, h_goal, d_obstacles, n_obs);
-                    const auto parents = kgmt.treeParents();
-                    long long psum     = 0;
-                    for(int32_t p : parents) psum += p;
+                    // --- Execute planner ---
+                    kgmt.plan(h_initial, h_goal, d_obstacles, numObstacles);
+                    const uint n   = kgmt.h_treeSize_;
+                    const uint cap = cfg.capacity;
+                    std::vector<float> rows((size_t)cap * SAMPLE_DIM), costs(cap);
+                    std::vector<int> parents(cap);
+                    if(hipMemcpy(rows.data(), kgmt.d_treeSamples_ptr_, rows.size() * sizeof(float),
+                                 hipMemcpyDeviceToHost) != hipSuccess ||
+                       hipMemcpy(parents.data(), kgmt.d_treeSamplesParentIdxs_ptr_, cap * sizeof(int),
+                                 hipMemcpyDeviceToHost) != hipSuccess ||
+                       hipMemcpy(costs.data(), kgmt.d_treeSampleCosts_ptr_, cap * sizeof(float),
+                                 hipMemcpyDeviceToHost) != hipSuccess)
+                        return 5;
+                    // entries past the tree as the reference's fills leave them
+                    bool tail = true;
+                    for(size_t k = (size_t)n * SAMPLE_DIM; k < rows.size(); ++k) tail = tail && rows[k] == 0.0f;
+                    for(uint k = n; k < cap; ++k) tail = tail && parents[k] == -1 && costs[k] == 0.0f;
+                    rows.resize((size_t)n * SAMPLE_DIM), costs.resize(n), parents.resize(n);
+                    // the device members agree with the host getters
+                    const bool same = rows == kgmt.treeSamples() && parents == kgmt.treeParents() &&
+                                      costs == kgmt.treeCosts();
+                    long long psum = 0;
+                    for(int p : parents) psum += p;
                     std::printf("{\"run\": %d, \"tree_size\": %u, \"iterations\": %u, \"goal\": %d, \"path_rows\": %u, "
-                                "\"parent_sum\": %lld, \"time_ms\": %.3f}\n",
+                                "\"parent_sum\": %lld, \"sample_bits\": %llu, \"cost_bits\": %llu, "
+                                "\"path_bits\": %llu, \"device_tree_matches_host\": %s, \"device_tree_tail_ok\": %s, "
+                                "\"time_ms\": %.3f}\n",
                                 i, kgmt.h_treeSize_, kgmt.h_itr_, kgmt.h_pathToGoal_, kgmt.h_pathRows_, psum,
-                                kgmt.result().time_ms);
+                                (unsigned long long)bit_sum(rows.data(), rows.size()),
+                                (unsigned long long)bit_sum(costs.data(), costs.size()),
+                                (unsigned long long)bit_sum(kgmt.h_controlPathToGoal_,
+                                                            (size_t)kgmt.h_pathRows_ * SAMPLE_DIM),
+                                same ? "true" : "false", tail ? "true" : "false", kgmt.result().time_ms);
                 }
         }
     catch(const kgmt::Error& e)
@@ -76,6 +108,7 @@ int main(int argc, char** argv)
             std::fprintf(stderr, "%s\n", e.what());
             return 1;
         }
+    // --- Free memory ---
     (void)hipFree(d_obstacles);
     return 0;
 }

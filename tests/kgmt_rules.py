@@ -77,10 +77,58 @@ def goal_distance(pos, goal, w):
     return np.sqrt(acc)
 
 
-def replay(orc, q):
+def frontier_update(orc, cfg, itr, tree, samples, parents, costs, frontier, child, parent_u, fnext, valid, score,
+                    goal):
+    """The frontier update of iteration itr on copies of the arrays: samples
+    (capacity, D), parents, costs, frontier (capacity), child (D, n),
+    parent_u, fnext (n), region valid counts and scores. Returns a dict with
+    the updated arrays, the repeat count of every node, the next expansion
+    list (its first capacity entries), its size R, the appended count, the
+    goal node (-1) and counts of the rules that acted."""
+    N, D, w = cfg.capacity, cfg.dims[2], cfg.wdim
+    samples, parents, costs = samples.copy(), parents.copy(), costs.copy()
+    frontier = frontier.astype(bool).copy()
+    acc = np.flatnonzero(np.asarray(fnext) != 0)
+    avail = N - tree
+    nxt = min(len(acc), avail)
+    fa = reference_faccept(cfg, itr, tree, nxt)
+    rep = np.zeros(N, np.int64)
+    # re-activation of old nodes (repeat counts were zeroed first); the
+    # region of an old node is recomputed from its sample, as the reference does
+    U = react_uniforms(orc, cfg, itr, tree)
+    pos = np.zeros((tree, 3), F32)
+    pos[:, :w] = samples[:tree, :w]
+    sc = score[orc.regions(cfg, pos)[0]]
+    react = ~frontier[:tree] & (U <= sc + fa)
+    by_fa = react & (U > sc)
+    frontier[:tree][react] = True
+    rep[:tree][react] = 1
+    # append the accepted children
+    u = acc[:nxt]
+    new = tree + np.arange(nxt)
+    samples[new] = child[:, u].T
+    parents[new] = parent_u[u]
+    d = goal_distance(samples[new, :w], goal, w)
+    costs[new] = d
+    frontier[new] = True
+    cpos = np.zeros((nxt, 3), F32)
+    cpos[:, :w] = samples[new, :w]
+    r1 = orc.regions(cfg, cpos)[0]
+    sparse = valid[r1] < cfg.sparse_valid_below
+    rep[new] = np.where(sparse, cfg.repeat_sparse, cfg.repeat_dense)
+    hits = np.flatnonzero(d < F32(cfg.goal_thresh))
+    M = np.repeat(np.arange(tree + nxt), rep[:tree + nxt])
+    return {"samples": samples, "parents": parents, "costs": costs, "frontier": frontier.astype(np.uint8),
+            "rep": rep, "M": M[:N], "R": len(M), "next": nxt, "goal": tree + int(hits[0]) if len(hits) else -1,
+            "react": int(react.sum()), "react_by_faccept": int(by_fa.sum()), "sparse": int(sparse.sum()),
+            "dense": int((~sparse).sum()), "clamped": len(acc) > avail, "hits": len(hits)}
+
+
+def replay(orc, q, hook=None):
     """Plan query q with the rules above. Returns (out, log): out has the
     fields of oracle.plan's result; log has one dict per iteration counting
-    how often each rule acted."""
+    how often each rule acted. hook(itr, inputs) (optional) receives the
+    inputs of every iteration's frontier update."""
     cfg = q.cfg
     S, Cd, D = dims(cfg.model)
     w = cfg.wdim
@@ -93,14 +141,10 @@ def replay(orc, q):
     costs = np.zeros(N, F32)
     frontier = np.zeros(N, bool)
     rep = np.zeros(N, np.int64)
-    r1node = np.zeros(N, np.int32)
     samples[0] = q.start[:D]
     parents[0] = -1
     frontier[0] = True
     rep[0] = cfg.root_repeat
-    root_pos = np.zeros((1, 3), F32)
-    root_pos[0, :w] = q.start[:w]
-    r1node[0] = orc.regions(cfg, root_pos)[0][0]
     total = np.zeros(R1, np.int32)
     valid = np.zeros(R1, np.int32)
     score = np.zeros(R1, F32)
@@ -126,36 +170,22 @@ def replay(orc, q):
         active = active.copy()
         active[r2[okb]] = 1
         score = orc.score(cfg, total, valid, active)
-        acc = np.flatnonzero(fnext)
-        nxt = min(len(acc), avail)
-        fa = reference_faccept(cfg, itr, tree, nxt)
-        # re-activation of old nodes: repeat counts were zeroed first
-        rep[:] = 0
-        U = react_uniforms(orc, cfg, itr, tree)
-        sc = score[r1node[:tree]]
-        react = ~frontier[:tree] & (U <= sc + fa)
-        by_fa = react & (U > sc)
-        frontier[:tree][react] = True
-        rep[:tree][react] = 1
-        # append the accepted children
-        u = acc[:nxt]
-        new = tree + np.arange(nxt)
-        samples[new] = child[:, u].T
-        parents[new] = parent[u]
-        d = goal_distance(samples[new, :w], goal, w)
-        costs[new] = d
-        frontier[new] = True
-        r1node[new] = r1[u]
-        sparse = valid[r1[u]] < cfg.sparse_valid_below
-        rep[new] = np.where(sparse, cfg.repeat_sparse, cfg.repeat_dense)
-        hits = np.flatnonzero(d < F32(cfg.goal_thresh))
-        log.append({"R": R, "children": nC, "per": per, "next": nxt, "react": int(react.sum()),
-                    "react_by_faccept": int(by_fa.sum()), "sparse": int(sparse.sum()),
-                    "dense": int((~sparse).sum()), "clamped": len(acc) > avail, "hits": len(hits)})
-        old_tree = tree
+        inputs = dict(itr=itr, tree=tree, samples=samples, parents=parents, costs=costs,
+                      frontier=frontier.astype(np.uint8), child=child, parent_u=parent, fnext=fnext.astype(np.uint8),
+                      valid=valid.copy(), score=score.copy())
+        if hook:
+            hook(itr, inputs)
+        f = frontier_update(orc, cfg, itr, tree, samples, parents, costs, frontier, child, parent, fnext, valid, score,
+                            goal)
+        samples, parents, costs = f["samples"], f["parents"], f["costs"]
+        frontier, rep = f["frontier"].astype(bool), f["rep"]
+        nxt = f["next"]
+        log.append({"R": R, "children": nC, "per": per, "next": nxt, "react": f["react"],
+                    "react_by_faccept": f["react_by_faccept"], "sparse": f["sparse"], "dense": f["dense"],
+                    "clamped": f["clamped"], "hits": f["hits"]})
         tree += nxt
-        if len(hits):
-            goal_idx = old_tree + int(hits[0])
+        if f["goal"] >= 0:
+            goal_idx = f["goal"]
             break
         if tree == N:
             break
@@ -179,3 +209,62 @@ def replay(orc, q):
     out = {"stats": stats, "samples": samples[:tree], "parents": parents[:tree], "costs": costs[:tree],
            "path": path, "total": total, "valid": valid, "score": score}
     return out, log
+
+
+def capture_updates(orc, q, iterations):
+    """Inputs of the frontier update at the given iterations of query q's
+    plan (a list of dicts, with the query's configuration and goal)."""
+    want = set(iterations)
+    out = []
+
+    def hook(itr, inputs):
+        if itr in want:
+            out.append(dict(inputs, cfg=q.cfg, goal=np.asarray(q.goal, F32),
+                            samples=inputs["samples"].copy(), parents=inputs["parents"].copy(),
+                            costs=inputs["costs"].copy(), child=inputs["child"].copy(),
+                            parent_u=inputs["parent_u"].copy()))
+
+    replay(orc, q, hook)
+    return out
+
+
+def update_variants(state):
+    """Edge cases built from one captured state: the capacity cut so the
+    accepted children overflow it, the goal moved onto accepted children
+    (several hits), no children at all, every old node already in the
+    frontier."""
+    cfg, tree = state["cfg"], state["tree"]
+    flagged = np.flatnonzero(state["fnext"])
+    D = cfg.dims[2]
+    out = {}
+    cap = tree + len(flagged) // 2
+    if len(flagged) >= 4 and cap < cfg.capacity:
+        c = dict(state, cfg=cfg.replace(capacity=cap))
+        c["samples"] = state["samples"][:cap].copy()
+        c["parents"] = state["parents"][:cap].copy()
+        c["costs"] = state["costs"][:cap].copy()
+        c["frontier"] = state["frontier"][:cap].copy()
+        out["clamped"] = c
+    if len(flagged) >= 4:
+        g = dict(state)
+        goal = np.zeros(D, F32)
+        goal[:cfg.wdim] = state["child"][:cfg.wdim, flagged[len(flagged) // 3]]
+        g["goal"] = goal
+        out["goal_hits"] = g
+    e = dict(state)
+    e["child"] = np.zeros((D, 0), F32)
+    e["parent_u"] = np.zeros(0, np.int32)
+    e["fnext"] = np.zeros(0, np.uint8)
+    out["no_children"] = e
+    f = dict(state)
+    f["frontier"] = state["frontier"].copy()
+    f["frontier"][:tree] = 1
+    out["no_reactivation"] = f
+    return out
+
+
+def run_update(fn, orc, st):
+    """Call a frontier-update implementation (kgmt_rules.frontier_update style
+    signature) on a captured state."""
+    return fn(orc, st["cfg"], st["itr"], st["tree"], st["samples"], st["parents"], st["costs"], st["frontier"],
+              st["child"], st["parent_u"], st["fnext"], st["valid"], st["score"], st["goal"])

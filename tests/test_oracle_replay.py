@@ -17,7 +17,8 @@ import numpy as np
 import pytest
 
 from kgmt_amd import workloads as W
-from kgmt_rules import reference_faccept, react_uniforms, replay
+from kgmt_rules import (capture_updates, frontier_update, reference_faccept, react_uniforms, replay, run_update,
+                        update_variants)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SRC = os.path.join(ROOT, "oracle", "kgmt_oracle.c")
@@ -104,7 +105,8 @@ def test_faccept_rule_matches_oracle(orc):
 MUTATIONS = [
     ("repeat counts swapped", "cfg->repeat_sparse : cfg->repeat_dense", "cfg->repeat_dense : cfg->repeat_sparse"),
     ("sparse test <=", "validc[r1] < cfg->sparse_valid_below", "validc[r1] <= cfg->sparse_valid_below"),
-    ("fAccept dropped", "U <= score[r1node[i]] + fA", "U <= score[r1node[i]]"),
+    ("fAccept dropped", "U <= score[orc_region(cfg, samples + (size_t)i * D)] + fA",
+     "U <= score[orc_region(cfg, samples + (size_t)i * D)]"),
     ("re-activated order reversed", "for(uint32_t k = 0; k < nReact && nM < N; ++k) M[nM++] = react[k];",
      "for(uint32_t k = nReact; k-- > 0 && nM < N;) M[nM++] = react[k];"),
     ("re-activation ignores the frontier flag", "if(frontier[i] == 0)", "if(1)"),
@@ -145,3 +147,54 @@ def test_replay_detects_altered_oracle(label, old, new, replays, tmp_path, orc):
     assert src.count(old) == 1, f"mutation anchor not found once: {old}"
     lib = _build_variant(tmp_path, src.replace(old, new), orc)
     assert _differences(lib, replays, orc), f"the replay did not detect: {label}"
+
+
+UPDATE_CASES = {
+    "di6d": ("di6d", 0, dict(capacity=1 << 13, max_iter=60), (1, 2, 5)),
+    "dubins5d": ("dubins5d", 1, dict(capacity=1 << 13, max_iter=60), (2, 4)),
+    "quad12d": ("quad12d", 0, dict(capacity=1 << 12, max_iter=12), (2, 6)),
+    "unicycle2d": ("unicycle2d", 2, dict(capacity=1 << 13, max_iter=60), (3,)),
+}
+
+
+def update_states(orc):
+    """Captured frontier-update inputs of mid-plan iterations, plus the edge
+    cases derived from each (capacity clamp, goal hits, no children, no
+    re-activation candidates)."""
+    out = []
+    for name, (wl, seed, kw, its) in UPDATE_CASES.items():
+        for st in capture_updates(orc, W.make_query(wl, seed, **kw), its):
+            out.append((f"{name}/itr{st['itr']}", st))
+            out += [(f"{name}/itr{st['itr']}/{k}", v) for k, v in update_variants(st).items()]
+    return out
+
+
+def compare_update(want, got):
+    """First differing output of two frontier updates, or None."""
+    for k in ("next", "R", "goal", "react"):
+        if int(want[k]) != int(got[k]):
+            return k
+    for k in ("samples", "parents", "costs", "frontier", "M"):
+        if not np.array_equal(_bits(np.asarray(want[k])), _bits(np.asarray(got[k]))):
+            return k
+    return None
+
+
+def oracle_update(orc, cfg, itr, tree, samples, parents, costs, frontier, child, parent_u, fnext, valid, score, goal):
+    return orc.frontier_update(cfg, itr, tree, samples, parents, costs, frontier, child, parent_u, fnext, valid,
+                               score, goal)
+
+
+def test_frontier_update_stage_against_rules(orc):
+    """orc_frontier_update (the oracle's stage function, the one orc_plan
+    calls) against the numpy rule on mid-plan states and edge cases."""
+    states = update_states(orc)
+    seen = {"clamped": 0, "goal": 0, "react": 0}
+    for name, st in states:
+        want = run_update(frontier_update, orc, st)
+        got = run_update(oracle_update, orc, st)
+        assert compare_update(want, got) is None, (name, compare_update(want, got))
+        seen["clamped"] += want["clamped"]
+        seen["goal"] += want["goal"] >= 0
+        seen["react"] += want["react"] > 0
+    assert all(v > 0 for v in seen.values()), seen
