@@ -300,17 +300,18 @@ def run_batch(n, seed=0, tune=True, launch=None, sequential=True, profile=True):
     return out
 
 
-def run_refine(n=4096, seed=0, cpu_sample=0):
+def run_refine(n=4096, seed=0, cpu_sample=0, objective="configured"):
     """Adam refinement of n quadcopter control sequences (T ~ U{20..80},
-    dt 0.01, 50 steps, lr 1e-2) through kgmt_refine; the CPU figure is the
-    oracle's FP64 cost+adjoint on cpu_sample trajectories, scaled to the
-    50 steps, one thread."""
+    dt 0.01, 50 steps, lr 1e-2) through kgmt_refine, cost weights of
+    `objective` (workloads.OBJECTIVES); the CPU figure is the oracle's FP64
+    cost+adjoint on cpu_sample trajectories, scaled to the 50 steps, one
+    thread."""
     import numpy as np
     import torch
     from kgmt_amd import adjoint as A
     from kgmt_amd import workloads as W
 
-    rb = W.make_refine_batch(n, seed=seed)
+    rb = W.make_refine_batch(n, seed=seed, objective=objective)
     d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
     u0, x0, goal, steps, obs = d(rb.u), d(rb.x0), d(rb.goal), d(rb.steps), d(rb.obstacles)
     u = u0.clone()
@@ -322,8 +323,12 @@ def run_refine(n=4096, seed=0, cpu_sample=0):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     h = hist.cpu().numpy()
+    p = rb.params
     out = {
         "trajectories": n,
+        "objective": f"{objective}: {p.w_effort:g}*|u - u_ref|^2 (u_ref thrust {p.u_ref[0]:g}) + "
+                     f"{p.w_obstacle:g}*softplus obstacle penalty (margin {p.margin:g}, tau {p.tau:g}) + "
+                     f"{p.w_goal:g}*|p_T - goal|^2",
         "adam_steps": int(rb.params.iterations),
         "ms": round(dt * 1e3, 3),
         "trajectories_per_s": round(n / dt, 1),
@@ -521,7 +526,8 @@ def main(argv=None):
             line["config"] = {"workload": "batched planner, " + line["batched"]["mix"], "parallelism": "single GPU"}
     if a.refine:
         print(f"bench: refine {a.refine}", file=sys.stderr, flush=True)
-        line["refine"] = run_refine(a.refine, cpu_sample=0 if a.no_cpu else 64)
+        line["refine"] = run_refine(a.refine, cpu_sample=0 if a.no_cpu else 64, objective="configured")
+        line["refine_hover_referenced"] = run_refine(a.refine, cpu_sample=0, objective="hover")
     print(json.dumps(line), flush=True)
     return 0
 

@@ -50,7 +50,7 @@
 extern "C" {
 #endif
 
-#define KGMT_ABI_VERSION 3
+#define KGMT_ABI_VERSION 4
 
 /* ---- status codes -------------------------------------------------------- */
 enum {
@@ -223,9 +223,28 @@ int kgmt_synchronize(kgmt_ctx* ctx);
  * last iteration closes, so the call does not wait for the few launches that
  * were queued past it; they exit at once on the context's stream, and the
  * tree and region getters wait for them. The obstacle rows are not read after
- * the call returns. */
+ * the call returns. Device obstacle rows are read on the context's own
+ * (non-blocking) stream: work that fills them asynchronously on another
+ * stream must be complete before the call (synchronise that stream, or share
+ * it with kgmt_set_stream). */
 int kgmt_plan(kgmt_ctx* ctx, const float* start, const float* goal, const float* obstacles,
               uint32_t n_obs, int32_t obstacles_on_device, kgmt_result* out);
+/* Step-by-step planning, as KGMT::planBench (KGMT.cu:107-165) runs the
+ * iterations and dumps the state after each (KGMT.cu:149-153, 405-488):
+ * kgmt_plan_begin prepares a plan with the arguments of kgmt_plan without
+ * running an iteration; each kgmt_plan_step runs one iteration, waits for it
+ * and fills *out with the plan so far (iterations run, tree size, success);
+ * *done becomes 1 once the plan has ended (goal reached, full tree or
+ * max_iter), and further steps do nothing. The getters (tree, path, regions,
+ * frontier) read the state after the last step. Results equal kgmt_plan's. */
+int kgmt_plan_begin(kgmt_ctx* ctx, const float* start, const float* goal, const float* obstacles,
+                    uint32_t n_obs, int32_t obstacles_on_device);
+int kgmt_plan_step(kgmt_ctx* ctx, kgmt_result* out, int32_t* done);
+/* Frontier flags (d_frontier_) and repeat counts (d_activeFrontierRepeatCount_:
+ * 15 or 1 for a new node, 1 for a re-activated one, 0 otherwise,
+ * KGMT.cu:336-339, 373) of the tree nodes after the last iteration. Any
+ * pointer may be NULL; *n receives the tree size. */
+int kgmt_get_frontier(kgmt_ctx* ctx, uint8_t* frontier, uint32_t* repeat, uint32_t cap, uint32_t* n);
 /* Host copies of the tree of the last plan: samples as rows of D floats
  * (row-major, like d_treeSamples_), parents (-1 for the root), costs
  * (distance of the node position to the goal). Any pointer may be NULL.

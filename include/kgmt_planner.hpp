@@ -23,8 +23,12 @@
 //     setDeviceTree(false) skips writing them.
 #pragma once
 
+#include <chrono>
 #include <cstdint>
+#include <cstdio>
+#include <filesystem>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <sstream>
 #include <stdexcept>
@@ -84,6 +88,43 @@ inline void check(int rc)
 }  // namespace kgmt
 
 typedef unsigned int uint;
+
+namespace kgmt
+{
+// include/helper/helper.cuh:56-85: rows x cols values, comma separated, fixed
+// six decimals for floating point (integers as they are), appended or new
+template <typename T>
+inline void writeVectorToCSV(const std::vector<T>& vec, const std::string& filename, int rows, int cols,
+                             bool append = false)
+{
+    std::ofstream file;
+    if(append)
+        file.open(filename, std::ios_base::app);
+    else
+        file.open(filename);
+    file << std::fixed << std::setprecision(6);
+    for(int i = 0; i < rows; i++)
+        {
+            for(int j = 0; j < cols; j++)
+                {
+                    file << vec[(size_t)i * cols + j];
+                    if(j < cols - 1) file << ",";
+                }
+            file << std::endl;
+        }
+}
+
+// include/helper/helper.cuh:95-109: one value appended, ten decimals for
+// floating point
+template <typename T>
+inline void writeValueToCSV(const T& value, const std::string& filename)
+{
+    std::ofstream file;
+    file.open(filename, std::ios_base::app);
+    if constexpr(std::is_floating_point_v<std::decay_t<T>>) file << std::fixed << std::setprecision(10);
+    file << value << std::endl;
+}
+}  // namespace kgmt
 
 // src/helper/helper.cu:13-39: every number of the file in order (a comma
 // after a number is skipped; a line stops at the first token that is not a
@@ -164,6 +205,12 @@ public:
     void plan(float* h_initial, float* h_goal, float* d_obstacles_ptr, uint h_obstaclesCount) override
     {
         kgmt::check(kgmt_plan(ctx_, h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount, 1, &result_));
+        publish();
+    }
+
+    // result members from the last plan or plan step
+    void publish()
+    {
         h_treeSize_   = result_.tree_size;
         h_itr_        = result_.iterations;
         h_pathToGoal_ = result_.success ? result_.goal_index : 0;
@@ -184,11 +231,83 @@ public:
             }
     }
 
-    // KGMT::planBench, src/planners/KGMT.cu:107-165: plan() (the
-    // per-iteration CSV dump of the reference is kgmt_amd.export in Python)
-    void planBench(float* h_initial, float* h_goal, float* d_obstacles_ptr, uint h_obstaclesCount, int /*benchItr*/)
+    // KGMT::planBench, src/planners/KGMT.cu:107-165: the plan's iterations one
+    // at a time, each followed by writeDeviceVectorsToCSV(benchItr) into
+    // Data/ under the current directory (KGMT.cu:405-488); prints each
+    // iteration and the execution time as the reference does
+    void planBench(float* h_initial, float* h_goal, float* d_obstacles_ptr, uint h_obstaclesCount, int benchItr)
     {
-        plan(h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount);
+        const auto t0 = std::chrono::steady_clock::now();
+        kgmt::check(kgmt_plan_begin(ctx_, h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount, 1));
+        h_treeSize_ = 1, h_itr_ = 0, h_pathToGoal_ = 0, h_costToGoal_ = 0.0f, h_pathRows_ = 0;
+        h_frontierSize_ = 0;
+        uint64_t expandedBefore = 0;
+        int32_t done            = 0;
+        uint frontierNow        = 1;  // the root
+        while(!done)
+            {
+                std::printf("Iteration: %u, Tree Size: %u, Frontier Size: %u\n", h_itr_ + 1, h_treeSize_,
+                            h_frontierSize_);
+                h_frontierSize_ = frontierNow;
+                kgmt::check(kgmt_plan_step(ctx_, &result_, &done));
+                publish();
+                h_expandedNodes_ = (uint)(result_.expanded - expandedBefore);
+                expandedBefore   = result_.expanded;
+                writeDeviceVectorsToCSV(benchItr);
+                frontierNow = 0;
+                for(uint8_t f : frontier_) frontierNow += f;
+            }
+        if(h_pathToGoal_) std::printf("Goal Reached\n");
+        const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::cout << "KGMT execution time: " << s << " seconds. Iterations: " << h_itr_
+                  << ". Tree Size: " << h_treeSize_ << std::endl;
+    }
+
+    // KGMT::writeDeviceVectorsToCSV, src/planners/KGMT.cu:405-488: the state
+    // after the last iteration under Data/
+    void writeDeviceVectorsToCSV(int itr)
+    {
+        namespace fs    = std::filesystem;
+        const uint cap  = cfg_.capacity;
+        const bool app  = h_itr_ != 0;
+        const auto dir  = [&](const char* sub) {
+            const std::string d = std::string("Data/") + sub + "/" + sub + std::to_string(itr);
+            fs::create_directories(d);
+            return d + "/";
+        };
+        std::vector<float> samples((size_t)cap * D_, 0.0f), costs(h_treeSize_);
+        std::vector<int> parents(cap, -1);
+        uint32_t n = 0;
+        kgmt::check(kgmt_get_tree(ctx_, samples.data(), parents.data(), costs.data(), cap, &n));
+        int32_t nr = 0;
+        uint32_t nreg = 0;
+        kgmt::check(kgmt_get_regions(ctx_, nullptr, nullptr, nullptr, 0, &nreg));
+        std::vector<int> total(nreg), valid(nreg);
+        std::vector<float> score(nreg);
+        kgmt::check(kgmt_get_regions(ctx_, total.data(), valid.data(), score.data(), nreg, &nreg));
+        nr = (int32_t)nreg;
+        frontier_.assign(h_treeSize_, 0);
+        std::vector<uint32_t> rep(h_treeSize_);
+        kgmt::check(kgmt_get_frontier(ctx_, frontier_.data(), rep.data(), h_treeSize_, &n));
+        std::vector<int> fr(cap, 0);
+        std::vector<uint> repeat(cap, 0);
+        for(uint i = 0; i < h_treeSize_; ++i) fr[i] = frontier_[i], repeat[i] = rep[i];
+        kgmt::writeVectorToCSV(samples, dir("Samples") + "samples" + std::to_string(h_itr_) + ".csv", cap, D_, app);
+        kgmt::writeVectorToCSV(parents, dir("Parents") + "parents" + std::to_string(h_itr_) + ".csv", cap, 1, app);
+        kgmt::writeVectorToCSV(total, dir("TotalCountPerVertex") + "totalCountPerVertex.csv", 1, nr, app);
+        kgmt::writeVectorToCSV(valid, dir("ValidCountPerVertex") + "validCountPerVertex.csv", 1, nr, app);
+        kgmt::writeVectorToCSV(fr, dir("Frontier") + "frontier.csv", 1, cap, app);
+        kgmt::writeVectorToCSV(repeat, dir("FrontierRepeatCount") + "frontierRepeatCount.csv", 1, cap, app);
+        kgmt::writeVectorToCSV(score, dir("VertexScores") + "vertexScores.csv", 1, nr, app);
+        kgmt::writeValueToCSV(h_frontierSize_, dir("FrontierSize") + "frontierSize.csv");
+        kgmt::writeValueToCSV(h_treeSize_, dir("TreeSize") + "treeSize.csv");
+        kgmt::writeValueToCSV(h_expandedNodes_, dir("ExpandedNodes") + "expandedNodes.csv");
+        const std::string pathFile = dir("ControlPathToGoal") + "controlPathToGoal.csv";
+        if(h_pathToGoal_)
+            {
+                std::vector<float> path(h_controlPathToGoal_, h_controlPathToGoal_ + (size_t)h_pathRows_ * D_);
+                kgmt::writeVectorToCSV(path, pathFile, (int)h_pathRows_, D_, app);
+            }
     }
 
     void setSeed(uint64_t seed) { kgmt::check(kgmt_set_seed(ctx_, seed)); }
@@ -225,6 +344,9 @@ public:
         return c;
     }
 
+    uint h_frontierSize_ = 0;   // frontier size at the start of the last iteration (planBench)
+    uint h_expandedNodes_ = 0;  // children propagated by the last iteration (planBench)
+
     const kgmt_result& result() const { return result_; }
     const kgmt_config& config() const { return cfg_; }
     int sampleDim() const { return D_; }
@@ -236,4 +358,5 @@ private:
     kgmt_result result_{};
     int32_t S_ = 0, C_ = 0, D_ = 0;
     bool deviceTree_ = true;
+    std::vector<uint8_t> frontier_;
 };

@@ -44,6 +44,9 @@ SIGNATURES = {
     "kgmt_set_stream": (C.c_int, [_P, _P]),
     "kgmt_synchronize": (C.c_int, [_P]),
     "kgmt_plan": (C.c_int, [_P, _P, _P, _P, _u32, _i32, _resp]),
+    "kgmt_plan_begin": (C.c_int, [_P, _P, _P, _P, _u32, _i32]),
+    "kgmt_plan_step": (C.c_int, [_P, _resp, C.POINTER(_i32)]),
+    "kgmt_get_frontier": (C.c_int, [_P, _P, _P, _u32, C.POINTER(_u32)]),
     "kgmt_get_tree": (C.c_int, [_P, _P, _P, _P, _u32, C.POINTER(_u32)]),
     "kgmt_get_tree_device": (C.c_int, [_P, C.POINTER(KgmtDeviceTree)]),
     "kgmt_get_path": (C.c_int, [_P, _P, _u32, C.POINTER(_u32)]),

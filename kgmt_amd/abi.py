@@ -6,7 +6,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-KGMT_ABI_VERSION = 3
+KGMT_ABI_VERSION = 4
 
 KGMT_OK = 0
 KGMT_E_ARG = -1

@@ -575,6 +575,108 @@ struct Engine
         return KGMT_OK;
     }
 
+    // Step-by-step planning (planBench): begin() prepares the plan as run()
+    // does, step() runs one iteration and waits for it, so that the state
+    // after every iteration can be read.
+    bool stepping     = false;
+    uint32_t stepItr  = 0;
+    int begin()
+    {
+        mirValid = false;
+        HIP_TRY(hipMemcpyAsync(dq, hq.data(), sizeof(QueryDesc) * nq, hipMemcpyHostToDevice, stream));
+        if(counterDirty) HIP_TRY(hipMemsetAsync(W.counter, 0, sizeof(uint32_t), stream));
+        counterDirty = true;
+        __atomic_store_n(pinned, 0u, __ATOMIC_RELEASE);
+        if(hostRes) hostRes->iterations = 0;
+        HIP_TRY(launch_init(dq, ds, P, W, sceneBuild ? sceneRows : nullptr, stream));
+        int rc = pull_state();
+        if(rc) return rc;
+        stepping = true;
+        stepItr  = 0;
+        itersRun = 0;
+        return KGMT_OK;
+    }
+
+    // the device query states, with each iteration's closing values (the
+    // next-state fields) taken as current, as the next score kernel would
+    int pull_state()
+    {
+        HIP_TRY(hipMemcpyAsync(hs.data(), ds, sizeof(QueryState) * nq, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        for(uint32_t q = 0; q < nq; ++q)
+            {
+                QueryState& st = hs[q];
+                st.itr = st.itrN, st.treeSize = st.treeSizeN, st.nChildren = st.nChildrenN, st.per = st.perN;
+                st.done = st.doneN, st.success = st.successN, st.finishItr = st.finishItrN;
+                st.expanded = st.expandedN, st.flagged = st.flaggedN;
+            }
+        return KGMT_OK;
+    }
+
+    int step(int32_t* done)
+    {
+        if(!stepping) return fail(KGMT_E_ARG, "no plan in progress (kgmt_plan_begin)");
+        mirValid = false;
+        bool all = true;
+        for(uint32_t q = 0; q < nq; ++q) all = all && hs[q].done;
+        if(!all && stepItr < maxIter)
+            {
+                int rc = enqueue_iteration(stepItr & 1u);
+                if(rc) return rc;
+                ++stepItr;
+                itersRun = stepItr;
+                if((rc = pull_state())) return rc;
+                all = true;
+                for(uint32_t q = 0; q < nq; ++q) all = all && hs[q].done;
+            }
+        if(all || stepItr >= maxIter)
+            {
+                // goal paths of solved queries (a single query has walked its own)
+                HIP_TRY(launch_paths(dq, ds, P, nq, stream));
+                HIP_TRY(hipMemcpyAsync(hs.data(), ds, sizeof(QueryState) * nq, hipMemcpyDeviceToHost, stream));
+                HIP_TRY(hipStreamSynchronize(stream));
+                if(hostRes && hs[0].success && hostRes->pathRows) hs[0].pathRows = hostRes->pathRows;
+                stepping     = false;
+                counterDirty = false;
+                all          = true;
+            }
+        *done = all ? 1 : 0;
+        return KGMT_OK;
+    }
+
+    // frontier flags and repeat counts of query q's tree nodes after the
+    // last iteration: 1 for an old node it re-activated, 15 or 1 for a node it
+    // appended (its region's valid count, final for that iteration, below
+    // sparse_valid_below or not), 0 otherwise (KGMT.cu:336-339, 373)
+    int get_frontier(uint32_t q, uint8_t* frontier, uint32_t* repeat, uint32_t cap, uint32_t* n)
+    {
+        HIP_TRY(hipStreamSynchronize(stream));  // a plan returns before its trailing launches drain
+        QueryState st;
+        HIP_TRY(hipMemcpy(&st, ds + q, sizeof(QueryState), hipMemcpyDeviceToHost));
+        const QueryDesc& qd = hq[q];
+        const uint32_t ts   = st.treeSizeN;
+        if(n) *n = ts;
+        if(!frontier && !repeat) return KGMT_OK;
+        if(cap < ts) return fail(KGMT_E_CAPACITY, "frontier output buffer too small");
+        if(frontier) HIP_TRY(hipMemcpy(frontier, P.frontier + qd.nodeOff, ts, hipMemcpyDeviceToHost));
+        if(repeat)
+            {
+                const uint32_t next = std::min(st.next, ts), old = ts - next;
+                std::vector<uint8_t> react(old);
+                std::vector<int32_t> r1(next), valid(qd.R1);
+                if(old) HIP_TRY(hipMemcpy(react.data(), P.react + qd.nodeOff, old, hipMemcpyDeviceToHost));
+                if(next)
+                    HIP_TRY(hipMemcpy(r1.data(), P.r1node + qd.nodeOff + old, sizeof(int32_t) * next,
+                                      hipMemcpyDeviceToHost));
+                HIP_TRY(hipMemcpy(valid.data(), P.validc + qd.regOff, sizeof(int32_t) * qd.R1, hipMemcpyDeviceToHost));
+                for(uint32_t i = 0; i < old; ++i) repeat[i] = react[i] ? 1u : 0u;
+                for(uint32_t j = 0; j < next; ++j)
+                    repeat[old + j] = valid[r1[j]] < qd.cfg.sparse_valid_below ? qd.cfg.repeat_sparse
+                                                                                  : qd.cfg.repeat_dense;
+            }
+        return KGMT_OK;
+    }
+
     void result(uint32_t q, kgmt_result* out, double wallMs) const
     {
         const QueryState& s = hs[q];
@@ -927,6 +1029,39 @@ int kgmt_plan(kgmt_ctx* ctx, const float* start, const float* goal, const float*
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     e.result(0, out, ms);
     return KGMT_OK;
+}
+
+int kgmt_plan_begin(kgmt_ctx* ctx, const float* start, const float* goal, const float* obstacles, uint32_t n_obs,
+                    int32_t on_device)
+{
+    if(!ctx || !start || !goal) return fail(KGMT_E_ARG, "null argument");
+    Engine& e     = ctx->eng;
+    QueryDesc& qd = e.hq[0];
+    std::memset(qd.start, 0, sizeof(qd.start));
+    std::memcpy(qd.start, start, sizeof(float) * qd.D);
+    qd.goal[0] = goal[0], qd.goal[1] = goal[1], qd.goal[2] = qd.wdim == 3 ? goal[2] : 0.0f;
+    int rc = upload_obstacles(e, obstacles, nullptr, nullptr, n_obs, on_device);
+    if(rc) return rc;
+    return e.begin();
+}
+
+int kgmt_plan_step(kgmt_ctx* ctx, kgmt_result* out, int32_t* done)
+{
+    if(!ctx || !out || !done) return fail(KGMT_E_ARG, "null argument");
+    Engine& e     = ctx->eng;
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc        = e.step(done);
+    if(rc) return rc;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    e.result(0, out, ms);
+    if(!*done) out->iterations = e.stepItr;
+    return KGMT_OK;
+}
+
+int kgmt_get_frontier(kgmt_ctx* ctx, uint8_t* frontier, uint32_t* repeat, uint32_t cap, uint32_t* n)
+{
+    if(!ctx) return fail(KGMT_E_ARG, "null context");
+    return ctx->eng.get_frontier(0, frontier, repeat, cap, n);
 }
 
 int kgmt_set_profiling(kgmt_ctx* ctx, int32_t on)

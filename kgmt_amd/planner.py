@@ -134,22 +134,27 @@ class KGMT(Planner):
         _lib.check(_lib.lib().kgmt_plan(self._h, _ptr(start), _ptr(goal), p, n, on_dev, C.byref(res)))
         del keep
         self.result = res
+        self._publish()
+        return res
+
+    def _publish(self):
+        """Result members from self.result (Planner.cuh:31-34)."""
+        res = self.result
         self.h_treeSize_ = res.tree_size
         self.h_itr_ = res.iterations
         self.h_pathToGoal_ = res.goal_index if res.success else 0
         self.h_costToGoal_ = res.path_cost if res.success else 0.0
         self.h_pathRows_ = res.path_rows
         self.h_controlPathToGoal_ = self.path() if res.success else np.zeros((0, self.D), np.float32)
-        return res
 
-    # KGMT::planBench, src/planners/KGMT.cu:107-165: plan, then dump the tree
-    # as CSV under out_dir like writeDeviceVectorsToCSV (KGMT.cu:405-488)
+    # KGMT::planBench, src/planners/KGMT.cu:107-165: the plan's iterations one
+    # at a time, the state after each written as CSV under out_dir like
+    # writeDeviceVectorsToCSV (KGMT.cu:405-488)
     def planBench(self, h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount, benchItr: int,
-                  out_dir: str = "Data") -> KgmtResult:
-        res = self.plan(h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount)
-        from .export import write_plan_csv
-        write_plan_csv(self, out_dir, benchItr)
-        return res
+                  out_dir: str = "Data", verbose: bool = False) -> KgmtResult:
+        from .export import plan_bench_csv
+        return plan_bench_csv(self, h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount, out_dir, benchItr,
+                              verbose=verbose)
 
     def tree(self):
         """(samples (n, D), parents (n,), costs (n,)) of the last plan."""
@@ -162,6 +167,38 @@ class KGMT(Planner):
         c = np.zeros(k, np.float32)
         _lib.check(L.kgmt_get_tree(self._h, _ptr(s), _ptr(p), _ptr(c), k, C.byref(n)))
         return s, p, c
+
+    # step-by-step planning (the iterations of KGMT::planBench, KGMT.cu:107-165)
+    def plan_begin(self, h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount):
+        """Prepare a plan like plan() without running an iteration."""
+        L = _lib.lib()
+        S, Cd, D = dims(self.cfg.model)
+        x0 = _f32(h_initial, KGMT_MAX_SAMPLE)
+        xg = _f32(h_goal, KGMT_MAX_SAMPLE)
+        n = int(h_obstaclesCount)
+        optr, on_dev, keep = _obstacle_arg(d_obstacles_ptr, n, wdim(self.cfg.model))
+        self._keep = keep
+        _lib.check(L.kgmt_plan_begin(self._h, _ptr(x0), _ptr(xg), optr, n, on_dev))
+        self.h_treeSize_, self.h_itr_, self.h_pathToGoal_, self.h_pathRows_ = 1, 0, 0, 0
+
+    def plan_step(self) -> bool:
+        """Run one iteration; returns True once the plan has ended. The
+        result members and getters then describe the state after it."""
+        done = C.c_int32()
+        _lib.check(_lib.lib().kgmt_plan_step(self._h, C.byref(self.result), C.byref(done)))
+        self._publish()
+        return bool(done.value)
+
+    def frontier(self):
+        """Frontier flags and repeat counts of the tree nodes after the last
+        iteration (d_frontier_, d_activeFrontierRepeatCount_)."""
+        L = _lib.lib()
+        n = C.c_uint32()
+        _lib.check(L.kgmt_get_frontier(self._h, None, None, 0, C.byref(n)))
+        f = np.zeros(n.value, np.uint8)
+        r = np.zeros(n.value, np.uint32)
+        _lib.check(L.kgmt_get_frontier(self._h, _ptr(f), _ptr(r), n.value, C.byref(n)))
+        return f, r
 
     def tree_device(self):
         """The tree in the reference's device layout (d_treeSamples_ rows of D

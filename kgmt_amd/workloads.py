@@ -189,15 +189,29 @@ class RefineBatch:
     obstacles: np.ndarray      # (m, 6) boxes shared by all trajectories
 
 
-def refine_params(cfg: KgmtConfig, **kw):
-    """Cost weights of the refinement workload: effort measured from hover
-    thrust, softplus obstacle penalty with margin 0.02, squared goal distance;
-    Adam with lr 1e-2 for 50 steps."""
+# Cost weights of the refinement workload. "configured": the objective as
+# the workload states it, J = sum_t |u_t|^2 + softplus obstacle penalty
+# (margin 0.02) + |p_T - goal|^2. "hover": effort measured from hover thrust
+# and scaled down, goal weighted up, so that the effort term does not reduce
+# to pulling thrust towards zero (with |u|^2 about u = 0 the cost is almost
+# all hover thrust, m g per step, which Adam at lr 1e-2 lowers by at most
+# 0.5 in 50 steps).
+OBJECTIVES = {
+    "configured": dict(w_effort=1.0, w_obstacle=1.0, w_goal=1.0, hover_ref=False),
+    "hover": dict(w_effort=1e-3, w_obstacle=1.0, w_goal=100.0, hover_ref=True),
+}
+
+
+def refine_params(cfg: KgmtConfig, objective: str = "configured", **kw):
+    """Parameters of the refinement workload (softplus temperature 0.01,
+    margin 0.02; Adam with lr 1e-2 for 50 steps, controls clamped to their
+    bounds) for one of OBJECTIVES."""
     from .abi import KgmtAdjointParams
+    o = OBJECTIVES[objective]
     p = KgmtAdjointParams()
-    p.w_effort, p.w_obstacle, p.w_goal = 1e-3, 1.0, 100.0
+    p.w_effort, p.w_obstacle, p.w_goal = o["w_effort"], o["w_obstacle"], o["w_goal"]
     p.margin, p.tau = 0.02, 0.01
-    hover = -cfg.gravity * cfg.quad_mass
+    hover = -cfg.gravity * cfg.quad_mass if o["hover_ref"] else 0.0
     for i, v in enumerate([hover, 0.0, 0.0, 0.0]):
         p.u_ref[i] = v
     p.lr, p.beta1, p.beta2, p.adam_eps = 1e-2, 0.9, 0.999, 1e-8
@@ -207,10 +221,12 @@ def refine_params(cfg: KgmtConfig, **kw):
     return p
 
 
-def make_refine_batch(n: int = 4096, seed: int = 0, steps_lo: int = 20, steps_hi: int = 80) -> RefineBatch:
+def make_refine_batch(n: int = 4096, seed: int = 0, steps_lo: int = 20, steps_hi: int = 80,
+                      objective: str = "configured") -> RefineBatch:
     """Trajectory refinement workload: n quadcopter control sequences of
     length U{steps_lo..steps_hi} at dt 0.01 from rest in the unit cube,
-    controls hover thrust + noise, goals 0.05-0.15 away, 50 shared boxes."""
+    controls hover thrust + noise, goals 0.05-0.15 away, 50 shared boxes;
+    cost weights of `objective` (OBJECTIVES)."""
     cfg = config("quad12d")
     rng = _rng(seed, 5)
     steps = rng.integers(steps_lo, steps_hi + 1, n).astype(np.int32)
@@ -225,7 +241,7 @@ def make_refine_batch(n: int = 4096, seed: int = 0, steps_lo: int = 20, steps_hi
     u[:, 0] = np.clip(hover + rng.normal(0, 1.0, (T, n)), cfg.ctrl_min[0], cfg.ctrl_max[0])
     u[:, 1:] = np.clip(rng.normal(0, 0.005, (T, 3, n)), cfg.ctrl_min[1], cfg.ctrl_max[1])
     obs = random_boxes(rng, 50, 0.02, 0.08)
-    return RefineBatch(cfg, refine_params(cfg), steps, x0, goal, u, obs)
+    return RefineBatch(cfg, refine_params(cfg, objective), steps, x0, goal, u, obs)
 
 
 def pack_batch(queries):

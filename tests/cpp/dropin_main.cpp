@@ -7,7 +7,10 @@
 // d_treeSampleCosts_ptr_.
 //
 // usage: dropin_main <obstacles.csv> <capacity> <max_iter> <seed> <runs>
-// prints one JSON object per plan.
+//        prints one JSON object per plan;
+//        dropin_main <obstacles.csv> <capacity> <max_iter> <seed> bench <dir>
+//        runs planBench(..., 0) with <dir> as the working directory (its
+//        Data/ dump lands there).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -64,8 +67,15 @@ int main(int argc, char** argv)
             cfg.capacity    = (uint32_t)std::atoi(argv[2]);
             cfg.max_iter    = (uint32_t)std::atoi(argv[3]);
             cfg.seed        = (uint64_t)std::atoll(argv[4]);
-            const int runs  = std::atoi(argv[5]);
             KGMT kgmt(cfg);
+            if(std::string(argv[5]) == "bench" && argc > 6)
+                {
+                    std::filesystem::current_path(argv[6]);
+                    kgmt.planBench(h_initial, h_goal, d_obstacles, numObstacles, 0);
+                    (void)hipFree(d_obstacles);
+                    return 0;
+                }
+            const int runs = std::atoi(argv[5]);
             for(int i = 0; i < runs; ++i)
                 {
                     // --- Execute planner ---

@@ -172,7 +172,7 @@ def replay(orc, q, hook=None):
         score = orc.score(cfg, total, valid, active)
         inputs = dict(itr=itr, tree=tree, samples=samples, parents=parents, costs=costs,
                       frontier=frontier.astype(np.uint8), child=child, parent_u=parent, fnext=fnext.astype(np.uint8),
-                      valid=valid.copy(), score=score.copy())
+                      valid=valid.copy(), score=score.copy(), total=total.copy(), children=nC)
         if hook:
             hook(itr, inputs)
         f = frontier_update(orc, cfg, itr, tree, samples, parents, costs, frontier, child, parent, fnext, valid, score,
@@ -237,13 +237,24 @@ def update_variants(state):
     flagged = np.flatnonzero(state["fnext"])
     D = cfg.dims[2]
     out = {}
-    cap = tree + len(flagged) // 2
-    if len(flagged) >= 4 and cap < cfg.capacity:
+    # capacity cut to the tree plus half the accepted children, children
+    # truncated to fit it (the planner never has more children than capacity)
+    k = len(state["fnext"])
+    while True:
+        f = int(np.count_nonzero(state["fnext"][:k]))
+        cap = tree + max(1, f // 2)
+        if k <= cap:
+            break
+        k = cap
+    if f >= 4 and cap < cfg.capacity:
         c = dict(state, cfg=cfg.replace(capacity=cap))
         c["samples"] = state["samples"][:cap].copy()
         c["parents"] = state["parents"][:cap].copy()
         c["costs"] = state["costs"][:cap].copy()
         c["frontier"] = state["frontier"][:cap].copy()
+        c["child"] = state["child"][:, :k].copy()
+        c["parent_u"] = state["parent_u"][:k].copy()
+        c["fnext"] = state["fnext"][:k].copy()
         out["clamped"] = c
     if len(flagged) >= 4:
         g = dict(state)

@@ -126,7 +126,9 @@ def test_obstacle_csv_fixtures():
 
 
 def test_csv_export_roundtrip(tmp_path):
-    """export.write_plan_csv on a stand-in planner object (host arrays only)."""
+    """export.write_iteration on a stand-in planner object (host arrays only):
+    capacity rows of samples and parents, one row per iteration of counters,
+    flags and scores, the path when found."""
     from kgmt_amd import export
     from kgmt_amd.abi import KgmtResult
 
@@ -136,6 +138,9 @@ def test_csv_export_roundtrip(tmp_path):
         h_pathToGoal_ = 3
         result = KgmtResult()
 
+        class cfg:
+            capacity = 6
+
         def tree(self):
             s = np.arange(40, dtype=np.float32).reshape(4, 10) / 7
             return s, np.array([-1, 0, 0, 2], np.int32), np.zeros(4, np.float32)
@@ -143,12 +148,64 @@ def test_csv_export_roundtrip(tmp_path):
         def regions(self):
             return np.arange(8, dtype=np.int32), np.arange(8, dtype=np.int32) // 2, np.ones(8, np.float32)
 
+        def frontier(self):
+            return np.array([0, 1, 1, 0], np.uint8), np.array([0, 15, 1, 0], np.uint32)
+
     st = Stub()
     st.h_controlPathToGoal_ = st.tree()[0][[3, 2, 0]]
-    export.write_plan_csv(st, str(tmp_path), 0)
+    export.write_iteration(st, str(tmp_path), 0, frontier_size=2, expanded=64)
     s = export.read_rows(tmp_path / "Samples" / "Samples0" / "samples3.csv")
-    assert np.allclose(s, st.tree()[0], atol=5e-7)
-    p = export.read_rows(tmp_path / "Parents" / "Parents0" / "parents3.csv")
-    assert list(p[:, 0]) == [-1, 0, 0, 2]
+    assert s.shape == (6, 10) and np.allclose(s[:4], st.tree()[0], atol=5e-7) and np.all(s[4:] == 0)
+    p = (tmp_path / "Parents" / "Parents0" / "parents3.csv").read_text().split()
+    assert p == ["-1", "0", "0", "2", "-1", "-1"]
+    fr = (tmp_path / "Frontier" / "Frontier0" / "frontier.csv").read_text()
+    assert fr == "0,1,1,0,0,0\n"
+    rep = (tmp_path / "FrontierRepeatCount" / "FrontierRepeatCount0" / "frontierRepeatCount.csv").read_text()
+    assert rep == "0,15,1,0,0,0\n"
+    assert (tmp_path / "ExpandedNodes" / "ExpandedNodes0" / "expandedNodes.csv").read_text() == "64\n"
     path = export.read_rows(tmp_path / "ControlPathToGoal" / "ControlPathToGoal0" / "controlPathToGoal.csv")
     assert path.shape == (3, 10)
+
+
+WRITER_SRC = r"""
+#include "kgmt_planner.hpp"
+int main(int argc, char** argv)
+{
+    std::vector<float> f = {0.0f, -0.0f, 1.0f / 3.0f, -2.5e-7f, 3.0e-7f, 123456.789f, -1.0e-3f, 0.5e-6f, 1.5e-6f,
+                            2.5e-6f, 1e30f, -7.0000005f};
+    std::vector<int> i = {-1, 0, 7, 2147483647};
+    std::vector<unsigned int> u = {0u, 15u, 1u, 4000000000u};
+    kgmt::writeVectorToCSV(f, std::string(argv[1]) + "/f.csv", 3, 4, false);
+    kgmt::writeVectorToCSV(i, std::string(argv[1]) + "/i.csv", 4, 1, false);
+    kgmt::writeVectorToCSV(u, std::string(argv[1]) + "/u.csv", 1, 4, true);
+    kgmt::writeValueToCSV(12345u, std::string(argv[1]) + "/v.csv");
+    kgmt::writeValueToCSV(0.25, std::string(argv[1]) + "/v.csv");
+    return 0;
+}
+"""
+
+
+def test_csv_writers_match_the_iostream_format(tmp_path):
+    """The Python writers of kgmt_amd.export produce the bytes of the
+    reference's iostream writers (helper.cuh:56-109: std::fixed with
+    setprecision 6 for arrays, 10 for single values; integers plain), here
+    the same writers in include/kgmt_planner.hpp compiled with g++."""
+    import subprocess
+    from kgmt_amd import export
+    src = tmp_path / "w.cpp"
+    src.write_text(WRITER_SRC)
+    exe = tmp_path / "w"
+    subprocess.run(["g++", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", str(src), "-o", str(exe)], check=True)
+    cdir = tmp_path / "c"
+    cdir.mkdir()
+    subprocess.run([str(exe), str(cdir)], check=True)
+    f = np.array([0.0, -0.0, 1.0 / 3.0, -2.5e-7, 3.0e-7, 123456.789, -1.0e-3, 0.5e-6, 1.5e-6, 2.5e-6, 1e30,
+                  -7.0000005], np.float32)
+    assert (cdir / "f.csv").read_text() == export._rows_text(f.reshape(3, 4), "%.6f")
+    assert (cdir / "i.csv").read_text() == export._rows_text(np.array([-1, 0, 7, 2147483647])[:, None], "%d")
+    assert (cdir / "u.csv").read_text() == export._rows_text(np.array([[0, 15, 1, 4000000000]], np.uint32), "%d")
+    pdir = tmp_path / "p"
+    pdir.mkdir()
+    export._value(str(pdir / "v.csv"), np.uint32(12345))
+    export._value(str(pdir / "v.csv"), 0.25)
+    assert (cdir / "v.csv").read_text() == (pdir / "v.csv").read_text()

@@ -110,23 +110,28 @@ def test_planner_check_on_dubins5d_workload(seed, orc):
 
 @pytest.mark.parametrize("case", ["di_ref", "dubins5d", "dubins6d", "unicycle"])
 def test_planner_check_1500_boxes(case, orc):
-    """The largest scene a query may carry: candidate lists overflow (128
-    boxes per wave) and fall back to the whole scene in dense waves."""
+    """The largest scene a query may carry, small boxes and large ones,
+    children clustered by parent and spread: the wave candidate lists stay
+    within 128 boxes in some waves and overflow in others (then the wave
+    falls back to the whole scene); both checks must run, bit-exact."""
     cfg = CASES[case]()
     rng = np.random.default_rng(31)
     span = cfg.ws_max - cfg.ws_min
-    obs = W.random_boxes(rng, 1500, 0.005 * span, 0.03 * span, ws=(cfg.ws_min, cfg.ws_max), dim=cfg.wdim)
     n, npar = 30000, 900
     X = parent_states(cfg, npar, seed=4)
     ctrl, steps, _ = orc.draw_children(cfg, 0, 3, 0, n)
-    pidx = rng.integers(0, npar, n).astype(np.int32)
-    want, chk = assert_bit_exact(cfg, pidx, X, ctrl, steps, obs, orc)
-    paths = np.bincount(chk, minlength=4)
-    print(f"{case} 1500 boxes: checks {paths.tolist()}", flush=True)
-    assert paths[KGMT_CHECK_SCENE] > 0  # the fallback ran
-    pidx = clustered_parents(rng, npar, n)
-    want, chk = assert_bit_exact(cfg, pidx, X, ctrl, steps, obs, orc)
-    assert np.bincount(chk, minlength=4)[KGMT_CHECK_CANDIDATES] > 0
+    seen = np.zeros(4, np.int64)
+    for sides in ((0.005, 0.03), (0.05, 0.15)):
+        obs = W.random_boxes(rng, 1500, sides[0] * span, sides[1] * span, ws=(cfg.ws_min, cfg.ws_max),
+                             dim=cfg.wdim)
+        for layout in ("spread", "clustered"):
+            pidx = rng.integers(0, npar, n).astype(np.int32) if layout == "spread" else clustered_parents(rng, npar, n)
+            want, chk = assert_bit_exact(cfg, pidx, X, ctrl, steps, obs, orc)
+            paths = np.bincount(chk, minlength=4)
+            seen += paths
+            print(f"{case} 1500 boxes {sides} {layout}: checks {paths.tolist()}, valid {want[1].mean():.3f}",
+                  flush=True)
+    assert seen[KGMT_CHECK_SCENE] > 0 and seen[KGMT_CHECK_CANDIDATES] > 0, seen
 
 
 @pytest.mark.parametrize("case", ["di_ref", "dubins5d", "dubins6d", "unicycle", "quad12d"])
@@ -198,8 +203,8 @@ def _dubins_stress(model):
                 W.config("dubins5d", dt=0.1, steps_min=10, steps_max=50, ctrl_min=[-3.0, -1.5, 0.01, 0.0],
                          ctrl_max=[3.0, 1.5, 2.0, 0.0])]
     return [dubins6d_config(),
-            dubins6d_config(dt=0.1, steps_min=10, steps_max=50, ctrl_min=[-3.0, -1.5, -1.0, 0.0],
-                            ctrl_max=[3.0, 1.5, 1.0, 0.0], v_max=3.0, pitch_min=-pi, pitch_max=pi)]
+            dubins6d_config().replace(steps_min=10, steps_max=50, ctrl_min=[-3.0, -1.5, -1.0, 0.0],
+                                      ctrl_max=[3.0, 1.5, 1.0, 0.0], v_max=3.0, pitch_min=-pi, pitch_max=pi)]
 
 
 @pytest.mark.parametrize("model", [KGMT_MODEL_DUBINS_5D, KGMT_MODEL_DUBINS_AIRPLANE])
