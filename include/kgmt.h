@@ -331,6 +331,19 @@ int kgmt_propagate_collide(const kgmt_config* cfg, const int32_t* d_parent_idx,
                            const float* d_controls, const int32_t* d_steps, uint32_t n_children,
                            const float* d_obstacles, uint32_t n_obs, float* d_child_samples,
                            uint8_t* d_valid, int32_t* d_r1, int32_t* d_r2, void* stream);
+/* kgmt_propagate_collide without any device allocation: no counterpart in
+ * the reference; for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * outputs and bits as kgmt_propagate_collide; the obstacle scene goes into a
+ * library-held device array of one scene of at most 1500 boxes per GPU
+ * (54 KB, kept until the process ends), and n_obs > 1500 is rejected with
+ * KGMT_E_CAPACITY. Calls on one GPU share that array: calls that overlap on
+ * one GPU race. Asynchronous on `stream`. */
+int kgmt_propagate_collide_noalloc(const kgmt_config* cfg, const int32_t* d_parent_idx,
+                                   const float* d_parent_states, uint32_t n_parents,
+                                   const float* d_controls, const int32_t* d_steps, uint32_t n_children,
+                                   const float* d_obstacles, uint32_t n_obs, float* d_child_samples,
+                                   uint8_t* d_valid, int32_t* d_r1, int32_t* d_r2, void* stream);
 /* Checks the planner's propagate path can decide a child's verdict with
  * (kgmt_propagate_collide_planner reports which one ran). */
 enum {

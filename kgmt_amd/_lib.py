@@ -64,6 +64,7 @@ SIGNATURES = {
     "kgmt_batch_set_launch": (C.c_int, [_P, _u32, _u32, _u32, _u32]),
     "kgmt_draw_controls": (C.c_int, [_cfgp, _u32, _u32, _u32, _u32, _P, _P, _P, _P]),
     "kgmt_propagate_collide": (C.c_int, [_cfgp, _P, _P, _u32, _P, _P, _u32, _P, _u32, _P, _P, _P, _P, _P]),
+    "kgmt_propagate_collide_noalloc": (C.c_int, [_cfgp, _P, _P, _u32, _P, _P, _u32, _P, _u32, _P, _P, _P, _P, _P]),
     "kgmt_propagate_collide_planner": (C.c_int, [_cfgp, _P, _P, _u32, _P, _P, _u32, _P, _u32, _P, _P, _P, _P, _P,
                                                  _P, _P]),
     "kgmt_frontier_update": (C.c_int, [_cfgp, _u32, _u32, _P, _P, _P, _P, _P, _P, _P, _u32, _P, _P, _P, _P, _P,

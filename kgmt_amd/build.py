@@ -24,7 +24,7 @@ LIB = os.path.join(HERE, "libkgmt.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ARCH = os.environ.get("KGMT_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["kgmt_kernels.hip", "kgmt_adjoint.hip", "kgmt_engine.cpp"]
+HIP_SOURCES = ["kgmt_kernels.hip", "kgmt_adjoint.hip", "kgmt_noalloc.hip", "kgmt_engine.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",

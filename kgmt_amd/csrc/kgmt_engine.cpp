@@ -1323,6 +1323,37 @@ int kgmt_propagate_collide(const kgmt_config* cfg, const int32_t* d_parent_idx, 
     return KGMT_OK;
 }
 
+int kgmt_propagate_collide_noalloc(const kgmt_config* cfg, const int32_t* d_parent_idx, const float* d_parent_states,
+                                   uint32_t n_parents, const float* d_controls, const int32_t* d_steps,
+                                   uint32_t n_children, const float* d_obstacles, uint32_t n_obs,
+                                   float* d_child_samples, uint8_t* d_valid, int32_t* d_r1, int32_t* d_r2,
+                                   void* stream)
+{
+    if(!cfg) return fail(KGMT_E_ARG, "null configuration");
+    if(n_children && (!d_parent_idx || !d_parent_states || !d_controls || !d_steps || !d_child_samples || !d_valid ||
+                      !d_r1 || !d_r2))
+        return fail(KGMT_E_ARG, "null buffer");
+    if(n_obs > kMaxObs || scene_pairs(n_obs) > scratch_pairs())
+        return fail(KGMT_E_CAPACITY, "more than 1500 obstacles: the scratch scene holds at most 1500 boxes");
+    if(n_obs && !d_obstacles) return fail(KGMT_E_ARG, "obstacles pointer is null");
+    int rc = check_config(cfg);
+    if(rc) return rc;
+    if(n_obs && (rc = check_device_rows(d_obstacles))) return rc;
+    float4* scene = nullptr;
+    HIP_TRY(scratch_address(&scene));
+    QueryDesc h;
+    fill_desc(h, *cfg);
+    const hipStream_t s = (hipStream_t)stream;
+    const uint32_t nGrp = scene_groups(n_obs);
+    hipError_t e        = launch_build_scene(d_obstacles, n_obs, nGrp, scene, model_wdim(cfg->model), s);
+    if(e == hipSuccess)
+        e = launch_stage_propagate_by_value(cfg->model, h, d_parent_idx, d_parent_states, n_parents, d_controls,
+                                            d_steps, n_children, scene, nGrp, d_child_samples, d_valid, d_r1, d_r2,
+                                            s);
+    if(e != hipSuccess) return fail(KGMT_E_HIP, std::string("propagate: ") + hipGetErrorString(e));
+    return KGMT_OK;
+}
+
 int kgmt_propagate_collide_planner(const kgmt_config* cfg, const int32_t* d_parent_idx,
                                    const float* d_parent_states, uint32_t n_parents, const float* d_controls,
                                    const int32_t* d_steps, uint32_t n_children, const float* d_obstacles,

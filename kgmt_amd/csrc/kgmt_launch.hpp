@@ -71,6 +71,15 @@ hipError_t launch_stage_propagate_planner(int model, const QueryDesc* qd, const 
 hipError_t launch_stage_frontier_update(const QueryDesc* qd, QueryState* st, const Pools& P, const WorkLists& W,
                                         uint32_t itr, uint32_t tree, const float* child, const int32_t* childParent,
                                         const uint8_t* fnext, uint32_t nC, int32_t* info, hipStream_t s);
+// allocation-free propagate stage (kgmt_noalloc.hip): the scene in a
+// file-scope device array of scratch_pairs() (lo, hi) pairs, whose address on
+// the current device scratch_address gives; the descriptor passed by value
+uint32_t scratch_pairs();
+hipError_t scratch_address(float4** out);
+hipError_t launch_stage_propagate_by_value(int model, const QueryDesc& qd, const int32_t* parentIdx,
+                                           const float* parentStates, uint32_t nParents, const float* controls,
+                                           const int32_t* steps, uint32_t n, const float4* obs, uint32_t nGrp,
+                                           float* child, uint8_t* valid, int32_t* r1, int32_t* r2, hipStream_t s);
 hipError_t launch_stage_region_update(const QueryDesc* qd, const int32_t* r1, const int32_t* r2, const uint8_t* valid,
                                       const float* ufront, uint32_t n, const float* score, const uint32_t* actPrev,
                                       uint32_t* actNext, int32_t* total, int32_t* validc, uint8_t* fnext,
