@@ -50,7 +50,7 @@
 extern "C" {
 #endif
 
-#define KGMT_ABI_VERSION 4
+#define KGMT_ABI_VERSION 5
 
 /* ---- status codes -------------------------------------------------------- */
 enum {
@@ -150,9 +150,11 @@ typedef struct kgmt_result {
 
 /* Per-kernel device time accumulated while profiling is on (HIP events
  * around every launch of the plain, non-graph iteration loop). Kernel order:
- * 0 propagate, 1 score, 2 append, 3 expand, 4 finish (closes the iteration of
- * every query and builds the next iteration's work lists; a single-query
- * context has no finish launch: its first expand job does that). */
+ * 0 propagate, 1 update (region scores, tree append, re-activation, goal
+ * test and next expansion list in one launch), 2 finish (closes the
+ * iteration of every query and builds the next iteration's work lists; a
+ * single-query context has no finish launch: its last update job does
+ * that); slots 3 and 4 are unused. */
 #define KGMT_PROFILE_KERNELS 5
 typedef struct kgmt_profile {
     uint64_t launches[KGMT_PROFILE_KERNELS];
@@ -203,7 +205,7 @@ int kgmt_set_graph(kgmt_ctx* ctx, int32_t on);
 /* Launch shape of the planner's persistent kernels: threads per propagate
  * workgroup (128, 256 or 512), lanes per child (1; or 2 or 4 with 256
  * threads: the lanes split the child's box tests), and the workgroup counts
- * of the propagate and append/expand kernels (0 keeps the current value).
+ * of the propagate and update kernels (0 keeps the current value).
  * Results do not depend on it. No counterpart in the reference (fixed launch
  * shapes at KGMT.cu:183-211). */
 int kgmt_set_launch(kgmt_ctx* ctx, uint32_t propagate_block, uint32_t lanes_per_child, uint32_t propagate_grid,
@@ -374,7 +376,7 @@ int kgmt_propagate_collide_planner(const kgmt_config* cfg, const int32_t* d_pare
                                    uint8_t* d_valid, int32_t* d_r1, int32_t* d_r2, uint8_t* d_check,
                                    float* d_edge_box, void* stream);
 /* Frontier update of one iteration (updateFrontier + updateFrontier_kernel,
- * KGMT.cu:305-403) through the planner's own append and expand kernels, on
+ * KGMT.cu:305-403) through the planner's own update kernel, on
  * caller buffers (device memory, capacity = cfg->capacity):
  *   tree: samples [D][capacity] (SoA), parents, costs [capacity], frontier
  *   flags [capacity] (u8); the first tree_size entries are the tree;

@@ -6,7 +6,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-KGMT_ABI_VERSION = 4
+KGMT_ABI_VERSION = 5
 
 KGMT_OK = 0
 KGMT_E_ARG = -1
@@ -184,7 +184,7 @@ class KgmtDeviceTree(C.Structure):
 
 
 KGMT_PROFILE_KERNELS = 5
-PROFILE_KERNEL_NAMES = ("propagate", "score", "append", "expand", "finish")
+PROFILE_KERNEL_NAMES = ("propagate", "update", "finish")  # slots 3 and 4 are unused
 
 
 class KgmtProfile(C.Structure):

@@ -3,15 +3,15 @@
 The tunables are the propagate workgroup shape (threads per workgroup 128,
 256 or 512 with one lane per child, or 256 threads with 2 or 4 lanes per
 child splitting each child's box tests: tiles of 64 to 512 children), the
-number of persistent propagate workgroups and the number of append/expand
+number of persistent propagate workgroups and the number of update
 workgroups (multiples of the device's compute units). None of them changes a
 result: every candidate's outcomes are checked against the first candidate's,
 query by query, and a mismatch is an error.
 
 A search times candidates on a calibration workload (plans run as the caller
 will run them: batched with or without a hipGraph, or plain) in two passes:
-every propagate shape x propagate grid at the default append/expand grid,
-then the best of those x every append/expand grid. It keeps the fastest by
+every propagate shape x propagate grid at the default update grid,
+then the best of those x every update grid. It keeps the fastest by
 the median of `reps` timed runs and records it in a JSON cache keyed by
 device and workload signature, so later processes reuse it.
 
@@ -37,7 +37,7 @@ DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "kgmt_amd", "lau
 def candidates(cus: int, shapes: Iterable = SHAPES, prop_per_cu: Iterable[int] = PROP_PER_CU,
                chunk_grid: int | None = None) -> list[dict]:
     """First-pass launch shapes: every propagate shape x propagate grid, with
-    the append/expand grid fixed (default 4 workgroups per CU); the engine's
+    the update grid fixed (default 4 workgroups per CU); the engine's
     default (256 threads, 1 lane, 4 per CU) first."""
     cg = int(chunk_grid if chunk_grid is not None else 4 * cus)
     out = []
@@ -50,7 +50,7 @@ def candidates(cus: int, shapes: Iterable = SHAPES, prop_per_cu: Iterable[int] =
 
 
 def chunk_candidates(best: dict, cus: int, chunk_per_cu: Iterable[int] = CHUNK_PER_CU) -> list[dict]:
-    """Second pass: the first pass's best with every append/expand grid (the
+    """Second pass: the first pass's best with every update grid (the
     best itself first, so the outcome reference stays the same)."""
     out = [dict(best)]
     for c in chunk_per_cu:

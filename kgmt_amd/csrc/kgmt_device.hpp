@@ -84,9 +84,9 @@ struct QueryState
     uint32_t finishItr;
     uint32_t pad[3];
     // The next iteration's values, written when an iteration closes (by the
-    // first expand job of a single-query plan, by k_finish for a batch); propagate
-    // reads them, the score kernel copies them to the fields above (k_paths
-    // after the last iteration), and append/expand read those.
+    // last update job of a single-query plan, by k_finish for a batch);
+    // propagate reads them and its first workgroup copies them to the fields
+    // above (k_paths after the last iteration), which the update kernel reads.
     uint32_t itrN, treeSizeN, nChildrenN, perN, doneN, successN, finishItrN, padN;
     uint64_t expandedN, flaggedN;
 };
@@ -119,11 +119,9 @@ struct Pools
     int32_t* r1child;    // region of each child
     uint8_t* fnext;      // child flagged for the next frontier
     uint8_t* react;      // old node re-activated this iteration
-    uint32_t* repU;      // repeat count of each appended child, 0 otherwise
     uint32_t* repIdx;    // expansion list: node id of multiset entry r
     uint32_t* partA;     // per child chunk: flagged children, then exclusive offsets
-    uint32_t* partB;     // per old chunk: re-activated nodes, then exclusive offsets
-    uint32_t* partC;     // per child chunk: sum of repU, then exclusive offsets
+    uint64_t* look;      // look-back word per update job: 4 x (nChunks + 1) per query, two sets by parity
     int32_t* total;      // per region: children counted
     int32_t* validc;     // per region: valid children
     float* score;        // per region score
@@ -151,10 +149,12 @@ struct HostResult
 struct WorkLists
 {
     uint32_t* tileStart;   // 2 x (nq + 1): propagate tiles
-    uint32_t* jobStart;    // 2 x (nq + 1): append / expand chunk jobs
+    uint32_t* jobStart;    // 2 x (nq + 1): update jobs
     uint32_t* active;      // 2: queries not done in each list
     uint32_t* counter;     // arrivals of k_init's workgroups (the last builds the first work lists)
+    uint32_t* claim;       // 2 (by parity): update jobs taken in a launch (the launch before zeroes it)
     uint32_t* allDone;     // host-mapped word: 1 once every query is done
+    uint32_t* error;       // 1 once an update job's look-back wait ran out (host-mapped for plans)
     HostResult* result;    // host-mapped result of a single-query plan (null: batch, or no early result)
     float* hostPath;       // host-mapped goal -> root rows of that plan ((max_iter + 2) x D floats)
     uint32_t nq;

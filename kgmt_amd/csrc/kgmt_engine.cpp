@@ -214,7 +214,7 @@ struct Engine
     uint32_t maxIter = 0;
     hipStream_t stream = nullptr;
     bool ownStream     = false;
-    uint32_t* pinned   = nullptr;  // host-mapped completion word (written by the kernels)
+    uint32_t* pinned   = nullptr;  // host-mapped words written by the kernels: completion, look-back error
     hipEvent_t ev[kRing] = {};
     Launch L;
     hipGraph_t graph         = nullptr;
@@ -297,23 +297,23 @@ struct Engine
            (rc = dev_alloc(P.frontier, nodes, owned)) || (rc = dev_alloc(P.r1node, nodes, owned)) ||
            (rc = dev_alloc(P.r1child, nodes, owned)) ||
            (rc = dev_alloc(P.fnext, nodes, owned)) || (rc = dev_alloc(P.react, nodes, owned)) ||
-           (rc = dev_alloc(P.repU, nodes, owned)) || (rc = dev_alloc(P.repIdx, nodes, owned)) ||
-           (rc = dev_alloc(P.partA, chunks, owned)) || (rc = dev_alloc(P.partB, chunks, owned)) ||
-           (rc = dev_alloc(P.partC, chunks, owned)) || (rc = dev_alloc(P.total, regs, owned)) ||
+           (rc = dev_alloc(P.repIdx, nodes, owned)) || (rc = dev_alloc(P.partA, chunks, owned)) ||
+           (rc = dev_alloc(P.look, 4 * (size_t)chunks, owned)) || (rc = dev_alloc(P.total, regs, owned)) ||
            (rc = dev_alloc(P.validc, regs, owned)) || (rc = dev_alloc(P.score, regs, owned)) ||
            (rc = dev_alloc(P.actPrev, subs, owned)) || (rc = dev_alloc(P.actNext, subs, owned)) ||
            (rc = dev_alloc(P.path, paths, owned)) || (rc = dev_alloc(W.tileStart, 2 * (size_t)(n + 1), owned)) ||
            (rc = dev_alloc(W.jobStart, 2 * (size_t)(n + 1), owned)) || (rc = dev_alloc(W.active, 2, owned)) ||
-           (rc = dev_alloc(W.counter, 1, owned)))
+           (rc = dev_alloc(W.counter, 1, owned)) || (rc = dev_alloc(W.claim, 2, owned)))
             return rc;
         W.nq   = n;
         W.tile = L.propBlock / L.lanes;
         HIP_TRY(hipMemset(W.counter, 0, sizeof(uint32_t)));
         HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         ownStream = true;
-        HIP_TRY(hipHostMalloc((void**)&pinned, sizeof(uint32_t), hipHostMallocMapped));
-        *pinned = 0;
+        HIP_TRY(hipHostMalloc((void**)&pinned, 2 * sizeof(uint32_t), hipHostMallocMapped));
+        pinned[0] = pinned[1] = 0;
         HIP_TRY(hipHostGetDevicePointer((void**)&W.allDone, pinned, 0));
+        W.error = W.allDone + 1;
         W.result   = nullptr;
         W.hostPath = nullptr;
         const size_t pathFloats = (size_t)(cfgs[0].max_iter + 2) * (size_t)hq[0].D;
@@ -400,7 +400,7 @@ struct Engine
         return KGMT_OK;
     }
 
-    // One iteration: 4 launches for one query, 5 for a batch (k_finish).
+    // One iteration: 2 launches for one query, 3 for a batch (k_finish).
     // marks: timing events around them (profiling).
     int enqueue_iteration(uint32_t parity, hipEvent_t* marks = nullptr)
     {
@@ -408,17 +408,13 @@ struct Engine
         if(marks) HIP_TRY(hipEventRecord(marks[0], stream));
         HIP_TRY(launch_propagate(model, dq, ds, P, W, parity, maxPairs, L, stream));
         if(marks) HIP_TRY(hipEventRecord(marks[1], stream));
-        HIP_TRY(launch_score(dq, ds, P, nq, stream));
+        HIP_TRY(launch_update(dq, ds, P, W, parity, L, true, stream));
         if(all) HIP_TRY(hipEventRecord(marks[2], stream));
-        HIP_TRY(launch_append(dq, ds, P, W, parity, L, stream));
-        if(all) HIP_TRY(hipEventRecord(marks[3], stream));
-        HIP_TRY(launch_expand(dq, ds, P, W, parity, L, stream));
-        if(all) HIP_TRY(hipEventRecord(marks[4], stream));
-        // a single query's first expand job closes the iteration itself
+        // a single query's last update job closes the iteration itself
         if(nq > 1)
             {
                 HIP_TRY(launch_finish(dq, ds, W, parity, stream));
-                if(all) HIP_TRY(hipEventRecord(marks[5], stream));
+                if(all) HIP_TRY(hipEventRecord(marks[3], stream));
             }
         return KGMT_OK;
     }
@@ -443,8 +439,9 @@ struct Engine
         for(uint32_t it = 0; it < iters; ++it)
             {
                 hipEvent_t* m = &profEv[(size_t)it * (KGMT_PROFILE_KERNELS + 1)];
-                // no finish launch for a single query
-                const int nk = profiling > 1 ? (nq > 1 ? KGMT_PROFILE_KERNELS : KGMT_PROFILE_KERNELS - 1) : 1;
+                // propagate, update and (batches) finish; the last two
+                // profile slots stay unused
+                const int nk = profiling > 1 ? (nq > 1 ? 3 : 2) : 1;
                 for(int k = 0; k < nk; ++k)
                     {
                         float ms = 0.0f;
@@ -490,6 +487,7 @@ struct Engine
         if(counterDirty) HIP_TRY(hipMemsetAsync(W.counter, 0, sizeof(uint32_t), stream));
         counterDirty = true;
         __atomic_store_n(pinned, 0u, __ATOMIC_RELEASE);
+        __atomic_store_n(pinned + 1, 0u, __ATOMIC_RELEASE);
         HIP_TRY(launch_init(dq, ds, P, W, sceneBuild ? sceneRows : nullptr, stream));
         if(useGraph && (!graphExec || graphObs != maxPairs || graphLaunch.propGrid != L.propGrid ||
                         graphLaunch.chunkGrid != L.chunkGrid || graphLaunch.propBlock != L.propBlock ||
@@ -564,14 +562,25 @@ struct Engine
                 s.expanded = h->expanded, s.flagged = h->flagged;
                 counterDirty = false;
                 itersRun     = std::min(it, maxIter);
-                return KGMT_OK;
+                return device_error();
             }
         HIP_TRY(launch_paths(dq, ds, P, nq, stream));
         HIP_TRY(hipMemcpyAsync(hs.data(), ds, sizeof(QueryState) * nq, hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         counterDirty = false;
         itersRun     = std::min(it, maxIter);
+        if(int rc = device_error()) return rc;
         if(prof) return collect_profile(it);
+        return KGMT_OK;
+    }
+
+    // An update job whose look-back wait ran out of its bound. Every wait
+    // ends (jobs are taken in order by running workgroups); the bound only
+    // keeps a kernel from spinning forever if that ever failed.
+    int device_error() const
+    {
+        if(__atomic_load_n(pinned + 1, __ATOMIC_ACQUIRE))
+            return fail(KGMT_E_HIP, "frontier update: a look-back wait ran out of its bound");
         return KGMT_OK;
     }
 
@@ -587,6 +596,7 @@ struct Engine
         if(counterDirty) HIP_TRY(hipMemsetAsync(W.counter, 0, sizeof(uint32_t), stream));
         counterDirty = true;
         __atomic_store_n(pinned, 0u, __ATOMIC_RELEASE);
+        __atomic_store_n(pinned + 1, 0u, __ATOMIC_RELEASE);
         if(hostRes) hostRes->iterations = 0;
         HIP_TRY(launch_init(dq, ds, P, W, sceneBuild ? sceneRows : nullptr, stream));
         int rc = pull_state();
@@ -625,7 +635,7 @@ struct Engine
                 if(rc) return rc;
                 ++stepItr;
                 itersRun = stepItr;
-                if((rc = pull_state())) return rc;
+                if((rc = pull_state()) || (rc = device_error())) return rc;
                 all = true;
                 for(uint32_t q = 0; q < nq; ++q) all = all && hs[q].done;
             }
@@ -1415,12 +1425,11 @@ int kgmt_frontier_update(const kgmt_config* cfg, uint32_t itr, uint32_t tree_siz
        (rc = a.get(P.treeS, (size_t)h.D * cap)) || (rc = a.get(P.childR, (size_t)row_words(h.D) * cap)) ||
        (rc = a.get(P.parent, nodes)) || (rc = a.get(P.grand, nodes)) || (rc = a.get(P.cost, nodes)) ||
        (rc = a.get(P.frontier, nodes)) || (rc = a.get(P.r1node, nodes)) || (rc = a.get(P.r1child, nodes)) ||
-       (rc = a.get(P.fnext, nodes)) || (rc = a.get(P.react, nodes)) || (rc = a.get(P.repU, nodes)) ||
-       (rc = a.get(P.repIdx, nodes)) || (rc = a.get(P.partA, h.nChunks + 1)) ||
-       (rc = a.get(P.partB, h.nChunks + 1)) || (rc = a.get(P.partC, h.nChunks + 1)) ||
+       (rc = a.get(P.fnext, nodes)) || (rc = a.get(P.react, nodes)) || (rc = a.get(P.repIdx, nodes)) ||
+       (rc = a.get(P.partA, h.nChunks + 1)) || (rc = a.get(P.look, 4 * ((size_t)h.nChunks + 1))) ||
        (rc = a.get(P.validc, h.R1)) || (rc = a.get(P.score, h.R1)) || (rc = a.get(W.tileStart, 4)) ||
        (rc = a.get(W.jobStart, 4)) || (rc = a.get(W.active, 2)) || (rc = a.get(W.counter, 1)) ||
-       (rc = a.get(W.allDone, 1)))
+       (rc = a.get(W.claim, 2)) || (rc = a.get(W.allDone, 1)) || (rc = a.get(W.error, 1)))
         return rc;
     W.nq   = 1;
     W.tile = kTile;
@@ -1440,7 +1449,10 @@ int kgmt_frontier_update(const kgmt_config* cfg, uint32_t itr, uint32_t tree_siz
     HIP_TRY(hipMemcpyAsync(d_frontier, P.frontier, cap, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_rep_idx, P.repIdx, sizeof(uint32_t) * cap, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_info, info, sizeof(int32_t) * 4, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));  // h is released on return
+    uint32_t err = 0;
+    HIP_TRY(hipMemcpyAsync(&err, W.error, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));  // h and err are released on return
+    if(err) return fail(KGMT_E_HIP, "frontier update: a look-back wait ran out of its bound");
     return KGMT_OK;
 }
 

@@ -1,7 +1,7 @@
 // kgmt_kernels.hip — HIP kernels of the KGMT per-iteration expansion path (gfx950).
 //
-// One iteration of the planner is five launches over a table of queries, four
-// for a single query (its first k_expand job does k_finish's work):
+// One iteration of the planner is three launches over a table of queries, two
+// for a single query (its last k_update job does k_finish's work):
 //   k_propagate  persistent over tiles of children: Philox draws, per-wave
 //                sort by step count, RK4 rollout, AABB test against the
 //                obstacles staged in LDS (candidate boxes per wave), region
@@ -10,15 +10,13 @@
 //                and new subregion bits summed per tile in LDS, per-chunk
 //                flag counts.
 //                (KGMT.cu:218-299, statePropagator.cu, collisionCheck.cu)
-//   k_score      one workgroup per query: region scores with a fixed pairwise
-//                summation order, bitset snapshot, scan of the flag counts.
-//                (Graph.cu:207-358, KGMT.cu:383-388)
-//   k_append     persistent over 1024-element chunk jobs: compaction of the
-//                flagged children and tree append, goal test; re-activation
-//                of old nodes. (KGMT.cu:305-378)
-//   k_expand     same jobs: writes the next expansion list (segmented repeat);
-//                each job sums the chunk counts before its own.
-//                (KGMT.cu:170-181, Planner.cu:52-80)
+//   k_update     one workgroup per 1024-element job, old-node chunks then
+//                child chunks: region scores with a fixed pairwise summation
+//                order (each re-activation job for itself), re-activation of
+//                old nodes, compaction of the flagged children, tree append
+//                and goal test, and the next expansion list (segmented
+//                repeat) at offsets found by a decoupled look-back.
+//                (Graph.cu:207-358, KGMT.cu:170-181, 305-403, Planner.cu:52-80)
 //   k_finish     (batches) one workgroup: closes the iteration of every query (tree
 //                size, stop rules, next launch regime; KGMT.cu:84-95, 183-201,
 //                399-402) and builds the next iteration's work lists.
@@ -120,6 +118,33 @@ __device__ __forceinline__ float lane_down(float v)
         return __shfl_down(v, C, 64);
 }
 
+// Active subregions of region r (bits r P .. r P + P - 1 of the bitset): one
+// word when P divides 32 (the bits never straddle a word), else count_bits.
+__device__ __forceinline__ uint32_t region_active(const uint32_t* __restrict__ act, uint32_t r, uint32_t P)
+{
+    if(P <= 32u && 32u % P == 0u)
+        {
+            const uint32_t b = r * P;
+            const uint32_t m = P == 32u ? 0xffffffffu : ((1u << P) - 1u) << (b & 31u);
+            return (uint32_t)__popc(act[b >> 5] & m);
+        }
+    return count_bits(act, r * P, P);
+}
+
+// Raw score of a region with vc > 0 valid children of tc, cnt of its P
+// subregions active (Graph.cu:239-257, Q6: fv^4 = (fv*fv)*(fv*fv))
+__device__ __forceinline__ float region_raw_score(const QueryDesc& qd, int32_t vc, int32_t tc, uint32_t cnt)
+{
+    const float eps = qd.cfg.epsilon;
+    const float cov = (float)cnt / (float)qd.P;
+    const float vf  = (float)vc;
+    const float fv  = (eps + vf) / ((eps + vf) + (float)(tc - vc)) * qd.vol;
+    const float f2  = fv * fv;
+    const float f4  = f2 * f2;
+    const float tt  = (float)tc;
+    return f4 / ((1.0f + cov) * (1.0f + tt * tt));
+}
+
 // Region scores of one query (block of NT threads, R1 <= kMaxRegions). Q6
 // semantics: closed formula, then the pairwise sum a[i] += a[i + st] for
 // st = P2/2, ..., 1 over the scores padded with zeros to P2 = pow2(R1). Thread
@@ -154,7 +179,7 @@ __device__ __forceinline__ void score_load(const QueryDesc& qd, const int32_t* _
             const int r = t + k * NT;
             in.vc[k]    = r < R1 ? validc[r] : 0;
             in.tc[k]    = r < R1 ? total[r] : 0;
-            in.cnt[k]   = r < R1 ? count_bits(act, (uint32_t)r * P, P) : 0u;
+            in.cnt[k]   = r < R1 ? region_active(act, (uint32_t)r, P) : 0u;
         }
 }
 
@@ -179,7 +204,6 @@ __device__ void score_regions_sum(const QueryDesc& qd, const ScoreInputs<NT>& in
     static_assert(NT % 64 == 0 && NT <= 1024, "score block");
     constexpr int KR = ScoreInputs<NT>::KR;
     const int R1     = qd.R1;
-    const uint32_t P = (uint32_t)qd.P;
     const float eps  = qd.cfg.epsilon;
     int P2           = 1;
     while(P2 < R1) P2 <<= 1;
@@ -191,17 +215,7 @@ __device__ void score_regions_sum(const QueryDesc& qd, const ScoreInputs<NT>& in
 #pragma unroll
     for(int k = 0; k < KR; ++k)
         {
-            float s = 0.0f;
-            if(vc[k] > 0)
-                {
-                    const float cov = (float)cnt[k] / (float)P;
-                    const float vf  = (float)vc[k];
-                    const float fv  = (eps + vf) / ((eps + vf) + (float)(tc[k] - vc[k])) * qd.vol;
-                    const float f2  = fv * fv;
-                    const float f4  = f2 * f2;
-                    const float tt  = (float)tc[k];
-                    s               = f4 / ((1.0f + cov) * (1.0f + tt * tt));
-                }
+            const float s = vc[k] > 0 ? region_raw_score(qd, vc[k], tc[k], cnt[k]) : 0.0f;
             s0[k] = sv[k] = s;
         }
     int st = P2 / 2;
@@ -366,6 +380,7 @@ __device__ void init_query(const QueryDesc& qd, QueryState& st, const Pools& P)
             P.actNext[qd.subOff + w] = 0u;
         }
     for(uint32_t c = threadIdx.x; c <= qd.nChunks; c += kBlock) P.partA[qd.chunkOff + c] = 0u;
+    for(uint32_t i = threadIdx.x; i < 4 * (qd.nChunks + 1); i += kBlock) P.look[4 * (size_t)qd.chunkOff + i] = 0ull;
     const uint32_t rootRep = qd.cfg.root_repeat < qd.cap ? qd.cfg.root_repeat : qd.cap;
     for(uint32_t k = threadIdx.x; k < rootRep; k += kBlock) P.repIdx[qd.nodeOff + k] = 0u;
     for(int d = threadIdx.x; d < qd.D; d += kBlock) P.treeS[qd.sampleOff + (size_t)d * qd.cap] = qd.start[d];
@@ -649,13 +664,18 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
 
 template <int MODEL, int NT, int K>
 __global__ __launch_bounds__(NT, 4) void k_propagate(const QueryDesc* __restrict__ qds,
-                                                  const QueryState* __restrict__ sts, Pools P, WorkLists W,
+                                                  QueryState* __restrict__ sts, Pools P, WorkLists W,
                                                   uint32_t parity)
 {
     static_assert(NT % 64 == 0 && kChunk % (NT / K) == 0, "tile size");
     extern __shared__ float4 sObs[];
     __shared__ TileLds<NT, K> L;
     const uint32_t nq        = W.nq;
+    // the first workgroup commits the next-state fields (written when the
+    // last iteration closed) to the current ones, which the update kernel
+    // reads; propagate itself reads only the next-state fields
+    if(blockIdx.x == 0)
+        for(uint32_t q = threadIdx.x; q < nq; q += NT) commit_state(sts[q]);
     const uint32_t* tileList = W.tiles(parity);
     const uint32_t nTiles    = tileList[nq];
     uint32_t staged          = 0xffffffffu;
@@ -703,223 +723,6 @@ __global__ __launch_bounds__(NT, 4) void k_propagate(const QueryDesc* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// score: one workgroup (1024 threads) per query
-// ---------------------------------------------------------------------------
-
-__global__ __launch_bounds__(kScoreThreads) void k_score(const QueryDesc* __restrict__ qds,
-                                                         QueryState* __restrict__ sts, Pools P)
-{
-    __shared__ float a[kScoreThreads];
-    __shared__ uint32_t sbuf[kScoreThreads / 64 + 1];
-    const uint32_t q    = blockIdx.x;
-    QueryState& st      = sts[q];
-    const QueryDesc& qd = qds[q];
-    // the state of this iteration (closed by the previous one; every thread
-    // reads the next-state fields) and everything that depends only on the
-    // query's description are loaded together, before the done test: the
-    // score inputs, the first bitset word to copy, the first chunk counts
-    const uint32_t done = st.doneN, nC = st.nChildrenN, tree = st.treeSizeN, itr = st.itrN;
-    const uint32_t t    = threadIdx.x;
-    const uint32_t nW   = (uint32_t)qd.R2words;
-    uint32_t* pa        = P.partA + qd.chunkOff;
-    const uint32_t aw   = t < nW ? P.actNext[qd.subOff + t] : 0u;
-    const uint32_t cRaw = t < qd.nChunks ? pa[t] : 0u;
-    ScoreInputs<kScoreThreads> in;
-    score_load<kScoreThreads>(qd, P.total + qd.regOff, P.validc + qd.regOff, P.actNext + qd.subOff, in);
-    // the first thread copies the next-state fields to the current ones
-    if(threadIdx.x == 0) commit_state(st);
-    if(done) return;
-    const uint32_t nCC = (nC + kChunk - 1) / kChunk;
-    const uint32_t c0  = t < nCC ? cRaw : 0u;
-    score_regions_sum<kScoreThreads>(qd, in, P.score + qd.regOff, a);
-    if(t < nW) P.actPrev[qd.subOff + t] = aw;
-    for(uint32_t w = t + kScoreThreads; w < nW; w += kScoreThreads) P.actPrev[qd.subOff + w] = P.actNext[qd.subOff + w];
-    // exclusive scan of the chunk counts in place
-    uint32_t nAcc;
-    const uint32_t ex0 = block_excl_scan<kScoreThreads>(c0, sbuf, nAcc);
-    if(t < nCC) pa[t] = ex0;
-    for(uint32_t base = kScoreThreads; base < nCC; base += kScoreThreads)
-        {
-            const uint32_t i = base + t;
-            const uint32_t v = i < nCC ? pa[i] : 0u;
-            uint32_t tot;
-            const uint32_t ex = block_excl_scan<kScoreThreads>(v, sbuf, tot);
-            if(i < nCC) pa[i] = nAcc + ex;
-            nAcc += tot;
-        }
-    if(threadIdx.x == 0)
-        {
-            const uint32_t avail = qd.cap - tree;
-            const uint32_t next  = nAcc < avail ? nAcc : avail;
-            st.nAcc              = nAcc;
-            st.next              = next;
-            st.fAccept           = faccept(qd, itr, tree, next);
-        }
-}
-
-// ---------------------------------------------------------------------------
-// append: child chunks (compaction + tree append + goal) and old-node chunks
-// (re-activation)
-// ---------------------------------------------------------------------------
-// Each thread of a chunk job handles kPer consecutive slots, so one block
-// scan covers the chunk and the flag bytes come in one 4-byte load (node
-// ranges are kNodeAlign-aligned and padded, so the load stays inside the
-// query's range even past its capacity).
-constexpr uint32_t kPer = kChunk / kBlock;
-static_assert(kPer == 4, "four slots per thread");
-
-__device__ void append_children(const QueryDesc& qd, QueryState& st, uint32_t c, const Pools& P, uint32_t* sbuf)
-{
-    const uint32_t cap = qd.cap, nC = st.nChildren, tree = st.treeSize, next = st.next;
-    const size_t nodeO     = qd.nodeOff;
-    float* __restrict__ T       = P.treeS + qd.sampleOff;
-    const int D            = qd.D;
-    const uint32_t u0      = c * kChunk + kPer * threadIdx.x;
-    const uint32_t chunkA  = P.partA[qd.chunkOff + c];  // flagged children before the chunk
-    uint32_t fl            = 0;
-    int32_t r1[kPer]       = {0, 0, 0, 0};
-    int32_t vcount[kPer]   = {0, 0, 0, 0};
-    constexpr int kRowMax = row_words(KGMT_MAX_SAMPLE);
-    const int RW          = row_words(D);
-    float v[kPer][kRowMax];  // rows as stored: parent and grandparent bits, then the sample
-    if(u0 < nC)
-        {
-            // flags and regions together
-            const uint32_t w = *reinterpret_cast<const uint32_t*>(P.fnext + nodeO + u0);
-            const int4 rr    = *reinterpret_cast<const int4*>(P.r1child + nodeO + u0);
-            r1[0] = rr.x, r1[1] = rr.y, r1[2] = rr.z, r1[3] = rr.w;
-#pragma unroll
-            for(uint32_t k = 0; k < kPer; ++k)
-                if(u0 + k < nC && ((w >> (8 * k)) & 0xffu)) fl |= 1u << k;
-        }
-    // Before the rank scan: the regions' valid counts for the repeat count
-    // and the rows of the flagged children (appended unless the tree fills
-    // up). A flagged child's row sits at its 32-child segment's start plus
-    // the flagged children before it in the segment (eight threads of four).
-    const uint32_t nf    = (uint32_t)__popc(fl);
-    uint32_t slot        = (u0 & ~31u) + group8_incl_scan(nf) - nf;
-#pragma unroll
-    for(uint32_t k = 0; k < kPer; ++k)
-        if((fl >> k) & 1u)
-            {
-                vcount[k]         = P.validc[qd.regOff + r1[k]];
-                const float4* row = reinterpret_cast<const float4*>(P.childR + qd.rowOffC + (size_t)slot * RW);
-#pragma unroll
-                for(int j = 0; j < kRowMax / 4; ++j)
-                    if(4 * j < RW)
-                        {
-                            const float4 x = row[j];
-                            v[k][4 * j] = x.x, v[k][4 * j + 1] = x.y, v[k][4 * j + 2] = x.z, v[k][4 * j + 3] = x.w;
-                        }
-                ++slot;
-            }
-    uint32_t tot;
-    uint32_t rank = chunkA + block_excl_scan<kBlock>((uint32_t)__popc(fl), sbuf, tot);
-    // appended slots: u0 + k with flag k set and rank < next
-    uint32_t app = 0, t0 = 0;
-#pragma unroll
-    for(uint32_t k = 0; k < kPer; ++k)
-        if((fl >> k) & 1u)
-            {
-                if(rank < next)
-                    {
-                        if(!app) t0 = tree + rank;
-                        app |= 1u << k;
-                    }
-                ++rank;
-            }
-    uint32_t rep[kPer] = {0u, 0u, 0u, 0u};
-    if(app)
-        {
-            uint32_t t = t0;
-#pragma unroll
-            for(uint32_t k = 0; k < kPer; ++k)
-                if((app >> k) & 1u)
-                    {
-#pragma unroll
-                        for(int d = 0; d < KGMT_MAX_SAMPLE; ++d)
-                            if(d < D) T[(size_t)d * cap + t] = v[k][2 + d];
-                        P.parent[nodeO + t]   = __float_as_int(v[k][0]);
-                        P.grand[nodeO + t]    = __float_as_int(v[k][1]);
-                        const float dg        = goal_dist(qd, v[k][2], v[k][3], v[k][4]);
-                        P.cost[nodeO + t]     = dg;
-                        P.frontier[nodeO + t] = 1;
-                        P.r1node[nodeO + t]   = r1[k];
-                        rep[k] = vcount[k] < qd.cfg.sparse_valid_below ? qd.cfg.repeat_sparse : qd.cfg.repeat_dense;
-                        if(dg < qd.cfg.goal_thresh) atomicMin(&st.goalIdx, (int32_t)t);
-                        ++t;
-                    }
-        }
-    if(u0 < nC) *reinterpret_cast<uint4*>(P.repU + nodeO + u0) = make_uint4(rep[0], rep[1], rep[2], rep[3]);
-    block_excl_scan<kBlock>(rep[0] + rep[1] + rep[2] + rep[3], sbuf, tot);
-    if(threadIdx.x == 0) P.partC[qd.chunkOff + c] = tot;
-}
-
-__device__ void reactivate(const QueryDesc& qd, QueryState& st, uint32_t o, const Pools& P,
-                           uint32_t* sbuf)
-{
-    const uint32_t tree = st.treeSize, itr = st.itr;
-    const float fA      = st.fAccept;
-    const size_t nodeO  = qd.nodeOff;
-    const uint32_t i0   = o * kChunk + kPer * threadIdx.x;
-    uint32_t cnt        = 0;
-    if(i0 < tree)
-        {
-            const uint32_t fw = *reinterpret_cast<const uint32_t*>(P.frontier + nodeO + i0);
-            const int4 rr     = *reinterpret_cast<const int4*>(P.r1node + nodeO + i0);
-            const int32_t r[kPer] = {rr.x, rr.y, rr.z, rr.w};
-            float sc[kPer];
-#pragma unroll
-            for(uint32_t k = 0; k < kPer; ++k)
-                {
-                    const bool cand = i0 + k < tree && ((fw >> (8 * k)) & 0xffu) == 0;
-                    sc[k]           = cand ? P.score[qd.regOff + r[k]] : 0.0f;
-                }
-            uint32_t rw = 0;
-#pragma unroll
-            for(uint32_t k = 0; k < kPer; ++k)
-                {
-                    const bool cand = i0 + k < tree && ((fw >> (8 * k)) & 0xffu) == 0;
-                    if(cand && draw_react(qd.cfg, qd.cfg.query_id, itr, i0 + k) <= sc[k] + fA)
-                        {
-                            // byte store: the new nodes' flags past the tree are
-                            // written concurrently by the child chunk jobs
-                            P.frontier[nodeO + i0 + k] = 1;
-                            rw |= 1u << (8 * k);
-                            ++cnt;
-                        }
-                }
-            *reinterpret_cast<uint32_t*>(P.react + nodeO + i0) = rw;  // bytes past the tree read as 0
-        }
-    uint32_t tot;
-    block_excl_scan<kBlock>(cnt, sbuf, tot);
-    if(threadIdx.x == 0) P.partB[qd.chunkOff + o] = tot;
-}
-
-__global__ __launch_bounds__(kBlock) void k_append(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
-                                                   Pools P, WorkLists W, uint32_t parity)
-{
-    __shared__ uint32_t sbuf[kBlock / 64 + 1];
-    const uint32_t nq       = W.nq;
-    const uint32_t* jobList = W.jobs(parity);
-    const uint32_t nJobs    = jobList[nq];
-    for(uint32_t j = blockIdx.x; j < nJobs; j += gridDim.x)
-        {
-            const uint32_t q     = locate(jobList, nq, j);
-            QueryState& st       = sts[q];
-            if(st.done) continue;  // stale list entry of a finished query
-            const QueryDesc& qd  = qds[q];
-            const uint32_t local = j - jobList[q];
-            const uint32_t nCC   = (st.nChildren + kChunk - 1) / kChunk;
-            if(local < nCC)
-                append_children(qd, st, local, P, sbuf);
-            else
-                reactivate(qd, st, local - nCC, P, sbuf);
-            __syncthreads();
-        }
-}
-
-// ---------------------------------------------------------------------------
 // closing an iteration
 // ---------------------------------------------------------------------------
 // End of an iteration for one query (one thread): tree size, stop rules,
@@ -958,8 +761,7 @@ __device__ void finish_query(const QueryDesc& qd, QueryState& st, uint32_t R)
 }
 
 // Work lists of a single-query table for its next iteration (parity
-// buffer), from the next-state fields (one thread). The completion word stays
-// 0 until publish_single sets it.
+// buffer), from the next-state fields (one thread).
 __device__ __forceinline__ void single_worklists(const QueryState& st, const WorkLists& W, uint32_t parity)
 {
     uint32_t* tileStart = W.tileStart + parity * 2;
@@ -972,147 +774,641 @@ __device__ __forceinline__ void single_worklists(const QueryState& st, const Wor
     W.active[parity] = alive ? 1u : 0u;
 }
 
-__device__ void walk_path(const QueryDesc& qd, QueryState& st, const Pools& P, float* __restrict__ out);
+__device__ void walk_path(const QueryDesc& qd, QueryState& st, const Pools& P, float* __restrict__ out, int32_t goal);
 
-// End of a single-query plan (one thread, next-state fields final): the goal
-// path rows and the result into host-mapped memory, then the completion word,
-// so the host can return without waiting for the launches it queued past the
-// last iteration (they find the query done and exit). Without a host result
-// buffer only the completion word is set.
-__device__ void publish_single(const QueryDesc& qd, QueryState& st, const Pools& P, const WorkLists& W)
+// How a single-query plan ended.
+struct Outcome
+{
+    uint32_t success, iterations, treeSize;
+    int32_t goal;
+    uint64_t expanded, flagged;
+};
+
+// End of a single-query plan (one thread): the goal path rows and the result
+// into host-mapped memory, then the completion word, so the host can return
+// without waiting for the launches it queued past the last iteration (they
+// find the query done and exit). Without a host result buffer only the
+// completion word is set.
+__device__ void publish_single(const QueryDesc& qd, QueryState& st, const Pools& P, const WorkLists& W,
+                               const Outcome& r)
 {
     if(!W.result)
         {
             __hip_atomic_store(W.allDone, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             return;
         }
-    if(st.successN && st.pathRows == 0) walk_path(qd, st, P, W.hostPath);
+    if(r.success && st.pathRows == 0) walk_path(qd, st, P, W.hostPath, r.goal);
     HostResult* h = W.result;
-    h->success    = st.successN;
-    h->iterations = st.finishItrN;
-    h->treeSize   = st.treeSizeN;
-    h->pathRows   = st.successN ? st.pathRows : 0u;
-    h->goalIdx    = st.successN ? st.goalIdx : -1;
-    h->pathCost   = st.successN ? st.pathCost : 0.0f;
-    h->pathLen    = st.successN ? st.pathLen : 0.0f;
-    h->expanded   = st.expandedN;
-    h->flagged    = st.flaggedN;
+    h->success    = r.success;
+    h->iterations = r.iterations;
+    h->treeSize   = r.treeSize;
+    h->pathRows   = r.success ? st.pathRows : 0u;
+    h->goalIdx    = r.success ? r.goal : -1;
+    h->pathCost   = r.success ? st.pathCost : 0.0f;
+    h->pathLen    = r.success ? st.pathLen : 0.0f;
+    h->expanded   = r.expanded;
+    h->flagged    = r.flagged;
     __threadfence_system();
     __hip_atomic_store(W.allDone, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ---------------------------------------------------------------------------
-// expand: next expansion list (re-activated ascending, then new nodes each
-// repeated repU times), truncated at capacity entries. Each job sums the
-// partial counts before its chunk itself (no communication between
-// workgroups inside the kernel); the first job of a query also records the
-// list length R.
+// update: the rest of an iteration in one launch: region scores, tree
+// append, goal test, re-activation and the next expansion list
+// (Graph.cu:207-358, KGMT.cu:305-403)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_expand(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
-                                                   Pools P, WorkLists W, uint32_t parity)
+// A query's jobs, in the order of their entries in the next expansion list:
+// its old-node chunks (one entry per re-activated node, ascending), then its
+// child chunks (each appended node repeated 15 or 1 times). A job finds the
+// entries before it by a decoupled look-back (Merrill and Garland,
+// "Single-pass Parallel Prefix Scan with Decoupled Look-back", 2016) over one
+// 64-bit word per job (two sets, by iteration parity; each launch empties
+// the other set for the next): it publishes its own count and smallest goal
+// node (aggregate) as soon as it has them, reads the words before it until
+// one holds a prefix that includes its job (inclusive), and then publishes
+// its own inclusive word. A job waits only for jobs held by workgroups that
+// are running, and those publish their aggregates without waiting, so every
+// wait ends: with no more jobs than workgroups, workgroup j does job j (the
+// dispatcher starts the workgroups of an XCD in index order, so a workgroup
+// whose predecessor has not started leaves that predecessor's XCD slots to
+// lower-index workgroups or to other kernels, all of which finish); with
+// more, workgroups take jobs from a counter in order. The wait is bounded all
+// the same (kLookBackPolls); a job that runs out sets the error word, which
+// the host reports, and goes on with what it has.
+// Every job derives the iteration's totals it needs itself rather than
+// waiting for a launch that computes them once: the flagged children of all
+// chunks (and of the chunks before its own) from the chunk counts, and, for
+// a re-activation job, the region scores (their sum is one fixed pairwise
+// tree, so every job gets the same bits). Job 0 stores the scores and the
+// subregion snapshot that the next propagate reads.
+// Each thread of a job handles kPer consecutive slots, so one block scan
+// covers the chunk and the flag bytes come in one 4-byte load (node ranges
+// are kNodeAlign-aligned and padded, so the load stays inside the query's
+// range even past its capacity).
+constexpr uint32_t kPer = kChunk / kBlock;
+static_assert(kPer == 4, "four slots per thread");
+constexpr uint32_t kLbNone        = (1u << 27) - 1;  // no goal node (node indices stay below 2^26)
+constexpr uint64_t kLbAggregate   = 1ull << 62;
+constexpr uint64_t kLbInclusive   = 2ull << 62;
+constexpr uint32_t kLookBackPolls = 1u << 20;
+
+// LDS of an update workgroup
+struct UpdateLds
 {
-    __shared__ uint32_t sbuf[3 * (kBlock / 64)];
-    const uint32_t nq       = W.nq;
-    const uint32_t* jobList = W.jobs(parity);
-    const uint32_t nJobs    = jobList[nq];
-    for(uint32_t j = blockIdx.x; j < nJobs; j += gridDim.x)
+    union
+    {
+        float raw[kMaxRegions];        // raw region scores (-1: no valid child) of a re-activation job
+        uint32_t stage[kMaxRegions];   // then the job's expansion-list entries
+    };
+    float part[kBlock];                 // the score tree's partial sums
+    uint32_t sbuf[3 * (kBlock / 64)];   // block scans
+    uint32_t job, before, goalBefore, nAcc;
+    float sum;
+};
+
+// word: flag (bits 62-63), smallest goal node (32-58), entries (0-31)
+__device__ __forceinline__ uint64_t lb_word(uint64_t flag, uint32_t entries, uint32_t goal)
+{
+    return flag | ((uint64_t)goal << 32) | entries;
+}
+
+__device__ __forceinline__ uint32_t wave_min_u(uint32_t v)
+{
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1)
         {
-            const uint32_t q      = locate(jobList, nq, j);
-            QueryState& st        = sts[q];
-            if(st.done) continue;  // stale list entry of a finished query
-            const QueryDesc& qd   = qds[q];
-            const uint32_t local  = j - jobList[q];
-            const uint32_t nC     = st.nChildren;
-            const uint32_t nCC    = (nC + kChunk - 1) / kChunk;
-            const uint32_t cap    = qd.cap, tree = st.treeSize;
-            const uint32_t nOC    = (tree + kChunk - 1) / kChunk;
-            const size_t nodeO    = qd.nodeOff;
-            uint32_t* out         = P.repIdx + nodeO;
-            const uint32_t* pB    = P.partB + qd.chunkOff;
-            const uint32_t* pC    = P.partC + qd.chunkOff;
-            const uint32_t tx     = threadIdx.x;
-            // one scan of three values per job: [0] the job's own counts,
-            // [1] the entries before the chunk, [2] (first job only) the rest
-            // of the list, so that R = total[1] + total[2] there
-            uint32_t v[3] = {0u, 0u, 0u}, ex[3], tot[3];
-            if(local < nCC)
+            const uint32_t t = (uint32_t)__shfl_xor((int)v, o, 64);
+            v                = t < v ? t : v;
+        }
+    return v;
+}
+
+// Entries and smallest goal node of the jobs before job li (words: the
+// query's job words); every lane of one wave. False if a word did not appear
+// within the poll bound (the sums then cover the words seen).
+__device__ bool look_back(const uint64_t* words, uint32_t li, uint32_t& entries, uint32_t& goal)
+{
+    const int32_t lane = (int32_t)lane_id();
+    uint32_t c = 0, g = kLbNone, polls = 0;
+    int32_t base = (int32_t)li;  // this round reads words base-1 (lane 0) down to base-64
+    while(true)
+        {
+            const int32_t idx = base - 1 - lane;
+            // before the first job: an inclusive word of nothing
+            const uint64_t w = idx >= 0 ? __hip_atomic_load(words + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                        : lb_word(kLbInclusive, 0u, kLbNone);
+            const uint32_t flag = (uint32_t)(w >> 62);
+            const uint64_t incl = __ballot(flag == 2u);
+            const uint64_t none = __ballot(flag == 0u);
+            // lanes up to and including the nearest inclusive word
+            const uint64_t need = incl ? ((incl & (~incl + 1ull)) << 1) - 1ull : ~0ull;
+            if(none & need)
                 {
-                    // new nodes of chunk `local`, each repeated repU times
-                    const uint32_t c    = local;
-                    const uint32_t u0   = c * kChunk + kPer * tx;
-                    uint32_t rep[kPer] = {0u, 0u, 0u, 0u};
-                    if(u0 < nC)
+                    if(++polls >= kLookBackPolls)
                         {
-                            const uint4 r4 = *reinterpret_cast<const uint4*>(P.repU + nodeO + u0);
-                            rep[0] = r4.x, rep[1] = r4.y, rep[2] = r4.z, rep[3] = r4.w;
-#pragma unroll
-                            for(uint32_t k = 1; k < kPer; ++k)
-                                if(u0 + k >= nC) rep[k] = 0u;
+                            entries = c, goal = g;
+                            return false;
                         }
-                    const uint32_t offA = P.partA[qd.chunkOff + c];
-                    for(uint32_t i = tx; i < nOC; i += kBlock) v[1] += pB[i];
-                    for(uint32_t i = tx; i < c; i += kBlock) v[1] += pC[i];
-                    if(local == 0)
-                        for(uint32_t i = tx; i < nCC; i += kBlock) v[2] += pC[i];
-                    uint32_t nn = 0, ns = 0;
-#pragma unroll
-                    for(uint32_t k = 0; k < kPer; ++k) nn += rep[k] ? 1u : 0u, ns += rep[k];
-                    v[0] = (nn << 16) | ns;
-                    block_excl_scan_n<kBlock, 3>(v, ex, tot, sbuf);
-                    uint32_t t        = tree + offA + (ex[0] >> 16);
-                    uint32_t pos      = tot[1] + (ex[0] & 0xffffu);
-#pragma unroll
-                    for(uint32_t k = 0; k < kPer; ++k)
-                        if(rep[k])
-                            {
-                                for(uint32_t m = 0; m < rep[k]; ++m)
-                                    if(pos + m < cap) out[pos + m] = t;
-                                pos += rep[k];
-                                ++t;
-                            }
-                    // chunk counts start at zero for the next propagate
-                    if(tx == 0) P.partA[qd.chunkOff + c] = 0u;
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
                 }
+            const bool use = (need >> lane) & 1ull;
+            const uint32_t wc = use ? (uint32_t)w : 0u;
+            const uint32_t wg = use ? (uint32_t)(w >> 32) & kLbNone : kLbNone;
+            c += (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(wc), 63);
+            const uint32_t gm = wave_min_u(wg);
+            g = gm < g ? gm : g;
+            if(incl)
+                {
+                    entries = c, goal = g;
+                    return true;
+                }
+            base -= 64;
+        }
+}
+
+// Exclusive scan of v and minimum of m over a block of NT threads, one
+// barrier pair; sbuf: 2 * NT/64 words. Call uniformly.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan_min(uint32_t v, uint32_t m, uint32_t* sbuf, uint32_t& total,
+                                                        uint32_t& mmin)
+{
+    constexpr int NW = NT / 64;
+    static_assert(NW >= 2 && NW <= 16, "block of 128 to 1024 threads");
+    const uint32_t incl = wave_incl_scan(v);
+    const uint32_t lane = lane_id();
+    const int wave      = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / 64));
+    if(__ballot(m != kLbNone)) m = wave_min_u(m);
+    if(lane == 63) sbuf[wave] = incl, sbuf[NW + wave] = m;
+    __syncthreads();
+    const uint32_t w  = row_incl_scan(lane < (uint32_t)NW ? sbuf[lane] : 0u);
+    const uint32_t pw = wave == 0 ? 0u : (uint32_t)__builtin_amdgcn_readlane((int)w, wave - 1);
+    total             = (uint32_t)__builtin_amdgcn_readlane((int)w, NW - 1);
+    uint32_t mm       = kLbNone;
+#pragma unroll
+    for(int i = 0; i < NW; ++i) mm = sbuf[NW + i] < mm ? sbuf[NW + i] : mm;
+    mmin = mm;
+    __syncthreads();
+    return pw + incl - v;
+}
+
+// Region scores for a re-activation job (Graph.cu:239-257 per region,
+// :297-304 and :349-357 for the sum): raw scores into L.raw (-1 for a region
+// without valid children), their sum by the pairwise tree of
+// score_regions_sum, here over all kMaxRegions leaves (the levels above the
+// query's own padded size add zeros, which leave every partial sum's bits
+// unchanged), into L.sum; with the flagged children of the iteration
+// (nAcc, acc: this thread's share of the chunk counts) into L.nAcc.
+// Barriers inside: call uniformly.
+__device__ void job_scores(const QueryDesc& qd, const Pools& P, uint32_t acc, UpdateLds& L)
+{
+    const int R1             = qd.R1;
+    const uint32_t Pn        = (uint32_t)qd.P;
+    const int t              = (int)threadIdx.x;
+    const int32_t* total     = P.total + qd.regOff;
+    const int32_t* validc    = P.validc + qd.regOff;
+    const uint32_t* act      = P.actNext + qd.subOff;
+    // 16 regions per thread and round (one round up to 4096 regions): every
+    // load of a round is issued before any is used (indices clamped into the
+    // query's regions, so no load sits behind a branch)
+    constexpr int KB     = 16;
+    const bool oneWord   = Pn <= 32u && 32u % Pn == 0u;
+    for(int k0 = t; k0 < R1; k0 += KB * (int)kBlock)
+        {
+            int32_t vc[KB], tc[KB];
+            uint32_t aw[KB];
+#pragma unroll
+            for(int k = 0; k < KB; ++k)
+                {
+                    const int r  = k0 + k * (int)kBlock;
+                    const int rr = r < R1 ? r : R1 - 1;
+                    vc[k]        = validc[rr];
+                    tc[k]        = total[rr];
+                    aw[k]        = oneWord ? act[((uint32_t)rr * Pn) >> 5] : 0u;
+                }
+#pragma unroll
+            for(int k = 0; k < KB; ++k)
+                {
+                    const int r = k0 + k * (int)kBlock;
+                    if(r < R1 && vc[k] > 0)
+                        {
+                            const uint32_t b   = (uint32_t)r * Pn;
+                            const uint32_t m   = Pn == 32u ? 0xffffffffu : ((1u << (Pn & 31u)) - 1u) << (b & 31u);
+                            const uint32_t cnt = oneWord ? (uint32_t)__popc(aw[k] & m) : count_bits(act, b, Pn);
+                            L.raw[r]           = region_raw_score(qd, vc[k], tc[k], cnt);
+                        }
+                    else if(r < R1)
+                        L.raw[r] = -1.0f;
+                }
+        }
+    // the flagged children of the iteration, per wave (used after the tree)
+    const uint32_t lane    = lane_id();
+    const int wave         = __builtin_amdgcn_readfirstlane(t / 64);
+    const uint32_t waveAcc = (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(acc), 63);
+    if(lane == 0) L.sbuf[wave] = waveAcc;
+    __syncthreads();
+    // the thread's leaves t + k kBlock; the levels with stride >= kBlock add
+    // the thread's own registers
+    constexpr int KR = kMaxRegions / (int)kBlock;
+    float sv[KR];
+#pragma unroll
+    for(int k = 0; k < KR; ++k)
+        {
+            const int r = t + k * (int)kBlock;
+            const float x = r < R1 ? L.raw[r] : 0.0f;
+            sv[k]         = x < 0.0f ? 0.0f : x;
+        }
+#pragma unroll
+    for(int m = KR / 2; m >= 1; m >>= 1)
+        {
+#pragma unroll
+            for(int k = 0; k < m; ++k) sv[k] = sv[k] + sv[k + m];
+        }
+    L.part[t] = sv[0];
+    __syncthreads();
+    if(t < 64)
+        {
+            // lane l holds partial sums l + 64 j; then the last six levels
+            // pair lanes within the wave (as score_regions_sum)
+            constexpr int NJ = (int)kBlock / 64;
+            float x[NJ];
+#pragma unroll
+            for(int j = 0; j < NJ; ++j) x[j] = L.part[t + 64 * j];
+#pragma unroll
+            for(int m = NJ / 2; m >= 1; m >>= 1)
+                {
+#pragma unroll
+                    for(int j = 0; j < m; ++j) x[j] = x[j] + x[j + m];
+                }
+            float v = x[0];
+            v       = v + lane_down<32>(v);
+            v       = v + lane_down<16>(v);
+            v       = v + lane_down<8>(v);
+            v       = v + lane_down<4>(v);
+            v       = v + lane_down<2>(v);
+            v       = v + lane_down<1>(v);
+            if(t == 0)
+                {
+                    L.sum       = v;
+                    uint32_t na = 0;
+#pragma unroll
+                    for(int w = 0; w < NJ; ++w) na += L.sbuf[w];
+                    L.nAcc = na;
+                }
+        }
+    __syncthreads();
+}
+
+// score of a region from its raw score and the sum (-1: no valid child)
+__device__ __forceinline__ float region_score(const QueryDesc& qd, float raw, float sum)
+{
+    return raw < 0.0f ? 1.0f : qd.cfg.epsilon + raw / sum;
+}
+
+// Old-node chunk o: the nodes not in the frontier whose draw u is at most
+// their region's score plus fA join it again (KGMT.cu:364-376); scores from
+// L.raw and sum (own) or from P.score; entries: one per re-activated node
+// (node[k] = its index, rep[k] = 1)
+__device__ __forceinline__ void react_job(const QueryDesc& qd, const QueryState& st, uint32_t o, const Pools& P,
+                                          uint32_t fw, const int4 rr, const float (&u)[kPer], bool own,
+                                          const float* raw, float sum, float fA, uint32_t (&node)[kPer],
+                                          uint32_t (&rep)[kPer])
+{
+    const uint32_t tree = st.treeSize;
+    const size_t nodeO  = qd.nodeOff;
+    const uint32_t i0   = o * kChunk + kPer * threadIdx.x;
+#pragma unroll
+    for(uint32_t k = 0; k < kPer; ++k) node[k] = i0 + k, rep[k] = 0u;
+    if(i0 >= tree) return;
+    const int32_t r[kPer] = {rr.x, rr.y, rr.z, rr.w};
+    float sc[kPer];
+#pragma unroll
+    for(uint32_t k = 0; k < kPer; ++k)
+        {
+            const bool cand = i0 + k < tree && ((fw >> (8 * k)) & 0xffu) == 0;
+            sc[k] = !cand ? 0.0f : own ? region_score(qd, raw[r[k]], sum) : P.score[qd.regOff + r[k]];
+        }
+    uint32_t rw = 0;
+#pragma unroll
+    for(uint32_t k = 0; k < kPer; ++k)
+        {
+            const bool cand = i0 + k < tree && ((fw >> (8 * k)) & 0xffu) == 0;
+            if(cand && u[k] <= sc[k] + fA)
+                {
+                    // byte store: the new nodes' flags past the tree are
+                    // written concurrently by the child chunk jobs
+                    P.frontier[nodeO + i0 + k] = 1;
+                    rw |= 1u << (8 * k);
+                    rep[k] = 1u;
+                }
+        }
+    *reinterpret_cast<uint32_t*>(P.react + nodeO + i0) = rw;  // bytes past the tree read as 0
+}
+
+// Child chunk c: the flagged children whose rank (flagged children before
+// them) is below `next` (nAcc clamped to the free capacity) are appended at
+// tree + rank with their distance to the goal (KGMT.cu:305-345); entries:
+// each appended node repeated 15 or 1 times by its region's valid count
+// (KGMT.cu:336-339); goal: this thread's smallest appended index within
+// goal_thresh (kLbNone if none). pAll / pBefore: this thread's share of the
+// chunk counts of all chunks / of the chunks before c.
+__device__ __forceinline__ void append_job(const QueryDesc& qd, const QueryState& st, uint32_t c, const Pools& P,
+                                           uint32_t pAll, uint32_t pBefore, uint32_t* sbuf, uint32_t (&node)[kPer],
+                                           uint32_t (&rep)[kPer], uint32_t& goal, uint32_t& nAcc)
+{
+    const uint32_t cap = qd.cap, nC = st.nChildren, tree = st.treeSize;
+    const size_t nodeO    = qd.nodeOff;
+    float* __restrict__ T = P.treeS + qd.sampleOff;
+    const int D           = qd.D;
+    const uint32_t u0     = c * kChunk + kPer * threadIdx.x;
+    uint32_t fl           = 0;
+    int32_t r1[kPer]      = {0, 0, 0, 0};
+    int32_t vcount[kPer]  = {0, 0, 0, 0};
+    constexpr int kRowMax = row_words(KGMT_MAX_SAMPLE);
+    const int RW          = row_words(D);
+    float v[kPer][kRowMax];  // rows as stored: parent and grandparent bits, then the sample
+    if(u0 < nC)
+        {
+            // flags and regions together
+            const uint32_t w = *reinterpret_cast<const uint32_t*>(P.fnext + nodeO + u0);
+            const int4 rr    = *reinterpret_cast<const int4*>(P.r1child + nodeO + u0);
+            r1[0] = rr.x, r1[1] = rr.y, r1[2] = rr.z, r1[3] = rr.w;
+#pragma unroll
+            for(uint32_t k = 0; k < kPer; ++k)
+                if(u0 + k < nC && ((w >> (8 * k)) & 0xffu)) fl |= 1u << k;
+        }
+    // Before the rank scan: the regions' valid counts for the repeat count
+    // and the rows of the flagged children (appended unless the tree fills
+    // up). A flagged child's row sits at its 32-child segment's start plus
+    // the flagged children before it in the segment (eight threads of four).
+    const uint32_t nf = (uint32_t)__popc(fl);
+    uint32_t slot     = (u0 & ~31u) + group8_incl_scan(nf) - nf;
+#pragma unroll
+    for(uint32_t k = 0; k < kPer; ++k)
+        if((fl >> k) & 1u)
+            {
+                vcount[k]         = P.validc[qd.regOff + r1[k]];
+                const float4* row = reinterpret_cast<const float4*>(P.childR + qd.rowOffC + (size_t)slot * RW);
+#pragma unroll
+                for(int j = 0; j < kRowMax / 4; ++j)
+                    if(4 * j < RW)
+                        {
+                            const float4 x = row[j];
+                            v[k][4 * j] = x.x, v[k][4 * j + 1] = x.y, v[k][4 * j + 2] = x.z, v[k][4 * j + 3] = x.w;
+                        }
+                ++slot;
+            }
+    // one scan: ranks inside the chunk, flagged children of all chunks and
+    // of the chunks before this one
+    const uint32_t in3[3] = {nf, pAll, pBefore};
+    uint32_t ex3[3], tot3[3];
+    block_excl_scan_n<kBlock, 3>(in3, ex3, tot3, sbuf);
+    nAcc                = tot3[1];
+    const uint32_t next = nAcc < cap - tree ? nAcc : cap - tree;
+    uint32_t rank       = tot3[2] + ex3[0];
+    goal                = kLbNone;
+    uint32_t t          = tree + rank;
+#pragma unroll
+    for(uint32_t k = 0; k < kPer; ++k)
+        {
+            rep[k]  = 0u;
+            node[k] = t;
+            if((fl >> k) & 1u)
+                {
+                    if(rank < next)
+                        {
+#pragma unroll
+                            for(int d = 0; d < KGMT_MAX_SAMPLE; ++d)
+                                if(d < D) T[(size_t)d * cap + t] = v[k][2 + d];
+                            P.parent[nodeO + t]   = __float_as_int(v[k][0]);
+                            P.grand[nodeO + t]    = __float_as_int(v[k][1]);
+                            const float dg        = goal_dist(qd, v[k][2], v[k][3], v[k][4]);
+                            P.cost[nodeO + t]     = dg;
+                            P.frontier[nodeO + t] = 1;
+                            P.r1node[nodeO + t]   = r1[k];
+                            rep[k] = vcount[k] < qd.cfg.sparse_valid_below ? qd.cfg.repeat_sparse : qd.cfg.repeat_dense;
+                            if(dg < qd.cfg.goal_thresh && goal == kLbNone) goal = t;
+                        }
+                    ++rank;
+                    ++t;
+                }
+        }
+}
+
+// One job of query q (li: its index among the query's jobs). own: the
+// scores come from the region counts (the planner); otherwise from P.score
+// (the stage entry, which is given them).
+__device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts, const Pools& P,
+                           const WorkLists& W, uint32_t parity, uint32_t q, uint32_t li, bool own, UpdateLds& L)
+{
+    QueryState& st      = sts[q];
+    const QueryDesc& qd = qds[q];
+    const uint32_t nq   = W.nq;
+    const uint32_t tree = st.treeSize, nC = st.nChildren, cap = qd.cap;
+    const uint32_t nOC  = (tree + kChunk - 1) / kChunk;
+    const uint32_t nCC  = (nC + kChunk - 1) / kChunk;
+    const uint32_t nJ   = nOC + nCC;
+    const uint32_t* pa  = P.partA + qd.chunkOff;  // flagged children per child chunk
+    const size_t wpq    = 2 * ((size_t)qd.nChunks + 1);  // words of one parity
+    uint64_t* words     = P.look + 4 * (size_t)qd.chunkOff + parity * wpq;
+    uint64_t* wordsNext = P.look + 4 * (size_t)qd.chunkOff + (parity ^ 1u) * wpq;
+    const uint32_t c    = li - nOC;  // child chunk of a child job
+    // this thread's share of the chunk counts: the first without a branch
+    // (the index clamped into the query's chunks), the rest past kBlock chunks
+    const uint32_t i0c = threadIdx.x < qd.nChunks ? threadIdx.x : qd.nChunks;
+    const uint32_t x0  = pa[i0c];
+    uint32_t pAll      = threadIdx.x < nCC ? x0 : 0u;
+    uint32_t pBefore   = threadIdx.x < nCC && threadIdx.x < c ? x0 : 0u;
+    for(uint32_t i = threadIdx.x + kBlock; i < nCC; i += kBlock)
+        {
+            const uint32_t x = pa[i];
+            pAll += x;
+            pBefore += i < c ? x : 0u;
+        }
+    uint32_t node[kPer], rep[kPer], goal = kLbNone, nAcc;
+    if(li < nOC)
+        {
+            // the job's nodes first, then the totals (their loads overlap)
+            const uint32_t i0 = li * kChunk + kPer * threadIdx.x;
+            uint32_t fw       = 0;
+            int4 rr           = make_int4(0, 0, 0, 0);
+            if(i0 < tree)
+                {
+                    fw = *reinterpret_cast<const uint32_t*>(P.frontier + qd.nodeOff + i0);
+                    rr = *reinterpret_cast<const int4*>(P.r1node + qd.nodeOff + i0);
+                }
+            // the draws need neither loads nor scores: computed while they come
+            float u[kPer];
+#pragma unroll
+            for(uint32_t k = 0; k < kPer; ++k) u[k] = draw_react(qd.cfg, qd.cfg.query_id, st.itr, i0 + k);
+            if(own)
+                job_scores(qd, P, pAll, L);
             else
                 {
-                    // re-activated old nodes of chunk `local - nCC`, ascending
-                    const uint32_t o  = local - nCC;
-                    const uint32_t i0 = o * kChunk + kPer * tx;
-                    const uint32_t rw = i0 < tree ? *reinterpret_cast<const uint32_t*>(P.react + nodeO + i0) : 0u;
-                    for(uint32_t i = tx; i < o; i += kBlock) v[1] += pB[i];
-                    if(local == 0)
-                        for(uint32_t i = tx; i < nOC; i += kBlock) v[2] += pB[i];
-                    uint32_t fl = 0;
-#pragma unroll
-                    for(uint32_t k = 0; k < kPer; ++k)
-                        if(i0 + k < tree && ((rw >> (8 * k)) & 0xffu)) fl |= 1u << k;
-                    v[0] = (uint32_t)__popc(fl);
-                    block_excl_scan_n<kBlock, 3>(v, ex, tot, sbuf);
-                    uint32_t pos = tot[1] + ex[0];
-#pragma unroll
-                    for(uint32_t k = 0; k < kPer; ++k)
-                        if((fl >> k) & 1u)
-                            {
-                                if(pos < cap) out[pos] = i0 + k;
-                                ++pos;
-                            }
+                    uint32_t tot;
+                    block_excl_scan<kBlock>(pAll, L.sbuf, tot);
+                    if(threadIdx.x == 0) L.nAcc = tot, L.sum = 0.0f;
+                    __syncthreads();
                 }
-            if(local == 0 && tx == 0)
+            nAcc                = L.nAcc;
+            const uint32_t next = nAcc < cap - tree ? nAcc : cap - tree;
+            react_job(qd, st, li, P, fw, rr, u, own, L.raw, L.sum, faccept(qd, st.itr, tree, next), node, rep);
+        }
+    else
+        append_job(qd, st, c, P, pAll, pBefore, L.sbuf, node, rep, goal, nAcc);
+    uint32_t total, jobGoal;
+    const uint32_t ex = block_excl_scan_min<kBlock>(rep[0] + rep[1] + rep[2] + rep[3], goal, L.sbuf, total, jobGoal);
+    if(threadIdx.x == 0)
+        {
+            __hip_atomic_store(words + li, lb_word(li == 0 ? kLbInclusive : kLbAggregate, total, jobGoal),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            L.before = 0u, L.goalBefore = kLbNone;
+        }
+    // the aggregate goes out before the look-back's loads
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    // the job's entries into LDS while the look-back runs (written out below
+    // as whole rows; a job with more entries than the stage holds, and job 0,
+    // whose region scores stay in L.raw, write them directly)
+    const bool staged = total <= (uint32_t)kMaxRegions && !(li == 0 && own);
+    if(staged && threadIdx.x >= 64)
+        {
+            uint32_t p = ex;
+#pragma unroll
+            for(uint32_t k = 0; k < kPer; ++k)
                 {
-                    const uint32_t R = tot[1] + tot[2];
+                    for(uint32_t m = 0; m < rep[k]; ++m) L.stage[p + m] = node[k];
+                    p += rep[k];
+                }
+        }
+    if(li > 0 && threadIdx.x < 64)
+        {
+            uint32_t pc, pg;
+            const bool ok = look_back(words, li, pc, pg);
+            if(threadIdx.x == 0)
+                {
+                    if(!ok) __hip_atomic_store(W.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    __hip_atomic_store(words + li, lb_word(kLbInclusive, pc + total, jobGoal < pg ? jobGoal : pg),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    L.before = pc, L.goalBefore = pg;
+                }
+        }
+    if(staged && threadIdx.x < 64)
+        {
+            uint32_t p = ex;
+#pragma unroll
+            for(uint32_t k = 0; k < kPer; ++k)
+                {
+                    for(uint32_t m = 0; m < rep[k]; ++m) L.stage[p + m] = node[k];
+                    p += rep[k];
+                }
+        }
+    // (work that no job of this launch reads) the other parity's words start
+    // empty for the next launch; job 0 stores the scores and the subregion
+    // snapshot for the next propagate
+    for(size_t i = (size_t)li * kBlock + threadIdx.x; i < wpq; i += (size_t)nJ * kBlock) wordsNext[i] = 0ull;
+    if(li == 0 && own)
+        {
+            for(int r = threadIdx.x; r < qd.R1; r += kBlock) P.score[qd.regOff + r] = region_score(qd, L.raw[r], L.sum);
+            for(int w = threadIdx.x; w < qd.R2words; w += kBlock) P.actPrev[qd.subOff + w] = P.actNext[qd.subOff + w];
+        }
+    __syncthreads();
+    const uint32_t before = L.before, goalBefore = L.goalBefore;
+    // the job's entries of the expansion list at before.., its first
+    // capacity entries kept
+    uint32_t* out = P.repIdx + qd.nodeOff;
+    if(staged)
+        {
+            for(uint32_t e = threadIdx.x; e < total; e += kBlock)
+                if(before + e < cap) out[before + e] = L.stage[e];
+        }
+    else
+        {
+            uint32_t pos = before + ex;
+#pragma unroll
+            for(uint32_t k = 0; k < kPer; ++k)
+                {
+                    for(uint32_t m = 0; m < rep[k]; ++m)
+                        if(pos + m < cap) out[pos + m] = node[k];
+                    pos += rep[k];
+                }
+        }
+    const uint32_t next = nAcc < cap - tree ? nAcc : cap - tree;
+    const bool last     = li == nJ - 1;
+    // The last job's look-back has seen a word of every other job, each
+    // published after that job read the chunk counts: they start at zero
+    // again for the next propagate.
+    if(last)
+        for(uint32_t i = threadIdx.x; i < nCC; i += kBlock) P.partA[qd.chunkOff + i] = 0u;
+    if(threadIdx.x == 0)
+        {
+            const uint32_t goalAll = jobGoal < goalBefore ? jobGoal : goalBefore;
+            // a single query's job holding the smallest goal node publishes
+            // the solved plan: the goal row was written by this workgroup,
+            // the rest of its path by earlier launches
+            if(nq == 1 && jobGoal != kLbNone && goalBefore == kLbNone)
+                {
+                    const Outcome r{1u, st.itr, tree + next, (int32_t)jobGoal, st.expanded + nC, st.flagged + nAcc};
+                    publish_single(qd, st, P, W, r);
+                }
+            // the last job knows the list size and the goal: it closes the
+            // iteration of a single query (the next-state fields and work
+            // lists, read from the next kernel on), k_finish those of a batch
+            if(last)
+                {
+                    const uint32_t R = before + total;
                     st.R             = R;
-                    // a single query's first job closes the iteration (the
-                    // next-state fields and work lists, read from the next
-                    // kernel on; the other jobs read only the current
-                    // fields), so it needs no k_finish
+                    st.goalIdx       = goalAll == kLbNone ? kNoGoal : (int32_t)goalAll;
+                    st.nAcc          = nAcc;
+                    st.next          = next;
+                    st.fAccept       = faccept(qd, st.itr, tree, next);
                     if(nq == 1)
                         {
                             finish_query(qd, st, R);
                             single_worklists(st, W, parity ^ 1u);
-                            if(st.doneN) publish_single(qd, st, P, W);
+                            if(st.doneN && !st.successN)
+                                {
+                                    const Outcome r{0u, st.finishItrN, st.treeSizeN, -1, st.expandedN, st.flaggedN};
+                                    publish_single(qd, st, P, W, r);
+                                }
                         }
                 }
-            __syncthreads();
+        }
+}
+
+__global__ __launch_bounds__(kBlock) void k_update(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
+                                                   Pools P, WorkLists W, uint32_t parity, uint32_t own)
+{
+    __shared__ UpdateLds L;
+    const uint32_t nq       = W.nq;
+    const uint32_t* jobList = W.jobs(parity);
+    // one query: its jobs 0 .. nJ-1 straight from its state (the list holds
+    // the same), without waiting for the list first
+    const QueryState& st0 = sts[0];
+    const uint32_t nJobs  = nq > 1 ? jobList[nq]
+                            : st0.done ? 0u
+                                       : (st0.treeSize + kChunk - 1) / kChunk + (st0.nChildren + kChunk - 1) / kChunk;
+    // with no more jobs than workgroups, workgroup j does job j; with more,
+    // every job comes from the counter (the other parity's counter starts at
+    // zero for the next launch)
+    if(blockIdx.x >= nJobs) return;
+    if(blockIdx.x == 0 && threadIdx.x == 0) W.claim[parity ^ 1u] = 0u;
+    const bool more = nJobs > gridDim.x;
+    while(true)
+        {
+            uint32_t j = blockIdx.x;
+            if(more)
+                {
+                    if(threadIdx.x == 0)
+                        L.job = __hip_atomic_fetch_add(W.claim + parity, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __syncthreads();
+                    j = L.job;
+                    if(j >= nJobs) return;
+                }
+            const uint32_t q = nq > 1 ? locate(jobList, nq, j) : 0u;
+            // (a stale list entry of a finished query has nothing to do)
+            if(!sts[q].done) update_job(qds, sts, P, W, parity, q, nq > 1 ? j - jobList[q] : j, own != 0, L);
+            if(!more) return;
+            __syncthreads();  // L is rewritten by the next job
         }
 }
 
@@ -1138,7 +1434,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(const QueryDesc* __restrict__
 // goal -> root rows of a solved query (pathRows etc. of st); two levels per
 // round of loads: a and its parent b are known, their rows, b's parent c and
 // grandparent d are loaded together
-__device__ void walk_path(const QueryDesc& qd, QueryState& st, const Pools& P, float* __restrict__ out)
+__device__ void walk_path(const QueryDesc& qd, QueryState& st, const Pools& P, float* __restrict__ out, int32_t goal)
 {
     const uint32_t cap            = qd.cap;
     const int D                   = qd.D;
@@ -1160,7 +1456,7 @@ __device__ void walk_path(const QueryDesc& qd, QueryState& st, const Pools& P, f
         px = r[0], py = r[1], pz = r[2];
         ++rows;
     };
-    int32_t a = st.goalIdx;
+    int32_t a = goal;
     int32_t b = par[a];
     while(a != -1 && rows < maxRows)
         {
@@ -1198,7 +1494,7 @@ __global__ __launch_bounds__(64) void k_paths(const QueryDesc* __restrict__ qds,
     QueryState& st = sts[q];
     commit_state(st);  // the closed last iteration
     if(!st.success || st.pathRows) return;
-    walk_path(qds[q], st, P, P.path + qds[q].pathOff);
+    walk_path(qds[q], st, P, P.path + qds[q].pathOff, st.goalIdx);
 }
 
 // ---------------------------------------------------------------------------
@@ -1349,6 +1645,7 @@ __global__ __launch_bounds__(kBlock) void k_init(const QueryDesc* __restrict__ q
                 build_scene_wg<3>(rows + 6 * (size_t)qd.rowOff, qd.nObs, qd.nGrp, P.obs + 2 * (size_t)qd.obsOff, L);
         }
     init_query(qd, sts[q], P);
+    if(q == 0 && threadIdx.x < 2) W.claim[threadIdx.x] = 0u;
     if(last_arrival(W.counter, nq, &sflag)) build_worklists<kBlock>(sts, W, 0, sbuf);
 }
 
@@ -1537,9 +1834,9 @@ __global__ __launch_bounds__(kBlock) void k_stage_fu_nodes(const QueryDesc* __re
     for(int e = D + 2; e < RW; ++e) row[e] = 0.0f;
 }
 
-// flagged children per chunk, exclusive offsets, and the state the score
-// kernel leaves (accepted count clamped to the free capacity, fAccept); the
-// work lists of the append / expand jobs (one workgroup)
+// flagged children per chunk, the state an iteration's propagate leaves
+// (accepted count and fAccept for the info output) and the work lists of the
+// update jobs (one workgroup)
 __global__ __launch_bounds__(kBlock) void k_stage_fu_state(const QueryDesc* __restrict__ qdp, QueryState* st,
                                                            Pools P, WorkLists W, uint32_t itr, uint32_t tree,
                                                            uint32_t nC)
@@ -1558,7 +1855,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_fu_state(const QueryDesc* __re
     uint32_t acc = 0;
     for(uint32_t c = 0; c < nCC; ++c)
         {
-            P.partA[c] = acc;
+            P.partA[c] = cnt[c];
             acc += cnt[c];
         }
     const uint32_t avail = qd.cap - tree;
@@ -1573,6 +1870,9 @@ __global__ __launch_bounds__(kBlock) void k_stage_fu_state(const QueryDesc* __re
     s.goalIdx   = kNoGoal;
     *st         = s;
     const uint32_t jobs = nCC + (tree + kChunk - 1) / kChunk;
+    for(uint32_t i = 0; i < jobs; ++i) P.look[i] = 0ull;  // parity 0's words
+    W.claim[0] = W.claim[1] = 0u;
+    *W.error   = 0u;
     for(uint32_t par = 0; par < 2; ++par)
         {
             W.jobStart[2 * par] = 0, W.jobStart[2 * par + 1] = jobs;
@@ -1586,9 +1886,9 @@ __global__ void k_stage_fu_info(const QueryDesc* __restrict__ qdp, const QuerySt
                                 int32_t* info)
 {
     if(threadIdx.x != 0) return;
-    const uint32_t nOC = (tree + kChunk - 1) / kChunk;
-    uint32_t react     = 0;
-    for(uint32_t o = 0; o < nOC; ++o) react += P.partB[o];
+    // the inclusive entries of the last old-node job: the re-activated count
+    const uint32_t nOC   = (tree + kChunk - 1) / kChunk;
+    const uint32_t react = (uint32_t)P.look[nOC - 1];
     info[0] = (int32_t)st->next;
     info[1] = (int32_t)st->R;
     info[2] = st->goalIdx == kNoGoal ? -1 : st->goalIdx;
@@ -1772,16 +2072,10 @@ hipError_t launch_propagate(int model, const QueryDesc* qds, QueryState* sts, co
         }
 }
 
-hipError_t launch_score(const QueryDesc* qds, QueryState* sts, const Pools& P, uint32_t nq, hipStream_t s)
+hipError_t launch_update(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
+                         uint32_t parity, const Launch& L, bool own, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_score, dim3(nq), dim3(kScoreThreads), 0, s, qds, sts, P);
-    return hipGetLastError();
-}
-
-hipError_t launch_append(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
-                         uint32_t parity, const Launch& L, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_append, dim3(L.chunkGrid), dim3(kBlock), 0, s, qds, sts, P, W, parity);
+    hipLaunchKernelGGL(k_update, dim3(L.chunkGrid), dim3(kBlock), 0, s, qds, sts, P, W, parity, own ? 1u : 0u);
     return hipGetLastError();
 }
 
@@ -1805,13 +2099,6 @@ hipError_t launch_paths(const QueryDesc* qds, QueryState* sts, const Pools& P, u
 {
     if(nq == 0) return hipSuccess;
     hipLaunchKernelGGL(k_paths, dim3((nq + 63) / 64), dim3(64), 0, s, qds, sts, P, nq);
-    return hipGetLastError();
-}
-
-hipError_t launch_expand(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
-                         uint32_t parity, const Launch& L, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_expand, dim3(L.chunkGrid), dim3(kBlock), 0, s, qds, sts, P, W, parity);
     return hipGetLastError();
 }
 
@@ -1899,8 +2186,7 @@ hipError_t launch_stage_frontier_update(const QueryDesc* qd, QueryState* st, con
     hipLaunchKernelGGL(k_stage_fu_state, dim3(1), dim3(kBlock), 0, s, qd, st, P, W, itr, tree, nC);
     Launch L;
     L.chunkGrid = 256;
-    hipError_t e = launch_append(qd, st, P, W, 0, L, s);
-    if(e == hipSuccess) e = launch_expand(qd, st, P, W, 0, L, s);
+    const hipError_t e = launch_update(qd, st, P, W, 0, L, false, s);
     if(e != hipSuccess) return e;
     hipLaunchKernelGGL(k_stage_fu_info, dim3(1), dim3(64), 0, s, qd, st, P, tree, info);
     return hipGetLastError();

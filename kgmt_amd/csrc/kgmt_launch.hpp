@@ -14,7 +14,7 @@ struct Launch
     uint32_t propGrid  = 1024;  // persistent workgroups of the propagate kernel
     uint32_t propBlock = kTile; // threads per propagate workgroup (128, 256, 512)
     uint32_t lanes     = 1;     // lanes per child (1; 2 or 4 with 256 threads); children per tile = propBlock / lanes
-    uint32_t chunkGrid = 1024;  // persistent workgroups of append / expand
+    uint32_t chunkGrid = 1024;  // workgroups of the update kernel
 };
 
 // Scenes: boxes sorted along a Morton curve of their centres, grouped by
@@ -37,11 +37,10 @@ hipError_t launch_init(const QueryDesc* qds, QueryState* sts, const Pools& P, co
                        hipStream_t s);
 hipError_t launch_propagate(int model, const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
                             uint32_t parity, uint32_t maxPairs, const Launch& L, hipStream_t s);
-hipError_t launch_score(const QueryDesc* qds, QueryState* sts, const Pools& P, uint32_t nq, hipStream_t s);
-hipError_t launch_append(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
-                         uint32_t parity, const Launch& L, hipStream_t s);
-hipError_t launch_expand(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
-                         uint32_t parity, const Launch& L, hipStream_t s);
+// own: the region scores from the region counts (the planner), or given in
+// P.score (the frontier update stage)
+hipError_t launch_update(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
+                         uint32_t parity, const Launch& L, bool own, hipStream_t s);
 hipError_t launch_finish(const QueryDesc* qds, QueryState* sts, const WorkLists& W, uint32_t parity, hipStream_t s);
 // goal paths of the solved queries, once a plan has ended
 hipError_t launch_paths(const QueryDesc* qds, QueryState* sts, const Pools& P, uint32_t nq, hipStream_t s);
@@ -66,8 +65,8 @@ hipError_t launch_stage_propagate_planner(int model, const QueryDesc* qd, const 
                                           float* box, hipStream_t s);
 // frontier update of one iteration on a one-query pool (the caller's tree
 // already in P.treeS / parent / cost / frontier, region counts and scores in
-// P.validc / P.score): staging kernels, then the planner's k_append and
-// k_expand; info[4] = {appended, multiset size, goal node or -1, re-activated}
+// P.validc / P.score): staging kernels, then the planner's k_update;
+// info[4] = {appended, multiset size, goal node or -1, re-activated}
 hipError_t launch_stage_frontier_update(const QueryDesc* qd, QueryState* st, const Pools& P, const WorkLists& W,
                                         uint32_t itr, uint32_t tree, const float* child, const int32_t* childParent,
                                         const uint8_t* fnext, uint32_t nC, int32_t* info, hipStream_t s);

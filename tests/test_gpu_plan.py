@@ -138,6 +138,10 @@ def test_profile_counts_children_and_flags(orc):
             else:
                 assert res.tree_size < q.cfg.capacity and p["flagged"] == res.tree_size - 1
             assert p["kernels"]["propagate"]["launches"] >= res.iterations
+            if level == 2:  # two launches per single-query iteration
+                k = p["kernels"]
+                assert k["update"]["launches"] == k["propagate"]["launches"]
+                assert k["finish"]["launches"] == 0
 
 
 def test_plan_device_obstacles(orc):
