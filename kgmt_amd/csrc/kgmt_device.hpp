@@ -155,6 +155,8 @@ struct WorkLists
     uint32_t* claim;       // 2 (by parity): update jobs taken in a launch (the launch before zeroes it)
     uint32_t* allDone;     // host-mapped word: 1 once every query is done
     uint32_t* error;       // 1 once an update job's look-back wait ran out (host-mapped for plans)
+    uint32_t* closed;      // host-mapped word: iterations closed in this plan (the host paces on it)
+    uint32_t* closedDev;   // the same count in device memory (its one writer reads it back)
     HostResult* result;    // host-mapped result of a single-query plan (null: batch, or no early result)
     float* hostPath;       // host-mapped goal -> root rows of that plan ((max_iter + 2) x D floats)
     uint32_t nq;
@@ -271,18 +273,29 @@ __device__ __forceinline__ void sincos_k(float x, float& s, float& c)
 // a box hit implies a hit of its group's bounds, and the verdict is an OR
 // over boxes, independent of their order.
 // -------------------------------------------------------------------------
+// A scene staged in LDS: nGrp group (lo, hi) pairs, then the lo corners of
+// its kGroup * nGrp boxes, then their hi corners (lanes that read
+// consecutive boxes read consecutive 16-byte words: no bank conflicts).
 struct Scene
 {
-    const float4* grp;  // nGrp (lo, hi) pairs, then kGroup * nGrp boxes (LDS)
+    const float4* grp;  // the staged scene
     int nGrp;
     int sub = 0;        // lane within the child's group of K lanes
+    __device__ __forceinline__ const float4* box_lo() const { return grp + 2 * nGrp; }
+    __device__ __forceinline__ const float4* box_hi() const { return grp + 2 * nGrp + nGrp * (int)kGroup; }
 };
 
-// copy `pairs` (lo, hi) float4 pairs of a scene into LDS
-__device__ __forceinline__ void stage_scene(const float4* __restrict__ pool, uint32_t off, uint32_t pairs,
+// copy a scene of nGrp groups, stored as (lo, hi) float4 pairs at pool + 2
+// off, into LDS in the layout of Scene
+__device__ __forceinline__ void stage_scene(const float4* __restrict__ pool, uint32_t off, uint32_t nGrp,
                                             float4* sObs)
 {
-    for(uint32_t i = threadIdx.x; i < 2 * pairs; i += blockDim.x) sObs[i] = pool[2 * (size_t)off + i];
+    const uint32_t nBox = nGrp * kGroup, n = 2 * (nGrp + nBox);
+    for(uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+        {
+            const uint32_t k = i - 2 * nGrp;  // box word: box k / 2, corner k % 2
+            sObs[i < 2 * nGrp ? i : 2 * nGrp + (k & 1u) * nBox + (k >> 1)] = pool[2 * (size_t)off + i];
+        }
 }
 
 // OR of a flag over each group of K consecutive lanes (K = 1, 2, 4), by DPP
@@ -318,8 +331,9 @@ __device__ __forceinline__ bool segment_free(float ax, float ay, float az, float
     const float mnx = ax > bx ? bx : ax, mxx = ax > bx ? ax : bx;
     const float mny = ay > by ? by : ay, mxy = ay > by ? ay : by;
     const float mnz = az > bz ? bz : az, mxz = az > bz ? az : bz;
-    const float4* box = sc.grp + 2 * sc.nGrp;
-    bool hit            = false;
+    const float4* blo = sc.box_lo();
+    const float4* bhi = sc.box_hi();
+    bool hit          = false;
     for(int g0 = 0; g0 < sc.nGrp; g0 += (int)kGroupBatch * K)
         {
             bool near[kGroupBatch];
@@ -346,10 +360,11 @@ __device__ __forceinline__ bool segment_free(float ax, float ay, float az, float
                 {
                     if(__any(near[k] && !hit))
                         {
-                            const float4* b = box + 2 * (int)kGroup * gi[k];
+                            const float4* lo = blo + (int)kGroup * gi[k];
+                            const float4* hi = bhi + (int)kGroup * gi[k];
 #pragma unroll 4
                             for(int m = 0; m < (int)kGroup; ++m)
-                                hit |= !separated(mnx, mny, mnz, mxx, mxy, mxz, b[2 * m], b[2 * m + 1]);
+                                hit |= !separated(mnx, mny, mnz, mxx, mxy, mxz, lo[m], hi[m]);
                         }
                 }
         }
@@ -941,20 +956,21 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
             float ulo[3], uhi[3];
 #pragma unroll
             for(int d = 0; d < 3; ++d) ulo[d] = wave_minf(dry.lo[d]), uhi[d] = wave_maxf(dry.hi[d]);
-            const float4* box = sc.grp + 2 * sc.nGrp;
+            const float4* blo = sc.box_lo();
+            const float4* bhi = sc.box_hi();
             const int nBox    = sc.nGrp * (int)kGroup;
             for(int b0 = 0; b0 < nBox && n <= kCand; b0 += 64)
                 {
                     const int b = b0 + (int)__lane_id();
                     bool pass   = false;
                     if(b < nBox)
-                        pass = !separated(ulo[0], ulo[1], ulo[2], uhi[0], uhi[1], uhi[2], box[2 * b], box[2 * b + 1]);
+                        pass = !separated(ulo[0], ulo[1], ulo[2], uhi[0], uhi[1], uhi[2], blo[b], bhi[b]);
                     uint64_t m = __ballot(pass);
                     while(m && n <= kCand)
                         {
                             const int k = __ffsll((unsigned long long)m) - 1;
                             m &= m - 1;
-                            const float4 lo = box[2 * (b0 + k)], hi = box[2 * (b0 + k) + 1];
+                            const float4 lo = blo[b0 + k], hi = bhi[b0 + k];
                             const bool c = !separated(dry.lo[0], dry.lo[1], dry.lo[2], dry.hi[0], dry.hi[1], dry.hi[2],
                                                       lo, hi);
                             if(__any(c))

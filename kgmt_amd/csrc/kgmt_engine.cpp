@@ -140,9 +140,8 @@ int dev_alloc(T*& p, size_t count, std::vector<void*>& owned)
     return KGMT_OK;
 }
 
-constexpr int kRing = 8;  // completion polls in flight
 constexpr size_t kHostPathFloats = size_t(1) << 20;  // host-mapped path rows of a single query (4 MiB)
-constexpr int kLag  = 2;  // launch units the host stays behind the GPU
+constexpr uint32_t kLag = 2;  // launch units the host stays ahead of the GPU
 
 }  // namespace
 
@@ -214,8 +213,8 @@ struct Engine
     uint32_t maxIter = 0;
     hipStream_t stream = nullptr;
     bool ownStream     = false;
-    uint32_t* pinned   = nullptr;  // host-mapped words written by the kernels: completion, look-back error
-    hipEvent_t ev[kRing] = {};
+    uint32_t* pinned   = nullptr;  // host-mapped words written by the kernels: completion, look-back error,
+                                   // iterations closed
     Launch L;
     hipGraph_t graph         = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -244,8 +243,6 @@ struct Engine
             if(e) (void)hipEventDestroy(e);
         if(graphExec) (void)hipGraphExecDestroy(graphExec);
         if(graph) (void)hipGraphDestroy(graph);
-        for(auto& e : ev)
-            if(e) (void)hipEventDestroy(e);
         if(pinned) (void)hipHostFree(pinned);
         if(hostRes) (void)hipHostFree(hostRes);
         if(hostPath) (void)hipHostFree(hostPath);
@@ -303,17 +300,19 @@ struct Engine
            (rc = dev_alloc(P.actPrev, subs, owned)) || (rc = dev_alloc(P.actNext, subs, owned)) ||
            (rc = dev_alloc(P.path, paths, owned)) || (rc = dev_alloc(W.tileStart, 2 * (size_t)(n + 1), owned)) ||
            (rc = dev_alloc(W.jobStart, 2 * (size_t)(n + 1), owned)) || (rc = dev_alloc(W.active, 2, owned)) ||
-           (rc = dev_alloc(W.counter, 1, owned)) || (rc = dev_alloc(W.claim, 2, owned)))
+           (rc = dev_alloc(W.counter, 1, owned)) || (rc = dev_alloc(W.claim, 2, owned)) ||
+           (rc = dev_alloc(W.closedDev, 1, owned)))
             return rc;
         W.nq   = n;
         W.tile = L.propBlock / L.lanes;
         HIP_TRY(hipMemset(W.counter, 0, sizeof(uint32_t)));
         HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         ownStream = true;
-        HIP_TRY(hipHostMalloc((void**)&pinned, 2 * sizeof(uint32_t), hipHostMallocMapped));
-        pinned[0] = pinned[1] = 0;
+        HIP_TRY(hipHostMalloc((void**)&pinned, 3 * sizeof(uint32_t), hipHostMallocMapped));
+        pinned[0] = pinned[1] = pinned[2] = 0;
         HIP_TRY(hipHostGetDevicePointer((void**)&W.allDone, pinned, 0));
-        W.error = W.allDone + 1;
+        W.error  = W.allDone + 1;
+        W.closed = W.allDone + 2;
         W.result   = nullptr;
         W.hostPath = nullptr;
         const size_t pathFloats = (size_t)(cfgs[0].max_iter + 2) * (size_t)hq[0].D;
@@ -328,7 +327,6 @@ struct Engine
                 HIP_TRY(hipHostMalloc((void**)&hostPath, pathFloats * sizeof(float), hipHostMallocMapped));
                 HIP_TRY(hipHostGetDevicePointer((void**)&W.hostPath, hostPath, 0));
             }
-        for(auto& e : ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         // launch shape: persistent workgroups sized to the device
         hipDeviceProp_t prop;
         HIP_TRY(hipGetDeviceProperties(&prop, device));
@@ -475,8 +473,9 @@ struct Engine
 
     // Upload descriptors, run init + iterations until every query is done.
     // The host enqueues launch units (one iteration, or the two-iteration
-    // graph) and only waits for units kLag behind the newest one, so the GPU
-    // always has queued work; kernels of finished queries exit at once.
+    // graph) and stays at most kLag units ahead of the iterations the device
+    // reports closed, so the GPU always has queued work; kernels of finished
+    // queries exit at once.
     int run(bool useGraph)
     {
         const auto t0 = std::chrono::steady_clock::now();
@@ -488,6 +487,7 @@ struct Engine
         counterDirty = true;
         __atomic_store_n(pinned, 0u, __ATOMIC_RELEASE);
         __atomic_store_n(pinned + 1, 0u, __ATOMIC_RELEASE);
+        __atomic_store_n(pinned + 2, 0u, __ATOMIC_RELEASE);
         HIP_TRY(launch_init(dq, ds, P, W, sceneBuild ? sceneRows : nullptr, stream));
         if(useGraph && (!graphExec || graphObs != maxPairs || graphLaunch.propGrid != L.propGrid ||
                         graphLaunch.chunkGrid != L.chunkGrid || graphLaunch.propBlock != L.propBlock ||
@@ -505,9 +505,20 @@ struct Engine
         if(hostRes) hostRes->iterations = 0;
         bool published = false;
         auto seen      = [&]() { return early && __atomic_load_n(pinned, __ATOMIC_ACQUIRE) != 0; };
+        // The closing job of every iteration (k_finish for a batch) writes the
+        // number of iterations closed so far into host-mapped memory; the host
+        // stays at most kLag launch units ahead of it. No event between the
+        // units: an event record is a system-scope release on the stream.
         iterMs.assign(maxIter + 2, 0.0);
-        uint32_t it = 0, units = 0, observed = 0;
+        uint32_t it = 0, noted = 0, spins = 0;
+        auto lastSeen = std::chrono::steady_clock::now();
         bool finished = false;
+        auto note = [&](uint32_t c) {  // when each iteration was seen closed
+            if(c <= noted) return;
+            const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            for(uint32_t k = noted; k < c && k < maxIter + 2; ++k) iterMs[k] = t;
+            noted = c;
+        };
         while(it < maxIter && !finished)
             {
                 if(seen())
@@ -515,7 +526,6 @@ struct Engine
                         finished = published = true;
                         break;
                     }
-                const int slot = (int)(units % kRing);
                 if(useGraph)
                     HIP_TRY(hipGraphLaunch(graphExec, stream));
                 else
@@ -524,28 +534,41 @@ struct Engine
                                                    prof ? &profEv[(size_t)it * (KGMT_PROFILE_KERNELS + 1)] : nullptr);
                         if(rc) return rc;
                     }
-                HIP_TRY(hipEventRecord(ev[slot], stream));
                 it += perUnit;
-                ++units;
-                while(observed + kLag < units || (it >= maxIter && observed < units))
+                // wait while the device is more than kLag units behind (after
+                // the last unit: until it has closed every iteration)
+                const uint32_t need = it < maxIter ? (it > kLag * perUnit ? it - kLag * perUnit : 0u) : maxIter;
+                while(true)
                     {
-                        hipError_t q;
-                        while((q = hipEventQuery(ev[observed % kRing])) == hipErrorNotReady)
-                            if(seen()) break;
                         if(seen())
                             {
                                 finished = published = true;
                                 break;
                             }
-                        if(q != hipSuccess) return fail(KGMT_E_HIP, std::string("iteration: ") + hipGetErrorString(q));
-                        const double t =
-                          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-                        for(uint32_t k = 0; k < perUnit; ++k) iterMs[std::min<size_t>(observed * perUnit + k, maxIter + 1)] = t;
-                        ++observed;
                         if(__atomic_load_n(pinned, __ATOMIC_ACQUIRE))
                             {
                                 finished = true;
                                 break;
+                            }
+                        const uint32_t c = __atomic_load_n(pinned + 2, __ATOMIC_ACQUIRE);
+                        if(c != noted) lastSeen = std::chrono::steady_clock::now();
+                        note(c);
+                        if(c >= need) break;
+                        // a failed launch (or a stream that drained without
+                        // closing them) ends the wait: asked only after 200 us
+                        // without a closed iteration (a stream query is not free)
+                        if((++spins & 63u) == 0u &&
+                           std::chrono::steady_clock::now() - lastSeen > std::chrono::microseconds(200))
+                            {
+                                const hipError_t q = hipStreamQuery(stream);
+                                if(q == hipSuccess)
+                                    {
+                                        note(__atomic_load_n(pinned + 2, __ATOMIC_ACQUIRE));
+                                        if(__atomic_load_n(pinned, __ATOMIC_ACQUIRE)) finished = true;
+                                        break;
+                                    }
+                                if(q != hipErrorNotReady)
+                                    return fail(KGMT_E_HIP, std::string("iteration: ") + hipGetErrorString(q));
                             }
                     }
             }
@@ -597,6 +620,7 @@ struct Engine
         counterDirty = true;
         __atomic_store_n(pinned, 0u, __ATOMIC_RELEASE);
         __atomic_store_n(pinned + 1, 0u, __ATOMIC_RELEASE);
+        __atomic_store_n(pinned + 2, 0u, __ATOMIC_RELEASE);
         if(hostRes) hostRes->iterations = 0;
         HIP_TRY(launch_init(dq, ds, P, W, sceneBuild ? sceneRows : nullptr, stream));
         int rc = pull_state();
@@ -1429,7 +1453,8 @@ int kgmt_frontier_update(const kgmt_config* cfg, uint32_t itr, uint32_t tree_siz
        (rc = a.get(P.partA, h.nChunks + 1)) || (rc = a.get(P.look, 4 * ((size_t)h.nChunks + 1))) ||
        (rc = a.get(P.validc, h.R1)) || (rc = a.get(P.score, h.R1)) || (rc = a.get(W.tileStart, 4)) ||
        (rc = a.get(W.jobStart, 4)) || (rc = a.get(W.active, 2)) || (rc = a.get(W.counter, 1)) ||
-       (rc = a.get(W.claim, 2)) || (rc = a.get(W.allDone, 1)) || (rc = a.get(W.error, 1)))
+       (rc = a.get(W.claim, 2)) || (rc = a.get(W.allDone, 1)) || (rc = a.get(W.error, 1)) ||
+       (rc = a.get(W.closed, 1)) || (rc = a.get(W.closedDev, 1)))
         return rc;
     W.nq   = 1;
     W.tile = kTile;

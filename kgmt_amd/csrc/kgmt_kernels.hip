@@ -693,7 +693,7 @@ __global__ __launch_bounds__(NT, 4) void k_propagate(const QueryDesc* __restrict
             if(q != staged)
                 {
                     __syncthreads();
-                    stage_scene(P.obs, qd.obsOff, qd.nGrp * (1 + kGroup), sObs);
+                    stage_scene(P.obs, qd.obsOff, qd.nGrp, sObs);
                     __syncthreads();
                     staged = q;
                 }
@@ -758,6 +758,16 @@ __device__ void finish_query(const QueryDesc& qd, QueryState& st, uint32_t R)
     st.doneN      = done;
     st.successN   = success;
     st.finishItrN = fin;
+}
+
+// One more iteration closed in this plan: the count for the host, which
+// stays a few launch units ahead of it (one thread per iteration: the closing
+// job of a single query, k_finish for a batch)
+__device__ __forceinline__ void count_closed(const WorkLists& W)
+{
+    const uint32_t n = *W.closedDev + 1u;
+    *W.closedDev     = n;
+    __hip_atomic_store(W.closed, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Work lists of a single-query table for its next iteration (parity
@@ -1365,6 +1375,7 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
                         {
                             finish_query(qd, st, R);
                             single_worklists(st, W, parity ^ 1u);
+                            count_closed(W);
                             if(st.doneN && !st.successN)
                                 {
                                     const Outcome r{0u, st.finishItrN, st.treeSizeN, -1, st.expandedN, st.flaggedN};
@@ -1424,6 +1435,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(const QueryDesc* __restrict__
         if(!sts[q].done) finish_query(qds[q], sts[q], sts[q].R);
     __syncthreads();
     build_worklists<kBlock>(sts, W, parity ^ 1u, sbuf);
+    if(threadIdx.x == 0) count_closed(W);
 }
 
 // ---------------------------------------------------------------------------
@@ -1646,6 +1658,7 @@ __global__ __launch_bounds__(kBlock) void k_init(const QueryDesc* __restrict__ q
         }
     init_query(qd, sts[q], P);
     if(q == 0 && threadIdx.x < 2) W.claim[threadIdx.x] = 0u;
+    if(q == 0 && threadIdx.x == 0) *W.closedDev = 0u;
     if(last_arrival(W.counter, nq, &sflag)) build_worklists<kBlock>(sts, W, 0, sbuf);
 }
 
@@ -1706,7 +1719,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_propagate(const QueryDesc* __r
                                                             float* child, uint8_t* valid, int32_t* r1o, int32_t* r2o)
 {
     extern __shared__ float4 sObs[];
-    stage_scene(obs, 0, nGrp * (1 + kGroup), sObs);
+    stage_scene(obs, 0, nGrp, sObs);
     __syncthreads();
     stage_propagate_child<MODEL>(*qdp, parentIdx, parentStates, nParents, controls, steps, n, sObs, nGrp, child,
                                  valid, r1o, r2o);
@@ -1732,7 +1745,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_propagate_planner(
     __shared__ uint32_t hist[kBlock];
     __shared__ uint16_t slot[kBlock];
     const QueryDesc& qd = *qdp;
-    stage_scene(obs, 0, nGrp * (1 + kGroup), sObs);
+    stage_scene(obs, 0, nGrp, sObs);
     __syncthreads();
     const uint32_t t    = threadIdx.x;
     const uint32_t base = blockIdx.x * kBlock;

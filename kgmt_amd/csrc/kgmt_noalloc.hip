@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_propagate_by_value(
   const float4* __restrict__ obs, uint32_t nGrp, float* child, uint8_t* valid, int32_t* r1o, int32_t* r2o)
 {
     extern __shared__ float4 sObs[];
-    stage_scene(obs, 0, nGrp * (1 + kGroup), sObs);
+    stage_scene(obs, 0, nGrp, sObs);
     __syncthreads();
     stage_propagate_child<MODEL>(qd, parentIdx, parentStates, nParents, controls, steps, n, sObs, nGrp, child, valid,
                                  r1o, r2o);
