@@ -471,6 +471,29 @@ struct Engine
         return KGMT_OK;
     }
 
+    // The host-mapped words cleared, the descriptors to the device and k_init
+    // enqueued: a single query's descriptor travels as k_init's argument (no
+    // upload); a batch's is copied, and k_init's last workgroup leaves its
+    // arrival counter at zero (after a plan that did not complete it is
+    // cleared here).
+    int start_plan()
+    {
+        __atomic_store_n(pinned, 0u, __ATOMIC_RELEASE);
+        __atomic_store_n(pinned + 1, 0u, __ATOMIC_RELEASE);
+        __atomic_store_n(pinned + 2, 0u, __ATOMIC_RELEASE);
+        const float* rows = sceneBuild ? sceneRows : nullptr;
+        if(nq == 1)
+            {
+                HIP_TRY(launch_init_one(hq[0], dq, ds, P, W, rows, stream));
+                return KGMT_OK;
+            }
+        HIP_TRY(hipMemcpyAsync(dq, hq.data(), sizeof(QueryDesc) * nq, hipMemcpyHostToDevice, stream));
+        if(counterDirty) HIP_TRY(hipMemsetAsync(W.counter, 0, sizeof(uint32_t), stream));
+        counterDirty = true;
+        HIP_TRY(launch_init(dq, ds, P, W, rows, stream));
+        return KGMT_OK;
+    }
+
     // Upload descriptors, run init + iterations until every query is done.
     // The host enqueues launch units (one iteration, or the two-iteration
     // graph) and stays at most kLag units ahead of the iterations the device
@@ -480,15 +503,8 @@ struct Engine
     {
         const auto t0 = std::chrono::steady_clock::now();
         mirValid      = false;
-        HIP_TRY(hipMemcpyAsync(dq, hq.data(), sizeof(QueryDesc) * nq, hipMemcpyHostToDevice, stream));
-        // k_init's last workgroup leaves the arrival counter at zero; after a
-        // plan that did not complete it is cleared here
-        if(counterDirty) HIP_TRY(hipMemsetAsync(W.counter, 0, sizeof(uint32_t), stream));
-        counterDirty = true;
-        __atomic_store_n(pinned, 0u, __ATOMIC_RELEASE);
-        __atomic_store_n(pinned + 1, 0u, __ATOMIC_RELEASE);
-        __atomic_store_n(pinned + 2, 0u, __ATOMIC_RELEASE);
-        HIP_TRY(launch_init(dq, ds, P, W, sceneBuild ? sceneRows : nullptr, stream));
+        int rc0       = start_plan();
+        if(rc0) return rc0;
         if(useGraph && (!graphExec || graphObs != maxPairs || graphLaunch.propGrid != L.propGrid ||
                         graphLaunch.chunkGrid != L.chunkGrid || graphLaunch.propBlock != L.propBlock ||
                         graphLaunch.lanes != L.lanes))
@@ -615,15 +631,10 @@ struct Engine
     int begin()
     {
         mirValid = false;
-        HIP_TRY(hipMemcpyAsync(dq, hq.data(), sizeof(QueryDesc) * nq, hipMemcpyHostToDevice, stream));
-        if(counterDirty) HIP_TRY(hipMemsetAsync(W.counter, 0, sizeof(uint32_t), stream));
-        counterDirty = true;
-        __atomic_store_n(pinned, 0u, __ATOMIC_RELEASE);
-        __atomic_store_n(pinned + 1, 0u, __ATOMIC_RELEASE);
-        __atomic_store_n(pinned + 2, 0u, __ATOMIC_RELEASE);
         if(hostRes) hostRes->iterations = 0;
-        HIP_TRY(launch_init(dq, ds, P, W, sceneBuild ? sceneRows : nullptr, stream));
-        int rc = pull_state();
+        int rc = start_plan();
+        if(rc) return rc;
+        rc = pull_state();
         if(rc) return rc;
         stepping = true;
         stepItr  = 0;

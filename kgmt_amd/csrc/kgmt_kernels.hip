@@ -1662,6 +1662,32 @@ __global__ __launch_bounds__(kBlock) void k_init(const QueryDesc* __restrict__ q
     if(last_arrival(W.counter, nq, &sflag)) build_worklists<kBlock>(sts, W, 0, sbuf);
 }
 
+// k_init of a single query, its descriptor a kernel argument that the first
+// thread stores for the kernels after it (no upload before the launch), and
+// no hand-off between workgroups (there is one)
+__global__ __launch_bounds__(kBlock) void k_init_one(const QueryDesc qd, QueryDesc* __restrict__ dq,
+                                                     QueryState* __restrict__ sts, Pools P, WorkLists W,
+                                                     const float* __restrict__ rows)
+{
+    __shared__ SceneLds L;
+    if(threadIdx.x == 0) *dq = qd;
+    if(rows && qd.nObs)
+        {
+            if(qd.wdim == 2)
+                build_scene_wg<2>(rows + 4 * (size_t)qd.rowOff, qd.nObs, qd.nGrp, P.obs + 2 * (size_t)qd.obsOff, L);
+            else
+                build_scene_wg<3>(rows + 6 * (size_t)qd.rowOff, qd.nObs, qd.nGrp, P.obs + 2 * (size_t)qd.obsOff, L);
+        }
+    init_query(qd, sts[0], P);
+    if(threadIdx.x < 2) W.claim[threadIdx.x] = 0u;
+    // the first thread wrote the state: it builds the first work lists
+    if(threadIdx.x == 0)
+        {
+            *W.closedDev = 0u;
+            single_worklists(sts[0], W, 0u);
+        }
+}
+
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_build_scene(const float* __restrict__ rows, uint32_t n, uint32_t nGrp,
                                                         float4* __restrict__ out)
@@ -2031,6 +2057,13 @@ hipError_t launch_init(const QueryDesc* qds, QueryState* sts, const Pools& P, co
                        hipStream_t s)
 {
     hipLaunchKernelGGL(k_init, dim3(W.nq), dim3(kBlock), 0, s, qds, sts, P, W, W.nq, rows);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_one(const QueryDesc& qd, QueryDesc* dq, QueryState* sts, const Pools& P, const WorkLists& W,
+                           const float* rows, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_init_one, dim3(1), dim3(kBlock), 0, s, qd, dq, sts, P, W, rows);
     return hipGetLastError();
 }
 

@@ -35,6 +35,10 @@ bool launch_supported(uint32_t block, uint32_t lanes);
 // scenes of the queries built from `rows` first (null: no obstacles)
 hipError_t launch_init(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W, const float* rows,
                        hipStream_t s);
+// the same for a single query whose descriptor travels as a kernel argument
+// (stored into *dq by the kernel)
+hipError_t launch_init_one(const QueryDesc& qd, QueryDesc* dq, QueryState* sts, const Pools& P, const WorkLists& W,
+                           const float* rows, hipStream_t s);
 hipError_t launch_propagate(int model, const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
                             uint32_t parity, uint32_t maxPairs, const Launch& L, hipStream_t s);
 // own: the region scores from the region counts (the planner), or given in
