@@ -857,6 +857,9 @@ __device__ void publish_single(const QueryDesc& qd, QueryState& st, const Pools&
 // range even past its capacity).
 constexpr uint32_t kPer = kChunk / kBlock;
 static_assert(kPer == 4, "four slots per thread");
+// update workgroups: the first kBlock threads take a job's slots, all of them
+// the region scores
+constexpr uint32_t kUpd = 2 * kBlock;
 constexpr uint32_t kLbNone        = (1u << 27) - 1;  // no goal node (node indices stay below 2^26)
 constexpr uint64_t kLbAggregate   = 1ull << 62;
 constexpr uint64_t kLbInclusive   = 2ull << 62;
@@ -870,8 +873,8 @@ struct UpdateLds
         float raw[kMaxRegions];        // raw region scores (-1: no valid child) of a re-activation job
         uint32_t stage[kMaxRegions];   // then the job's expansion-list entries
     };
-    float part[kBlock];                 // the score tree's partial sums
-    uint32_t sbuf[3 * (kBlock / 64)];   // block scans
+    float part[kUpd];                   // the score tree's partial sums
+    uint32_t sbuf[3 * (kUpd / 64)];     // block scans
     uint32_t job, before, goalBefore, nAcc;
     float sum;
 };
@@ -978,19 +981,19 @@ __device__ void job_scores(const QueryDesc& qd, const Pools& P, uint32_t acc, Up
     const int32_t* total     = P.total + qd.regOff;
     const int32_t* validc    = P.validc + qd.regOff;
     const uint32_t* act      = P.actNext + qd.subOff;
-    // 16 regions per thread and round (one round up to 4096 regions): every
+    // 16 regions per thread and round (one round up to 8192 regions): every
     // load of a round is issued before any is used (indices clamped into the
     // query's regions, so no load sits behind a branch)
     constexpr int KB     = 16;
     const bool oneWord   = Pn <= 32u && 32u % Pn == 0u;
-    for(int k0 = t; k0 < R1; k0 += KB * (int)kBlock)
+    for(int k0 = t; k0 < R1; k0 += KB * (int)kUpd)
         {
             int32_t vc[KB], tc[KB];
             uint32_t aw[KB];
 #pragma unroll
             for(int k = 0; k < KB; ++k)
                 {
-                    const int r  = k0 + k * (int)kBlock;
+                    const int r  = k0 + k * (int)kUpd;
                     const int rr = r < R1 ? r : R1 - 1;
                     vc[k]        = validc[rr];
                     tc[k]        = total[rr];
@@ -999,7 +1002,7 @@ __device__ void job_scores(const QueryDesc& qd, const Pools& P, uint32_t acc, Up
 #pragma unroll
             for(int k = 0; k < KB; ++k)
                 {
-                    const int r = k0 + k * (int)kBlock;
+                    const int r = k0 + k * (int)kUpd;
                     if(r < R1 && vc[k] > 0)
                         {
                             const uint32_t b   = (uint32_t)r * Pn;
@@ -1017,14 +1020,14 @@ __device__ void job_scores(const QueryDesc& qd, const Pools& P, uint32_t acc, Up
     const uint32_t waveAcc = (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(acc), 63);
     if(lane == 0) L.sbuf[wave] = waveAcc;
     __syncthreads();
-    // the thread's leaves t + k kBlock; the levels with stride >= kBlock add
+    // the thread's leaves t + k kUpd; the levels with stride >= kUpd add
     // the thread's own registers
-    constexpr int KR = kMaxRegions / (int)kBlock;
+    constexpr int KR = kMaxRegions / (int)kUpd;
     float sv[KR];
 #pragma unroll
     for(int k = 0; k < KR; ++k)
         {
-            const int r = t + k * (int)kBlock;
+            const int r = t + k * (int)kUpd;
             const float x = r < R1 ? L.raw[r] : 0.0f;
             sv[k]         = x < 0.0f ? 0.0f : x;
         }
@@ -1040,7 +1043,7 @@ __device__ void job_scores(const QueryDesc& qd, const Pools& P, uint32_t acc, Up
         {
             // lane l holds partial sums l + 64 j; then the last six levels
             // pair lanes within the wave (as score_regions_sum)
-            constexpr int NJ = (int)kBlock / 64;
+            constexpr int NJ = (int)kUpd / 64;
             float x[NJ];
 #pragma unroll
             for(int j = 0; j < NJ; ++j) x[j] = L.part[t + 64 * j];
@@ -1089,7 +1092,7 @@ __device__ __forceinline__ void react_job(const QueryDesc& qd, const QueryState&
     const uint32_t i0   = o * kChunk + kPer * threadIdx.x;
 #pragma unroll
     for(uint32_t k = 0; k < kPer; ++k) node[k] = i0 + k, rep[k] = 0u;
-    if(i0 >= tree) return;
+    if(threadIdx.x >= kBlock || i0 >= tree) return;
     const int32_t r[kPer] = {rr.x, rr.y, rr.z, rr.w};
     float sc[kPer];
 #pragma unroll
@@ -1137,7 +1140,7 @@ __device__ __forceinline__ void append_job(const QueryDesc& qd, const QueryState
     constexpr int kRowMax = row_words(KGMT_MAX_SAMPLE);
     const int RW          = row_words(D);
     float v[kPer][kRowMax];  // rows as stored: parent and grandparent bits, then the sample
-    if(u0 < nC)
+    if(threadIdx.x < kBlock && u0 < nC)
         {
             // flags and regions together
             const uint32_t w = *reinterpret_cast<const uint32_t*>(P.fnext + nodeO + u0);
@@ -1172,7 +1175,7 @@ __device__ __forceinline__ void append_job(const QueryDesc& qd, const QueryState
     // of the chunks before this one
     const uint32_t in3[3] = {nf, pAll, pBefore};
     uint32_t ex3[3], tot3[3];
-    block_excl_scan_n<kBlock, 3>(in3, ex3, tot3, sbuf);
+    block_excl_scan_n<kUpd, 3>(in3, ex3, tot3, sbuf);
     nAcc                = tot3[1];
     const uint32_t next = nAcc < cap - tree ? nAcc : cap - tree;
     uint32_t rank       = tot3[2] + ex3[0];
@@ -1224,12 +1227,12 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
     uint64_t* wordsNext = P.look + 4 * (size_t)qd.chunkOff + (parity ^ 1u) * wpq;
     const uint32_t c    = li - nOC;  // child chunk of a child job
     // this thread's share of the chunk counts: the first without a branch
-    // (the index clamped into the query's chunks), the rest past kBlock chunks
+    // (the index clamped into the query's chunks), the rest past kUpd chunks
     const uint32_t i0c = threadIdx.x < qd.nChunks ? threadIdx.x : qd.nChunks;
     const uint32_t x0  = pa[i0c];
     uint32_t pAll      = threadIdx.x < nCC ? x0 : 0u;
     uint32_t pBefore   = threadIdx.x < nCC && threadIdx.x < c ? x0 : 0u;
-    for(uint32_t i = threadIdx.x + kBlock; i < nCC; i += kBlock)
+    for(uint32_t i = threadIdx.x + kUpd; i < nCC; i += kUpd)
         {
             const uint32_t x = pa[i];
             pAll += x;
@@ -1242,21 +1245,24 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
             const uint32_t i0 = li * kChunk + kPer * threadIdx.x;
             uint32_t fw       = 0;
             int4 rr           = make_int4(0, 0, 0, 0);
-            if(i0 < tree)
+            if(threadIdx.x < kBlock && i0 < tree)
                 {
                     fw = *reinterpret_cast<const uint32_t*>(P.frontier + qd.nodeOff + i0);
                     rr = *reinterpret_cast<const int4*>(P.r1node + qd.nodeOff + i0);
                 }
             // the draws need neither loads nor scores: computed while they come
-            float u[kPer];
+            float u[kPer] = {1.0f, 1.0f, 1.0f, 1.0f};
+            if(threadIdx.x < kBlock)
+                {
 #pragma unroll
-            for(uint32_t k = 0; k < kPer; ++k) u[k] = draw_react(qd.cfg, qd.cfg.query_id, st.itr, i0 + k);
+                    for(uint32_t k = 0; k < kPer; ++k) u[k] = draw_react(qd.cfg, qd.cfg.query_id, st.itr, i0 + k);
+                }
             if(own)
                 job_scores(qd, P, pAll, L);
             else
                 {
                     uint32_t tot;
-                    block_excl_scan<kBlock>(pAll, L.sbuf, tot);
+                    block_excl_scan<kUpd>(pAll, L.sbuf, tot);
                     if(threadIdx.x == 0) L.nAcc = tot, L.sum = 0.0f;
                     __syncthreads();
                 }
@@ -1267,7 +1273,7 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
     else
         append_job(qd, st, c, P, pAll, pBefore, L.sbuf, node, rep, goal, nAcc);
     uint32_t total, jobGoal;
-    const uint32_t ex = block_excl_scan_min<kBlock>(rep[0] + rep[1] + rep[2] + rep[3], goal, L.sbuf, total, jobGoal);
+    const uint32_t ex = block_excl_scan_min<kUpd>(rep[0] + rep[1] + rep[2] + rep[3], goal, L.sbuf, total, jobGoal);
     if(threadIdx.x == 0)
         {
             __hip_atomic_store(words + li, lb_word(li == 0 ? kLbInclusive : kLbAggregate, total, jobGoal),
@@ -1315,11 +1321,11 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
     // (work that no job of this launch reads) the other parity's words start
     // empty for the next launch; job 0 stores the scores and the subregion
     // snapshot for the next propagate
-    for(size_t i = (size_t)li * kBlock + threadIdx.x; i < wpq; i += (size_t)nJ * kBlock) wordsNext[i] = 0ull;
+    for(size_t i = (size_t)li * kUpd + threadIdx.x; i < wpq; i += (size_t)nJ * kUpd) wordsNext[i] = 0ull;
     if(li == 0 && own)
         {
-            for(int r = threadIdx.x; r < qd.R1; r += kBlock) P.score[qd.regOff + r] = region_score(qd, L.raw[r], L.sum);
-            for(int w = threadIdx.x; w < qd.R2words; w += kBlock) P.actPrev[qd.subOff + w] = P.actNext[qd.subOff + w];
+            for(int r = threadIdx.x; r < qd.R1; r += kUpd) P.score[qd.regOff + r] = region_score(qd, L.raw[r], L.sum);
+            for(int w = threadIdx.x; w < qd.R2words; w += kUpd) P.actPrev[qd.subOff + w] = P.actNext[qd.subOff + w];
         }
     __syncthreads();
     const uint32_t before = L.before, goalBefore = L.goalBefore;
@@ -1328,7 +1334,7 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
     uint32_t* out = P.repIdx + qd.nodeOff;
     if(staged)
         {
-            for(uint32_t e = threadIdx.x; e < total; e += kBlock)
+            for(uint32_t e = threadIdx.x; e < total; e += kUpd)
                 if(before + e < cap) out[before + e] = L.stage[e];
         }
     else
@@ -1348,7 +1354,7 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
     // published after that job read the chunk counts: they start at zero
     // again for the next propagate.
     if(last)
-        for(uint32_t i = threadIdx.x; i < nCC; i += kBlock) P.partA[qd.chunkOff + i] = 0u;
+        for(uint32_t i = threadIdx.x; i < nCC; i += kUpd) P.partA[qd.chunkOff + i] = 0u;
     if(threadIdx.x == 0)
         {
             const uint32_t goalAll = jobGoal < goalBefore ? jobGoal : goalBefore;
@@ -1386,7 +1392,7 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
         }
 }
 
-__global__ __launch_bounds__(kBlock) void k_update(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
+__global__ __launch_bounds__(kUpd) void k_update(const QueryDesc* __restrict__ qds, QueryState* __restrict__ sts,
                                                    Pools P, WorkLists W, uint32_t parity, uint32_t own)
 {
     __shared__ UpdateLds L;
@@ -2121,7 +2127,7 @@ hipError_t launch_propagate(int model, const QueryDesc* qds, QueryState* sts, co
 hipError_t launch_update(const QueryDesc* qds, QueryState* sts, const Pools& P, const WorkLists& W,
                          uint32_t parity, const Launch& L, bool own, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_update, dim3(L.chunkGrid), dim3(kBlock), 0, s, qds, sts, P, W, parity, own ? 1u : 0u);
+    hipLaunchKernelGGL(k_update, dim3(L.chunkGrid), dim3(kUpd), 0, s, qds, sts, P, W, parity, own ? 1u : 0u);
     return hipGetLastError();
 }
 
