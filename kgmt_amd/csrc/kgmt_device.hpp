@@ -966,28 +966,53 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
                     if(b < nBox)
                         pass = !separated(ulo[0], ulo[1], ulo[2], uhi[0], uhi[1], uhi[2], blo[b], bhi[b]);
                     uint64_t m = __ballot(pass);
+                    // a box that some lane's box meets joins the list (same
+                    // value in every lane; n past kCand ends the gather)
+                    auto take = [&](const float4& lo, const float4& hi, bool c) {
+                        if(n <= kCand && __any(c))
+                            {
+                                if(n < kCand)
+                                    {
+                                        cand[2 * n] = lo, cand[2 * n + 1] = hi;
+                                        if(c)
+                                            {
+                                                if(n < 64)
+                                                    ownLo |= 1ull << n;
+                                                else
+                                                    ownHi |= 1ull << (n - 64);
+                                            }
+                                    }
+                                ++n;
+                            }
+                    };
+                    // many passing boxes (children of many parents): four per
+                    // round, their reads and tests independent of each other,
+                    // taken in box order
+                    while(__popcll(m) >= 4 && n <= kCand)
+                        {
+                            float4 lo[4], hi[4];
+#pragma unroll
+                            for(int q = 0; q < 4; ++q)
+                                {
+                                    const int k = __ffsll((unsigned long long)m) - 1;
+                                    m &= m - 1;
+                                    lo[q] = blo[b0 + k], hi[q] = bhi[b0 + k];
+                                }
+                            bool c[4];
+#pragma unroll
+                            for(int q = 0; q < 4; ++q)
+                                c[q] = !separated(dry.lo[0], dry.lo[1], dry.lo[2], dry.hi[0], dry.hi[1], dry.hi[2], lo[q],
+                                                  hi[q]);
+#pragma unroll
+                            for(int q = 0; q < 4; ++q) take(lo[q], hi[q], c[q]);
+                        }
                     while(m && n <= kCand)
                         {
                             const int k = __ffsll((unsigned long long)m) - 1;
                             m &= m - 1;
                             const float4 lo = blo[b0 + k], hi = bhi[b0 + k];
-                            const bool c = !separated(dry.lo[0], dry.lo[1], dry.lo[2], dry.hi[0], dry.hi[1], dry.hi[2],
-                                                      lo, hi);
-                            if(__any(c))
-                                {
-                                    if(n < kCand)
-                                        {
-                                            cand[2 * n] = lo, cand[2 * n + 1] = hi;  // same value in every lane
-                                            if(c)
-                                                {
-                                                    if(n < 64)
-                                                        ownLo |= 1ull << n;
-                                                    else
-                                                        ownHi |= 1ull << (n - 64);
-                                                }
-                                        }
-                                    ++n;
-                                }
+                            take(lo, hi,
+                                 !separated(dry.lo[0], dry.lo[1], dry.lo[2], dry.hi[0], dry.hi[1], dry.hi[2], lo, hi));
                         }
                 }
         }
