@@ -365,22 +365,28 @@ __device__ __forceinline__ bool last_arrival(uint32_t* counter, uint32_t expecte
 // ---------------------------------------------------------------------------
 // init (KGMT.cu:50-81): per-query part of k_init, defined after the scene build
 // ---------------------------------------------------------------------------
-// root, cleared counters and the first iteration's state of one query
-__device__ void init_query(const QueryDesc& qd, QueryState& st, const Pools& P)
+// the cleared region counters, scores, subregion bits, chunk counts and
+// job words of one query: thread `rank` of `ranks` clears every ranks-th word
+__device__ void clear_query(const QueryDesc& qd, const Pools& P, uint32_t rank, uint32_t ranks)
 {
-    for(int r = threadIdx.x; r < qd.R1; r += kBlock)
+    for(uint32_t r = rank; r < (uint32_t)qd.R1; r += ranks)
         {
             P.total[qd.regOff + r]  = 0;
             P.validc[qd.regOff + r] = 0;
             P.score[qd.regOff + r]  = 0.0f;
         }
-    for(int w = threadIdx.x; w < qd.R2words; w += kBlock)
+    for(uint32_t w = rank; w < (uint32_t)qd.R2words; w += ranks)
         {
             P.actPrev[qd.subOff + w] = 0u;
             P.actNext[qd.subOff + w] = 0u;
         }
-    for(uint32_t c = threadIdx.x; c <= qd.nChunks; c += kBlock) P.partA[qd.chunkOff + c] = 0u;
-    for(uint32_t i = threadIdx.x; i < 4 * (qd.nChunks + 1); i += kBlock) P.look[4 * (size_t)qd.chunkOff + i] = 0ull;
+    for(uint32_t c = rank; c <= qd.nChunks; c += ranks) P.partA[qd.chunkOff + c] = 0u;
+    for(uint32_t i = rank; i < 4 * (qd.nChunks + 1); i += ranks) P.look[4 * (size_t)qd.chunkOff + i] = 0ull;
+}
+
+// root and the first iteration's state of one query (one workgroup)
+__device__ void root_query(const QueryDesc& qd, QueryState& st, const Pools& P)
+{
     const uint32_t rootRep = qd.cfg.root_repeat < qd.cap ? qd.cfg.root_repeat : qd.cap;
     for(uint32_t k = threadIdx.x; k < rootRep; k += kBlock) P.repIdx[qd.nodeOff + k] = 0u;
     for(int d = threadIdx.x; d < qd.D; d += kBlock) P.treeS[qd.sampleOff + (size_t)d * qd.cap] = qd.start[d];
@@ -1662,20 +1668,27 @@ __global__ __launch_bounds__(kBlock) void k_init(const QueryDesc* __restrict__ q
             else
                 build_scene_wg<3>(rows + 6 * (size_t)qd.rowOff, qd.nObs, qd.nGrp, P.obs + 2 * (size_t)qd.obsOff, L);
         }
-    init_query(qd, sts[q], P);
+    clear_query(qd, P, threadIdx.x, kBlock);
+    root_query(qd, sts[q], P);
     if(q == 0 && threadIdx.x < 2) W.claim[threadIdx.x] = 0u;
     if(q == 0 && threadIdx.x == 0) *W.closedDev = 0u;
     if(last_arrival(W.counter, nq, &sflag)) build_worklists<kBlock>(sts, W, 0, sbuf);
 }
 
 // k_init of a single query, its descriptor a kernel argument that the first
-// thread stores for the kernels after it (no upload before the launch), and
-// no hand-off between workgroups (there is one)
+// thread stores for the kernels after it (no upload before the launch):
+// workgroup 0 builds the scene, the root, the state and the first work
+// lists (no hand-off between workgroups), workgroups 1.. clear the counters
 __global__ __launch_bounds__(kBlock) void k_init_one(const QueryDesc qd, QueryDesc* __restrict__ dq,
                                                      QueryState* __restrict__ sts, Pools P, WorkLists W,
                                                      const float* __restrict__ rows)
 {
     __shared__ SceneLds L;
+    if(blockIdx.x > 0)
+        {
+            clear_query(qd, P, (blockIdx.x - 1) * kBlock + threadIdx.x, (gridDim.x - 1) * kBlock);
+            return;
+        }
     if(threadIdx.x == 0) *dq = qd;
     if(rows && qd.nObs)
         {
@@ -1684,7 +1697,7 @@ __global__ __launch_bounds__(kBlock) void k_init_one(const QueryDesc qd, QueryDe
             else
                 build_scene_wg<3>(rows + 6 * (size_t)qd.rowOff, qd.nObs, qd.nGrp, P.obs + 2 * (size_t)qd.obsOff, L);
         }
-    init_query(qd, sts[0], P);
+    root_query(qd, sts[0], P);
     if(threadIdx.x < 2) W.claim[threadIdx.x] = 0u;
     // the first thread wrote the state: it builds the first work lists
     if(threadIdx.x == 0)
@@ -2069,7 +2082,10 @@ hipError_t launch_init(const QueryDesc* qds, QueryState* sts, const Pools& P, co
 hipError_t launch_init_one(const QueryDesc& qd, QueryDesc* dq, QueryState* sts, const Pools& P, const WorkLists& W,
                            const float* rows, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_init_one, dim3(1), dim3(kBlock), 0, s, qd, dq, sts, P, W, rows);
+    // about 4 K words to clear per clearing workgroup
+    const uint32_t words = 3u * (uint32_t)qd.R1 + 2u * (uint32_t)qd.R2words + 9u * (qd.nChunks + 1u);
+    const uint32_t clear = std::min(32u, std::max(1u, (words + 4095u) / 4096u));
+    hipLaunchKernelGGL(k_init_one, dim3(1 + clear), dim3(kBlock), 0, s, qd, dq, sts, P, W, rows);
     return hipGetLastError();
 }
 
