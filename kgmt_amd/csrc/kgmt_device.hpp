@@ -802,25 +802,32 @@ __device__ __forceinline__ void imul(float alo, float ahi, float blo, float bhi,
     hi = fmaxf(fmaxf(p0, p1), fmaxf(p2, p3));
 }
 
-// Ranges of sin and cos over the angles [a, b]: the endpoint values, widened
-// to +-1 when an extremum (a multiple of pi/2 of the right kind) may lie
-// inside (tested on an interval widened by 1e-3 period, so rounding can only
-// widen the result).
-__device__ __forceinline__ void sincos_range(float a, float b, float& smin, float& smax, float& cmin, float& cmax)
+// Ranges of the components of (s0, c0) rotated by every angle d in [da, db]
+// (da <= 0 <= db): (s0 cos d + c0 sin d, c0 cos d - s0 sin d), with
+// cos d in [1 - D^2 / 2, 1] (D = max(-da, db)) and sin d in [da, db]
+// (|sin d| <= |d|, with the sign of d while |d| < pi). Past D = 1 the ranges
+// are the whole [-1, 1]. Every bound is widened by 1e-6 (a few ulps of
+// rounding) and the rotated vector's length is 1 to within that, so the
+// ranges hold the sin/cos pairs that start at (s0, c0) and turn by d. A
+// non-finite s0, c0, da or db makes the ranges NaN (the clamps keep NaN).
+__device__ __forceinline__ void rot_range(float s0, float c0, float da, float db, float& slo, float& shi, float& clo,
+                                          float& chi)
 {
-    float sa, ca, sb, cb;
-    sincos_k(a, sa, ca);
-    sincos_k(b, sb, cb);
-    smin = fminf(sa, sb), smax = fmaxf(sa, sb), cmin = fminf(ca, cb), cmax = fmaxf(ca, cb);
-    const float inv2pi = 0.15915494f, pi = 3.14159265f, halfpi = 1.57079633f;
-    auto hits = [&](float off) {
-        const float lo = (a - off) * inv2pi - 1e-3f, hi = (b - off) * inv2pi + 1e-3f;
-        return floorf(hi) >= ceilf(lo);
-    };
-    if(hits(0.0f)) cmax = 1.0f;
-    if(hits(pi)) cmin = -1.0f;
-    if(hits(halfpi)) smax = 1.0f;
-    if(hits(-halfpi)) smin = -1.0f;
+    constexpr float e = 1e-6f, one = 1.0f + 1e-6f;
+    const float z     = 0.0f * (s0 + c0) + 0.0f * (da + db);  // 0, or NaN
+    const float D     = fmaxf(-da, db);
+    if(D > 1.0f)
+        {
+            slo = clo = z - one, shi = chi = z + one;
+            return;
+        }
+    const float k  = 1.0f - 0.5f * D * D;
+    const float sk = s0 * k, ck = c0 * k;
+    const float b0 = c0 * da, b1 = c0 * db, d0 = -(s0 * da), d1 = -(s0 * db);
+    auto lower = [&](float x) { return x < -one ? -one : x; };
+    auto upper = [&](float x) { return x > one ? one : x; };
+    slo = lower(z + fminf(sk, s0) + fminf(b0, b1) - e), shi = upper(z + fmaxf(sk, s0) + fmaxf(b0, b1) + e);
+    clo = lower(z + fminf(ck, c0) + fminf(d0, d1) - e), chi = upper(z + fmaxf(ck, c0) + fmaxf(d0, d1) + e);
 }
 
 // Box containing every position of a Dubins edge (statePropagator.cu:147-241)
@@ -841,12 +848,15 @@ __device__ __forceinline__ void dubins_edge_box(const QueryDesc& qd, const float
     const float v0 = FIVE_D ? u[2] : s[5];
     const float dp = pr * T, dy = yr * T, dv = a * T;
     const float angPad = 1e-5f * (1.0f + fabsf(s[3]) + fabsf(s[4]) + fabsf(dp) + fabsf(dy));
-    const float pa = s[4] + fminf(0.0f, dp) - angPad, pb = s[4] + fmaxf(0.0f, dp) + angPad;
-    const float ya = s[3] + fminf(0.0f, dy) - angPad, yb = s[3] + fmaxf(0.0f, dy) + angPad;
     const float vlo = v0 + fminf(0.0f, dv), vhi = v0 + fmaxf(0.0f, dv);
+    // the start angles' sin/cos as the integration evaluates them, turned by
+    // the edge's angle ranges
+    float sp0, cp0, sy0, cy0;
+    sincos_k(s[4], sp0, cp0);
+    sincos_k(s[3], sy0, cy0);
     float spl, sph, cpl, cph, syl, syh, cyl, cyh;
-    sincos_range(pa, pb, spl, sph, cpl, cph);
-    sincos_range(ya, yb, syl, syh, cyl, cyh);
+    rot_range(sp0, cp0, fminf(0.0f, dp) - angPad, fmaxf(0.0f, dp) + angPad, spl, sph, cpl, cph);
+    rot_range(sy0, cy0, fminf(0.0f, dy) - angPad, fmaxf(0.0f, dy) + angPad, syl, syh, cyl, cyh);
     float dl[3], dh[3], tl, th;
     imul(cpl, cph, cyl, cyh, tl, th);
     imul(vlo, vhi, tl, th, dl[0], dh[0]);
