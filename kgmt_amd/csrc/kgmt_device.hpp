@@ -1008,6 +1008,16 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
                                     m &= m - 1;
                                     lo[q] = blo[b0 + k], hi[q] = bhi[b0 + k];
                                 }
+                            // the face gaps first (VALU only, one vote for all
+                            // four): a gap says separated only where
+                            // separated() does, so four boxes that no lane's
+                            // box meets by the gaps join nothing
+                            bool near = false;
+#pragma unroll
+                            for(int q = 0; q < 4; ++q)
+                                near |= !separated_gap(dry.lo[0], dry.lo[1], dry.lo[2], dry.hi[0], dry.hi[1], dry.hi[2],
+                                                       lo[q], hi[q]);
+                            if(!__any(near)) continue;
                             bool c[4];
 #pragma unroll
                             for(int q = 0; q < 4; ++q)
