@@ -50,6 +50,7 @@ struct QueryDesc
     uint32_t obsOff, nObs, nGrp;     // first pair, boxes, groups (multiple of 4) of the scene in the pool
     uint32_t pathOff;                // float offset of the path rows ((max_iter+2)*D)
     float r1size, vol;               // W_R1_SIZE, W_R1_VOL
+    float invSpan, invR1;            // exact 1 / (ws_max - ws_min) and 1 / r1size when powers of two, else 0
     float goal[3];
     float start[KGMT_MAX_SAMPLE];
     float qminv, qax, qbx, qay, qby, qaz, qbz;  // quad: 1/m, (Iy-Iz)/Ix, 1/Ix, ...
@@ -1079,14 +1080,20 @@ __device__ __forceinline__ int region_of(const QueryDesc& qd, float x, float y, 
     const int L          = c.r1_len;
     const float span     = c.ws_max - c.ws_min;
     const float fL       = (float)L;
-    iy                   = grid_index((fL * (y - c.ws_min)) / span, L);
-    ix                   = grid_index((fL * (x - c.ws_min)) / span, L);
+    // a / span, or the same correctly rounded value as a product with the
+    // exact reciprocal when the span is a power of two (a uniform branch)
+    const float inv = qd.invSpan;
+    const bool mul  = inv != 0.0f;
+    const float ay = fL * (y - c.ws_min), ax = fL * (x - c.ws_min);
+    iy = grid_index(mul ? ay * inv : ay / span, L);
+    ix = grid_index(mul ? ax * inv : ax / span, L);
     if(qd.wdim == 2)
         {
             iz = 0;
             return ix * L + iy;
         }
-    iz = grid_index((fL * (z - c.ws_min)) / span, L);
+    const float az = fL * (z - c.ws_min);
+    iz             = grid_index(mul ? az * inv : az / span, L);
     return (ix * L + iy) * L + iz;
 }
 
@@ -1097,12 +1104,16 @@ __device__ __forceinline__ int subregion_of(const QueryDesc& qd, float x, float 
     const int L2         = c.r2_len;
     const float fL2      = (float)L2;
     const float size     = qd.r1size;
+    const float inv  = qd.invR1;  // as in region_of
+    const bool mul   = inv != 0.0f;
     const float mx = c.ws_min + (float)ix * size, my = c.ws_min + (float)iy * size;
-    const int ly = grid_index((fL2 * (y - my)) / size, L2);
-    const int lx = grid_index((fL2 * (x - mx)) / size, L2);
+    const float ay = fL2 * (y - my), ax = fL2 * (x - mx);
+    const int ly   = grid_index(mul ? ay * inv : ay / size, L2);
+    const int lx   = grid_index(mul ? ax * inv : ax / size, L2);
     if(qd.wdim == 2) return r1 * qd.P + lx * L2 + ly;
     const float mz = c.ws_min + (float)iz * size;
-    const int lz   = grid_index((fL2 * (z - mz)) / size, L2);
+    const float az = fL2 * (z - mz);
+    const int lz   = grid_index(mul ? az * inv : az / size, L2);
     return r1 * qd.P + (lx * L2 + ly) * L2 + lz;
 }
 

@@ -103,6 +103,18 @@ int check_config(const kgmt_config* c)
     return KGMT_OK;
 }
 
+// 1 / d when d is a power of two whose reciprocal is a normal float (then
+// a * (1 / d) and a / d are the same correctly rounded value for every a),
+// else 0
+float exact_recip(float d)
+{
+    if(!(d > 0.0f) || !std::isfinite(d)) return 0.0f;
+    int e;
+    const float m = std::frexp(d, &e);  // d = m 2^e, m in [0.5, 1)
+    if(m != 0.5f || e - 1 < -126 || 1 - e > 127) return 0.0f;
+    return std::ldexp(1.0f, 1 - e);
+}
+
 // descriptor fields that depend only on the configuration
 void fill_desc(QueryDesc& qd, const kgmt_config& c)
 {
@@ -122,6 +134,8 @@ void fill_desc(QueryDesc& qd, const kgmt_config& c)
     // (W_R1_VOL = W_R1_SIZE^W_DIM: an area for a planar workspace)
     qd.r1size = (c.ws_max - c.ws_min) / (float)c.r1_len;
     qd.vol    = qd.wdim == 2 ? qd.r1size * qd.r1size : qd.r1size * qd.r1size * qd.r1size;
+    qd.invSpan = exact_recip(c.ws_max - c.ws_min);
+    qd.invR1   = exact_recip(qd.r1size);
     qd.qminv  = 1.0f / c.quad_mass;
     qd.qax = (c.quad_iy - c.quad_iz) / c.quad_ix, qd.qbx = 1.0f / c.quad_ix;
     qd.qay = (c.quad_iz - c.quad_ix) / c.quad_iy, qd.qby = 1.0f / c.quad_iy;
