@@ -50,7 +50,7 @@ struct QueryDesc
     uint32_t obsOff, nObs, nGrp;     // first pair, boxes, groups (multiple of 4) of the scene in the pool
     uint32_t pathOff;                // float offset of the path rows ((max_iter+2)*D)
     float r1size, vol;               // W_R1_SIZE, W_R1_VOL
-    float invSpan, invR1;            // exact 1 / (ws_max - ws_min) and 1 / r1size when powers of two, else 0
+    float invSpan, invR1, invP;      // exact 1 / (ws_max - ws_min), 1 / r1size, 1 / P when powers of two, else 0
     float goal[3];
     float start[KGMT_MAX_SAMPLE];
     float qminv, qax, qbx, qay, qby, qaz, qbz;  // quad: 1/m, (Iy-Iz)/Ix, 1/Ix, ...

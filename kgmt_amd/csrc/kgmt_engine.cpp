@@ -136,6 +136,7 @@ void fill_desc(QueryDesc& qd, const kgmt_config& c)
     qd.vol    = qd.wdim == 2 ? qd.r1size * qd.r1size : qd.r1size * qd.r1size * qd.r1size;
     qd.invSpan = exact_recip(c.ws_max - c.ws_min);
     qd.invR1   = exact_recip(qd.r1size);
+    qd.invP    = exact_recip((float)qd.P);
     qd.qminv  = 1.0f / c.quad_mass;
     qd.qax = (c.quad_iy - c.quad_iz) / c.quad_ix, qd.qbx = 1.0f / c.quad_ix;
     qd.qay = (c.quad_iz - c.quad_ix) / c.quad_iy, qd.qby = 1.0f / c.quad_iy;

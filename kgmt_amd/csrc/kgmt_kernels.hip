@@ -136,7 +136,8 @@ __device__ __forceinline__ uint32_t region_active(const uint32_t* __restrict__ a
 __device__ __forceinline__ float region_raw_score(const QueryDesc& qd, int32_t vc, int32_t tc, uint32_t cnt)
 {
     const float eps = qd.cfg.epsilon;
-    const float cov = (float)cnt / (float)qd.P;
+    // cnt / P, as the product with 1 / P when that is exact (same bits)
+    const float cov = qd.invP != 0.0f ? (float)cnt * qd.invP : (float)cnt / (float)qd.P;
     const float vf  = (float)vc;
     const float fv  = (eps + vf) / ((eps + vf) + (float)(tc - vc)) * qd.vol;
     const float f2  = fv * fv;
