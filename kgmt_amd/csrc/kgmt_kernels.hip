@@ -499,6 +499,20 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
     const uint32_t cap = qd.cap;
     const size_t nodeO = qd.nodeOff;
     const uint32_t t   = threadIdx.x;
+    const uint32_t per = st.perN;
+    const float* T     = P.treeS + qd.sampleOff;
+    // K == 1: the parent of child t (tile order) and its state, loaded while
+    // the draws and the sort run; after the sort the wave passes them to the
+    // lanes that take the children, through its candidate area (written
+    // again only by the two-phase check)
+    static_assert(K > 1 || (2 + S) * 64 <= 8 * TileLds<NT, K>::kCandBoxes, "parent stage fits a wave's candidates");
+    uint32_t pp = 0;
+    int32_t pg  = -1;
+    float ps[S];
+#pragma unroll
+    for(int d = 0; d < S; ++d) ps[d] = 0.0f;
+    const bool pre = K == 1 && t < TC && base + t < nC;
+    if(pre) pp = P.repIdx[nodeO + (base + t) / per];
     // draws of the child in tile order
     {
         const uint32_t un = base + t;
@@ -536,6 +550,19 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
                 for(int k = 0; k < C; ++k) L.ctrl[k][t] = uc[k];
                 L.steps[t]  = steps;
                 L.ufront[t] = uf;
+                if(pre)
+                    {
+#pragma unroll
+                        for(int d = 0; d < S; ++d) ps[d] = T[(size_t)d * cap + pp];
+                        pg = P.parent[nodeO + pp];
+                    }
+                // parent, grandparent and state of child t at word j * 64 + t % 64
+                float* stg          = reinterpret_cast<float*>(L.cand[t / 64]);
+                const uint32_t l    = t % 64u;
+                stg[l]              = __uint_as_float(pp);
+                stg[64 + l]         = __int_as_float(pg);
+#pragma unroll
+                for(int d = 0; d < S; ++d) stg[(2 + d) * 64 + l] = ps[d];
                 wave_sync();
             }
         else
@@ -567,7 +594,6 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
     bool ok = false, fn = false;
     bool setBit   = false;  // valid child in a subregion unseen at iteration start
     uint32_t r2bit = 0;
-    const uint32_t per = st.perN;
     uint32_t p         = 0;
     int32_t gp         = -1;  // parent of the parent
     float s[S], uc[C];
@@ -579,16 +605,29 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
     for(int k = 0; k < C; ++k) uc[k] = 0.0f;
     if(has)
         {
-            p              = P.repIdx[nodeO + u / per];
-            const float* T = P.treeS + qd.sampleOff;
+            if constexpr(K == 1)
+                {
+                    // child i is one of this wave's 64
+                    const float* stg = reinterpret_cast<const float*>(L.cand[t / 64]);
+                    const uint32_t l = i % 64u;
+                    p                = __float_as_uint(stg[l]);
+                    gp               = __float_as_int(stg[64 + l]);
 #pragma unroll
-            for(int d = 0; d < S; ++d) s[d] = T[(size_t)d * cap + p];
-            gp = P.parent[nodeO + p];
+                    for(int d = 0; d < S; ++d) s[d] = stg[(2 + d) * 64 + l];
+                }
+            else
+                {
+                    p = P.repIdx[nodeO + u / per];
+#pragma unroll
+                    for(int d = 0; d < S; ++d) s[d] = T[(size_t)d * cap + p];
+                    gp = P.parent[nodeO + p];
+                }
 #pragma unroll
             for(int k = 0; k < C; ++k) uc[k] = L.ctrl[k][i];
             steps = L.steps[i];
             uf    = L.ufront[i];
         }
+    if constexpr(K == 1) wave_sync();  // the stage is read before the check writes candidates
     const Scene sc{sObs, (int)qd.nGrp, (int)(t % K)};
     if constexpr(K == 1 && MODEL != KGMT_MODEL_QUADCOPTER)
         ok = propagate_two_phase<MODEL>(qd, s, uc, steps, sc, L.cand[t / 64], has);  // every lane
