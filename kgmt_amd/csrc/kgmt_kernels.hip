@@ -1018,7 +1018,8 @@ __device__ __forceinline__ uint32_t block_excl_scan_min(uint32_t v, uint32_t m, 
 // query's own padded size add zeros, which leave every partial sum's bits
 // unchanged), into L.sum; with the flagged children of the iteration
 // (nAcc, acc: this thread's share of the chunk counts) into L.nAcc.
-// Barriers inside: call uniformly.
+// KB regions per thread and round. Barriers inside: call uniformly.
+template <int KB>
 __device__ void job_scores(const QueryDesc& qd, const Pools& P, uint32_t acc, UpdateLds& L)
 {
     const int R1             = qd.R1;
@@ -1027,10 +1028,9 @@ __device__ void job_scores(const QueryDesc& qd, const Pools& P, uint32_t acc, Up
     const int32_t* total     = P.total + qd.regOff;
     const int32_t* validc    = P.validc + qd.regOff;
     const uint32_t* act      = P.actNext + qd.subOff;
-    // 16 regions per thread and round (one round up to 8192 regions): every
-    // load of a round is issued before any is used (indices clamped into the
-    // query's regions, so no load sits behind a branch)
-    constexpr int KB     = 16;
+    // KB regions per thread and round: every load of a round is issued
+    // before any is used (indices clamped into the query's regions, so no
+    // load sits behind a branch)
     const bool oneWord   = Pn <= 32u && 32u % Pn == 0u;
     for(int k0 = t; k0 < R1; k0 += KB * (int)kUpd)
         {
@@ -1303,8 +1303,11 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
 #pragma unroll
                     for(uint32_t k = 0; k < kPer; ++k) u[k] = draw_react(qd.cfg, qd.cfg.query_id, st.itr, i0 + k);
                 }
-            if(own)
-                job_scores(qd, P, pAll, L);
+            // one round of loads: 8 regions per thread up to 4096 regions, 16 above
+            if(own && qd.R1 <= 8 * (int)kUpd)
+                job_scores<8>(qd, P, pAll, L);
+            else if(own)
+                job_scores<16>(qd, P, pAll, L);
             else
                 {
                     uint32_t tot;
