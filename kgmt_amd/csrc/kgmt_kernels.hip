@@ -491,7 +491,8 @@ __device__ __forceinline__ uint32_t tab_slot(int32_t* keys, int32_t key)
 // the order or on K.
 template <int MODEL, int NT, int K>
 __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState& st, uint32_t base,
-                                           const Pools& P, const float4* sObs, TileLds<NT, K>& L, uint32_t fb)
+                                           const Pools& P, float4* sObs, TileLds<NT, K>& L, uint32_t fb,
+                                           bool stage)
 {
     constexpr int S = ModelDims<MODEL>::S, C = ModelDims<MODEL>::C, D = S + C + 1;
     constexpr uint32_t TC = NT / K;
@@ -513,6 +514,13 @@ __device__ __forceinline__ void child_tile(const QueryDesc& qd, const QueryState
     for(int d = 0; d < S; ++d) ps[d] = 0.0f;
     const bool pre = K == 1 && t < TC && base + t < nC;
     if(pre) pp = P.repIdx[nodeO + (base + t) / per];
+    // the query's scene into LDS on the workgroup's first tile of it (its
+    // loads in flight with the parent index)
+    if(stage)
+        {
+            stage_scene(P.obs, qd.obsOff, qd.nGrp, sObs);
+            __syncthreads();
+        }
     // draws of the child in tile order
     {
         const uint32_t un = base + t;
@@ -736,32 +744,29 @@ __global__ __launch_bounds__(NT, 4) void k_propagate(const QueryDesc* __restrict
             const uint32_t q    = locate(tileList, nq, t);
             if(sts[q].doneN) continue;  // stale list entry of a finished query
             const QueryDesc& qd = qds[q];
-            if(q != staged)
-                {
-                    __syncthreads();
-                    stage_scene(P.obs, qd.obsOff, qd.nGrp, sObs);
-                    __syncthreads();
-                    staged = q;
-                }
+            // a new query's scene: the tile stages it (after the barrier
+            // that ends the last tile, so no wave still reads the old one)
+            const bool stage = q != staged;
+            staged           = q;
             const uint32_t base = (t - tileList[q]) * (NT / K);
             if constexpr(MODEL >= 0)
-                child_tile<MODEL, NT, K>(qd, sts[q], base, P, sObs, L, fb);
+                child_tile<MODEL, NT, K>(qd, sts[q], base, P, sObs, L, fb, stage);
             else
                 {
                     switch(qd.cfg.model)
                         {
                             case KGMT_MODEL_DOUBLE_INTEGRATOR:
-                                child_tile<KGMT_MODEL_DOUBLE_INTEGRATOR, NT, K>(qd, sts[q], base, P, sObs, L, fb);
+                                child_tile<KGMT_MODEL_DOUBLE_INTEGRATOR, NT, K>(qd, sts[q], base, P, sObs, L, fb, stage);
                                 break;
                             case KGMT_MODEL_DUBINS_AIRPLANE:
-                                child_tile<KGMT_MODEL_DUBINS_AIRPLANE, NT, K>(qd, sts[q], base, P, sObs, L, fb);
+                                child_tile<KGMT_MODEL_DUBINS_AIRPLANE, NT, K>(qd, sts[q], base, P, sObs, L, fb, stage);
                                 break;
                             case KGMT_MODEL_DUBINS_5D:
-                                child_tile<KGMT_MODEL_DUBINS_5D, NT, K>(qd, sts[q], base, P, sObs, L, fb);
+                                child_tile<KGMT_MODEL_DUBINS_5D, NT, K>(qd, sts[q], base, P, sObs, L, fb, stage);
                                 break;
                             // (a batch never mixes planar and 3D queries, so
                             // a mixed batch holds no unicycle)
-                            default: child_tile<KGMT_MODEL_QUADCOPTER, NT, K>(qd, sts[q], base, P, sObs, L, fb); break;
+                            default: child_tile<KGMT_MODEL_QUADCOPTER, NT, K>(qd, sts[q], base, P, sObs, L, fb, stage); break;
                         }
                 }
             fb ^= 1u;
