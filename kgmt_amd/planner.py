@@ -145,7 +145,20 @@ class KGMT(Planner):
         self.h_pathToGoal_ = res.goal_index if res.success else 0
         self.h_costToGoal_ = res.path_cost if res.success else 0.0
         self.h_pathRows_ = res.path_rows
-        self.h_controlPathToGoal_ = self.path() if res.success else np.zeros((0, self.D), np.float32)
+        self._pathRows = None  # h_controlPathToGoal_, read on first access
+
+    @property
+    def h_controlPathToGoal_(self) -> np.ndarray:
+        """Planner.cuh:34: the goal path's rows (goal -> root, D floats each) of
+        the last plan, read from the library on first access after it."""
+        if getattr(self, "_pathRows", None) is None:
+            ok = getattr(self, "_h", None) is not None and self.result.success
+            self._pathRows = self.path() if ok else np.zeros((0, getattr(self, "D", 0)), np.float32)
+        return self._pathRows
+
+    @h_controlPathToGoal_.setter
+    def h_controlPathToGoal_(self, rows):
+        self._pathRows = rows
 
     # KGMT::planBench, src/planners/KGMT.cu:107-165: the plan's iterations one
     # at a time, the state after each written as CSV under out_dir like
