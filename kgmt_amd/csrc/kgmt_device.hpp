@@ -841,7 +841,7 @@ __device__ __forceinline__ void rot_range(float s0, float c0, float da, float db
 // is widened by a margin that covers float rounding of the steps.
 template <bool FIVE_D>
 __device__ __forceinline__ void dubins_edge_box(const QueryDesc& qd, const float* s, const float* u, int steps,
-                                                float* lo, float* hi)
+                                                float* lo, float* hi, bool& ok)
 {
     const float h = qd.cfg.dt, T = h * (float)steps;
     const float yr = u[0], pr = u[1];
@@ -865,6 +865,10 @@ __device__ __forceinline__ void dubins_edge_box(const QueryDesc& qd, const float
     imul(vlo, vhi, tl, th, dl[1], dh[1]);
     imul(vlo, vhi, spl, sph, dl[2], dh[2]);
     const float vmax = fmaxf(fabsf(vlo), fabsf(vhi));
+    // a NaN velocity bound (non-finite angles or speed) would be dropped by
+    // fminf / fmaxf below: such an edge has no box
+    ok = (dl[0] == dl[0]) & (dh[0] == dh[0]) & (dl[1] == dl[1]) & (dh[1] == dh[1]) & (dl[2] == dl[2]) &
+         (dh[2] == dh[2]);
 #pragma unroll
     for(int d = 0; d < 3; ++d)
         {
@@ -932,8 +936,9 @@ __device__ __forceinline__ bool propagate_two_phase(const QueryDesc& qd, float* 
     if constexpr(BOUND)
         {
             // the edge's box from the bound, without a dry run
-            dubins_edge_box<MODEL == KGMT_MODEL_DUBINS_5D>(qd, s, u, steps, dry.lo, dry.hi);
-            dry.finite = __builtin_isfinite(dry.lo[0]) && __builtin_isfinite(dry.lo[1]) &&
+            bool ok;
+            dubins_edge_box<MODEL == KGMT_MODEL_DUBINS_5D>(qd, s, u, steps, dry.lo, dry.hi, ok);
+            dry.finite = ok && __builtin_isfinite(dry.lo[0]) && __builtin_isfinite(dry.lo[1]) &&
                          __builtin_isfinite(dry.lo[2]) && __builtin_isfinite(dry.hi[0]) &&
                          __builtin_isfinite(dry.hi[1]) && __builtin_isfinite(dry.hi[2]);
         }
