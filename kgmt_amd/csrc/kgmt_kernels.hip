@@ -902,15 +902,14 @@ __device__ void publish_single(const QueryDesc& qd, QueryState& st, const Pools&
 // a re-activation job, the region scores (their sum is one fixed pairwise
 // tree, so every job gets the same bits). Job 0 stores the scores and the
 // subregion snapshot that the next propagate reads.
-// Each thread of a job handles kPer consecutive slots, so one block scan
-// covers the chunk and the flag bytes come in one 4-byte load (node ranges
-// are kNodeAlign-aligned and padded, so the load stays inside the query's
-// range even past its capacity).
-constexpr uint32_t kPer = kChunk / kBlock;
-static_assert(kPer == 4, "four slots per thread");
-// update workgroups: the first kBlock threads take a job's slots, all of them
-// the region scores
+// update workgroups: kUpd threads, each taking kPer consecutive slots of a
+// job's chunk (so one block scan covers the chunk and a thread's flag bytes
+// come in one 2-byte load; node ranges are kNodeAlign-aligned and padded, so
+// the load stays inside the query's range even past its capacity), and all
+// of them the region scores
 constexpr uint32_t kUpd = 2 * kBlock;
+constexpr uint32_t kPer = kChunk / kUpd;
+static_assert(kPer == 2, "two slots per thread");
 constexpr uint32_t kLbNone        = (1u << 27) - 1;  // no goal node (node indices stay below 2^26)
 constexpr uint64_t kLbAggregate   = 1ull << 62;
 constexpr uint64_t kLbInclusive   = 2ull << 62;
@@ -1134,7 +1133,7 @@ __device__ __forceinline__ float region_score(const QueryDesc& qd, float raw, fl
 // L.raw and sum (own) or from P.score; entries: one per re-activated node
 // (node[k] = its index, rep[k] = 1)
 __device__ __forceinline__ void react_job(const QueryDesc& qd, const QueryState& st, uint32_t o, const Pools& P,
-                                          uint32_t fw, const int4 rr, const float (&u)[kPer], bool own,
+                                          uint32_t fw, const int2 rr, const float (&u)[kPer], bool own,
                                           const float* raw, float sum, float fA, uint32_t (&node)[kPer],
                                           uint32_t (&rep)[kPer])
 {
@@ -1143,8 +1142,8 @@ __device__ __forceinline__ void react_job(const QueryDesc& qd, const QueryState&
     const uint32_t i0   = o * kChunk + kPer * threadIdx.x;
 #pragma unroll
     for(uint32_t k = 0; k < kPer; ++k) node[k] = i0 + k, rep[k] = 0u;
-    if(threadIdx.x >= kBlock || i0 >= tree) return;
-    const int32_t r[kPer] = {rr.x, rr.y, rr.z, rr.w};
+    if(i0 >= tree) return;
+    const int32_t r[kPer] = {rr.x, rr.y};
     float sc[kPer];
 #pragma unroll
     for(uint32_t k = 0; k < kPer; ++k)
@@ -1166,7 +1165,7 @@ __device__ __forceinline__ void react_job(const QueryDesc& qd, const QueryState&
                     rep[k] = 1u;
                 }
         }
-    *reinterpret_cast<uint32_t*>(P.react + nodeO + i0) = rw;  // bytes past the tree read as 0
+    *reinterpret_cast<uint16_t*>(P.react + nodeO + i0) = (uint16_t)rw;  // bytes past the tree read as 0
 }
 
 // Child chunk c: the flagged children whose rank (flagged children before
@@ -1186,17 +1185,17 @@ __device__ __forceinline__ void append_job(const QueryDesc& qd, const QueryState
     const int D           = qd.D;
     const uint32_t u0     = c * kChunk + kPer * threadIdx.x;
     uint32_t fl           = 0;
-    int32_t r1[kPer]      = {0, 0, 0, 0};
-    int32_t vcount[kPer]  = {0, 0, 0, 0};
+    int32_t r1[kPer]      = {0, 0};
+    int32_t vcount[kPer]  = {0, 0};
     constexpr int kRowMax = row_words(KGMT_MAX_SAMPLE);
     const int RW          = row_words(D);
     float v[kPer][kRowMax];  // rows as stored: parent and grandparent bits, then the sample
-    if(threadIdx.x < kBlock && u0 < nC)
+    if(u0 < nC)
         {
             // flags and regions together
-            const uint32_t w = *reinterpret_cast<const uint32_t*>(P.fnext + nodeO + u0);
-            const int4 rr    = *reinterpret_cast<const int4*>(P.r1child + nodeO + u0);
-            r1[0] = rr.x, r1[1] = rr.y, r1[2] = rr.z, r1[3] = rr.w;
+            const uint32_t w = *reinterpret_cast<const uint16_t*>(P.fnext + nodeO + u0);
+            const int2 rr    = *reinterpret_cast<const int2*>(P.r1child + nodeO + u0);
+            r1[0] = rr.x, r1[1] = rr.y;
 #pragma unroll
             for(uint32_t k = 0; k < kPer; ++k)
                 if(u0 + k < nC && ((w >> (8 * k)) & 0xffu)) fl |= 1u << k;
@@ -1204,9 +1203,10 @@ __device__ __forceinline__ void append_job(const QueryDesc& qd, const QueryState
     // Before the rank scan: the regions' valid counts for the repeat count
     // and the rows of the flagged children (appended unless the tree fills
     // up). A flagged child's row sits at its 32-child segment's start plus
-    // the flagged children before it in the segment (eight threads of four).
+    // the flagged children before it in the segment (a row of sixteen
+    // threads of two).
     const uint32_t nf = (uint32_t)__popc(fl);
-    uint32_t slot     = (u0 & ~31u) + group8_incl_scan(nf) - nf;
+    uint32_t slot     = (u0 & ~31u) + row_incl_scan(nf) - nf;
 #pragma unroll
     for(uint32_t k = 0; k < kPer; ++k)
         if((fl >> k) & 1u)
@@ -1295,19 +1295,16 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
             // the job's nodes first, then the totals (their loads overlap)
             const uint32_t i0 = li * kChunk + kPer * threadIdx.x;
             uint32_t fw       = 0;
-            int4 rr           = make_int4(0, 0, 0, 0);
-            if(threadIdx.x < kBlock && i0 < tree)
+            int2 rr           = make_int2(0, 0);
+            if(i0 < tree)
                 {
-                    fw = *reinterpret_cast<const uint32_t*>(P.frontier + qd.nodeOff + i0);
-                    rr = *reinterpret_cast<const int4*>(P.r1node + qd.nodeOff + i0);
+                    fw = *reinterpret_cast<const uint16_t*>(P.frontier + qd.nodeOff + i0);
+                    rr = *reinterpret_cast<const int2*>(P.r1node + qd.nodeOff + i0);
                 }
             // the draws need neither loads nor scores: computed while they come
-            float u[kPer] = {1.0f, 1.0f, 1.0f, 1.0f};
-            if(threadIdx.x < kBlock)
-                {
+            float u[kPer];
 #pragma unroll
-                    for(uint32_t k = 0; k < kPer; ++k) u[k] = draw_react(qd.cfg, qd.cfg.query_id, st.itr, i0 + k);
-                }
+            for(uint32_t k = 0; k < kPer; ++k) u[k] = draw_react(qd.cfg, qd.cfg.query_id, st.itr, i0 + k);
             // one round of loads: 8 regions per thread up to 4096 regions, 16 above
             if(own && qd.R1 <= 8 * (int)kUpd)
                 job_scores<8>(qd, P, pAll, L);
@@ -1327,7 +1324,7 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
     else
         append_job(qd, st, c, P, pAll, pBefore, L.sbuf, node, rep, goal, nAcc);
     uint32_t total, jobGoal;
-    const uint32_t ex = block_excl_scan_min<kUpd>(rep[0] + rep[1] + rep[2] + rep[3], goal, L.sbuf, total, jobGoal);
+    const uint32_t ex = block_excl_scan_min<kUpd>(rep[0] + rep[1], goal, L.sbuf, total, jobGoal);
     if(threadIdx.x == 0)
         {
             __hip_atomic_store(words + li, lb_word(li == 0 ? kLbInclusive : kLbAggregate, total, jobGoal),
