@@ -1187,6 +1187,17 @@ __device__ __forceinline__ uint32_t group8_incl_scan(uint32_t v)
     return v + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)tq, 0x114, 0xF, 0xA, false);
 }
 
+// inclusive scan of v over each half wave (lanes 0-31, 32-63): row scans,
+// then the first row's total added to the second row of each half
+__device__ __forceinline__ uint32_t half_incl_scan(uint32_t v)
+{
+    v                   = row_incl_scan(v);
+    const uint32_t r15  = (uint32_t)__builtin_amdgcn_readlane((int)v, 15);
+    const uint32_t r47  = (uint32_t)__builtin_amdgcn_readlane((int)v, 47);
+    const uint32_t row  = lane_id() >> 4;
+    return v + (row == 1u ? r15 : row == 3u ? r47 : 0u);
+}
+
 // inclusive scan of v over the 64 lanes of a wave: row scans, then the row
 // totals (lanes 15, 31, 47) added to the rows after them
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)

@@ -903,13 +903,11 @@ __device__ void publish_single(const QueryDesc& qd, QueryState& st, const Pools&
 // tree, so every job gets the same bits). Job 0 stores the scores and the
 // subregion snapshot that the next propagate reads.
 // update workgroups: kUpd threads, each taking kPer consecutive slots of a
-// job's chunk (so one block scan covers the chunk and a thread's flag bytes
-// come in one 2-byte load; node ranges are kNodeAlign-aligned and padded, so
-// the load stays inside the query's range even past its capacity), and all
-// of them the region scores
-constexpr uint32_t kUpd = 2 * kBlock;
+// job's chunk (one slot: one block scan covers the chunk, and a thread's
+// flag byte is one load), and all of them the region scores
+constexpr uint32_t kUpd = 4 * kBlock;
 constexpr uint32_t kPer = kChunk / kUpd;
-static_assert(kPer == 2, "two slots per thread");
+static_assert(kPer == 1, "one slot per thread");
 constexpr uint32_t kLbNone        = (1u << 27) - 1;  // no goal node (node indices stay below 2^26)
 constexpr uint64_t kLbAggregate   = 1ull << 62;
 constexpr uint64_t kLbInclusive   = 2ull << 62;
@@ -1133,7 +1131,7 @@ __device__ __forceinline__ float region_score(const QueryDesc& qd, float raw, fl
 // L.raw and sum (own) or from P.score; entries: one per re-activated node
 // (node[k] = its index, rep[k] = 1)
 __device__ __forceinline__ void react_job(const QueryDesc& qd, const QueryState& st, uint32_t o, const Pools& P,
-                                          uint32_t fw, const int2 rr, const float (&u)[kPer], bool own,
+                                          uint32_t fw, const int32_t rr, const float (&u)[kPer], bool own,
                                           const float* raw, float sum, float fA, uint32_t (&node)[kPer],
                                           uint32_t (&rep)[kPer])
 {
@@ -1143,7 +1141,7 @@ __device__ __forceinline__ void react_job(const QueryDesc& qd, const QueryState&
 #pragma unroll
     for(uint32_t k = 0; k < kPer; ++k) node[k] = i0 + k, rep[k] = 0u;
     if(i0 >= tree) return;
-    const int32_t r[kPer] = {rr.x, rr.y};
+    const int32_t r[kPer] = {rr};
     float sc[kPer];
 #pragma unroll
     for(uint32_t k = 0; k < kPer; ++k)
@@ -1165,7 +1163,7 @@ __device__ __forceinline__ void react_job(const QueryDesc& qd, const QueryState&
                     rep[k] = 1u;
                 }
         }
-    *reinterpret_cast<uint16_t*>(P.react + nodeO + i0) = (uint16_t)rw;  // bytes past the tree read as 0
+    P.react[nodeO + i0] = (uint8_t)rw;
 }
 
 // Child chunk c: the flagged children whose rank (flagged children before
@@ -1185,17 +1183,16 @@ __device__ __forceinline__ void append_job(const QueryDesc& qd, const QueryState
     const int D           = qd.D;
     const uint32_t u0     = c * kChunk + kPer * threadIdx.x;
     uint32_t fl           = 0;
-    int32_t r1[kPer]      = {0, 0};
-    int32_t vcount[kPer]  = {0, 0};
+    int32_t r1[kPer]      = {0};
+    int32_t vcount[kPer]  = {0};
     constexpr int kRowMax = row_words(KGMT_MAX_SAMPLE);
     const int RW          = row_words(D);
     float v[kPer][kRowMax];  // rows as stored: parent and grandparent bits, then the sample
     if(u0 < nC)
         {
             // flags and regions together
-            const uint32_t w = *reinterpret_cast<const uint16_t*>(P.fnext + nodeO + u0);
-            const int2 rr    = *reinterpret_cast<const int2*>(P.r1child + nodeO + u0);
-            r1[0] = rr.x, r1[1] = rr.y;
+            const uint32_t w = P.fnext[nodeO + u0];
+            r1[0]            = P.r1child[nodeO + u0];
 #pragma unroll
             for(uint32_t k = 0; k < kPer; ++k)
                 if(u0 + k < nC && ((w >> (8 * k)) & 0xffu)) fl |= 1u << k;
@@ -1203,10 +1200,10 @@ __device__ __forceinline__ void append_job(const QueryDesc& qd, const QueryState
     // Before the rank scan: the regions' valid counts for the repeat count
     // and the rows of the flagged children (appended unless the tree fills
     // up). A flagged child's row sits at its 32-child segment's start plus
-    // the flagged children before it in the segment (a row of sixteen
-    // threads of two).
+    // the flagged children before it in the segment (the 32 threads of half
+    // a wave).
     const uint32_t nf = (uint32_t)__popc(fl);
-    uint32_t slot     = (u0 & ~31u) + row_incl_scan(nf) - nf;
+    uint32_t slot     = (u0 & ~31u) + half_incl_scan(nf) - nf;
 #pragma unroll
     for(uint32_t k = 0; k < kPer; ++k)
         if((fl >> k) & 1u)
@@ -1295,21 +1292,21 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
             // the job's nodes first, then the totals (their loads overlap)
             const uint32_t i0 = li * kChunk + kPer * threadIdx.x;
             uint32_t fw       = 0;
-            int2 rr           = make_int2(0, 0);
+            int32_t rr        = 0;
             if(i0 < tree)
                 {
-                    fw = *reinterpret_cast<const uint16_t*>(P.frontier + qd.nodeOff + i0);
-                    rr = *reinterpret_cast<const int2*>(P.r1node + qd.nodeOff + i0);
+                    fw = P.frontier[qd.nodeOff + i0];
+                    rr = P.r1node[qd.nodeOff + i0];
                 }
             // the draws need neither loads nor scores: computed while they come
             float u[kPer];
 #pragma unroll
             for(uint32_t k = 0; k < kPer; ++k) u[k] = draw_react(qd.cfg, qd.cfg.query_id, st.itr, i0 + k);
-            // one round of loads: 8 regions per thread up to 4096 regions, 16 above
-            if(own && qd.R1 <= 8 * (int)kUpd)
-                job_scores<8>(qd, P, pAll, L);
+            // one round of loads: 4 regions per thread up to 4096 regions, 8 above
+            if(own && qd.R1 <= 4 * (int)kUpd)
+                job_scores<4>(qd, P, pAll, L);
             else if(own)
-                job_scores<16>(qd, P, pAll, L);
+                job_scores<8>(qd, P, pAll, L);
             else
                 {
                     uint32_t tot;
@@ -1324,7 +1321,7 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
     else
         append_job(qd, st, c, P, pAll, pBefore, L.sbuf, node, rep, goal, nAcc);
     uint32_t total, jobGoal;
-    const uint32_t ex = block_excl_scan_min<kUpd>(rep[0] + rep[1], goal, L.sbuf, total, jobGoal);
+    const uint32_t ex = block_excl_scan_min<kUpd>(rep[0], goal, L.sbuf, total, jobGoal);
     if(threadIdx.x == 0)
         {
             __hip_atomic_store(words + li, lb_word(li == 0 ? kLbInclusive : kLbAggregate, total, jobGoal),
