@@ -945,25 +945,44 @@ __device__ __forceinline__ uint32_t wave_min_u(uint32_t v)
 }
 
 // Entries and smallest goal node of the jobs before job li (words: the
-// query's job words); every lane of one wave. False if a word did not appear
-// within the poll bound (the sums then cover the words seen).
+// query's job words); every lane of one wave. A round reads the 256 words
+// before `base` (lane l, slot k: word base - 1 - l - 64 k), all loads in
+// flight together, so the jobs of one launch (every aggregate published at
+// about the same time) find their prefix in one or two round trips. False if
+// a word did not appear within the poll bound (the sums then cover the words
+// seen).
 __device__ bool look_back(const uint64_t* words, uint32_t li, uint32_t& entries, uint32_t& goal)
 {
+    constexpr int KL   = 4;
     const int32_t lane = (int32_t)lane_id();
     uint32_t c = 0, g = kLbNone, polls = 0;
-    int32_t base = (int32_t)li;  // this round reads words base-1 (lane 0) down to base-64
+    int32_t base = (int32_t)li;
     while(true)
         {
-            const int32_t idx = base - 1 - lane;
-            // before the first job: an inclusive word of nothing
-            const uint64_t w = idx >= 0 ? __hip_atomic_load(words + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                        : lb_word(kLbInclusive, 0u, kLbNone);
-            const uint32_t flag = (uint32_t)(w >> 62);
-            const uint64_t incl = __ballot(flag == 2u);
-            const uint64_t none = __ballot(flag == 0u);
-            // lanes up to and including the nearest inclusive word
-            const uint64_t need = incl ? ((incl & (~incl + 1ull)) << 1) - 1ull : ~0ull;
-            if(none & need)
+            uint64_t w[KL];
+#pragma unroll
+            for(int k = 0; k < KL; ++k)
+                {
+                    const int32_t idx = base - 1 - lane - 64 * k;
+                    // before the first job: an inclusive word of nothing
+                    w[k] = idx >= 0 ? __hip_atomic_load(words + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                    : lb_word(kLbInclusive, 0u, kLbNone);
+                }
+            // per slot, the lanes up to and including the nearest inclusive
+            // word (none past the slot that holds it)
+            uint64_t need[KL];
+            bool found = false, wait = false;
+#pragma unroll
+            for(int k = 0; k < KL; ++k)
+                {
+                    const uint32_t flag = (uint32_t)(w[k] >> 62);
+                    const uint64_t incl = __ballot(flag == 2u);
+                    const uint64_t none = __ballot(flag == 0u);
+                    need[k] = found ? 0ull : incl ? ((incl & (~incl + 1ull)) << 1) - 1ull : ~0ull;
+                    wait    = wait || (none & need[k]) != 0ull;
+                    found   = found || incl != 0ull;
+                }
+            if(wait)
                 {
                     if(++polls >= kLookBackPolls)
                         {
@@ -973,18 +992,24 @@ __device__ bool look_back(const uint64_t* words, uint32_t li, uint32_t& entries,
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-            const bool use = (need >> lane) & 1ull;
-            const uint32_t wc = use ? (uint32_t)w : 0u;
-            const uint32_t wg = use ? (uint32_t)(w >> 32) & kLbNone : kLbNone;
+            uint32_t wc = 0u, wg = kLbNone;
+#pragma unroll
+            for(int k = 0; k < KL; ++k)
+                if((need[k] >> lane) & 1ull)
+                    {
+                        wc += (uint32_t)w[k];
+                        const uint32_t gk = (uint32_t)(w[k] >> 32) & kLbNone;
+                        wg                = gk < wg ? gk : wg;
+                    }
             c += (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(wc), 63);
             const uint32_t gm = wave_min_u(wg);
             g = gm < g ? gm : g;
-            if(incl)
+            if(found)
                 {
                     entries = c, goal = g;
                     return true;
                 }
-            base -= 64;
+            base -= 64 * KL;
         }
 }
 
