@@ -462,9 +462,12 @@ struct LaneCandCheck
     }
     __device__ __forceinline__ bool operator()(float ax, float ay, float az, float bx, float by, float bz) const
     {
-        const float mnx = ax > bx ? bx : ax, mxx = ax > bx ? ax : bx;
-        const float mny = ay > by ? by : ay, mxy = ay > by ? ay : by;
-        const float mnz = az > bz ? bz : az, mxz = az > bz ? az : bz;
+        // the positions are finite here, so fminf / fmaxf give the
+        // reference's box (up to the sign of a zero, which no face gap's
+        // comparison with 0 sees)
+        const float mnx = fminf(ax, bx), mxx = fmaxf(ax, bx);
+        const float mny = fminf(ay, by), mxy = fmaxf(ay, by);
+        const float mnz = fminf(az, bz), mxz = fmaxf(az, bz);
         bool hit = false;
 #pragma unroll
         for(int k = 0; k < R; ++k)
@@ -655,13 +658,18 @@ __device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const
     sincos_k(h * pr, pE.s, pE.c);
     sincos_k(hh * yr, yH.s, yH.c);
     sincos_k(h * yr, yE.s, yE.c);
+    // FIVE_D: the acceleration is 0, so fma(hh, 0, v), fma(h, 0, v) and the
+    // speed after every step are v + 0 (dt is positive and finite: the
+    // products are +0), the same bits computed once; only the first step's
+    // v * cos * cos terms see v itself
+    const float vn = v + 0.0f;
     for(int i = 0; i < steps; ++i)
         {
             const float px = x, py = y, pz = z;
             // RK4 with fused multiply-adds at fixed places (the oracle's
             // prop_dubins evaluates the same operations)
-            const float vm = __builtin_fmaf(hh, a, v);
-            const float ve = __builtin_fmaf(h, a, v);
+            const float vm = FIVE_D ? vn : __builtin_fmaf(hh, a, v);
+            const float ve = FIVE_D ? vn : __builtin_fmaf(h, a, v);
             float spm, cpm, sym, cym, spe, cpe, sye, cye;
             rotate(sp0, cp0, pH, spm, cpm);
             rotate(sy0, cy0, yH, sym, cym);
@@ -676,7 +684,7 @@ __device__ __forceinline__ bool prop_dubins(const QueryDesc& qd, float* s, const
             z = __builtin_fmaf(h6, __builtin_fmaf(ve, spe, __builtin_fmaf(2.0f, mz + mz, v * sp0)), z);
             yaw   = __builtin_fmaf(h, yr, yaw);
             pitch = __builtin_fmaf(h, pr, pitch);
-            v     = __builtin_fmaf(h6, __builtin_fmaf(2.0f, a + a, a) + a, v);
+            v     = FIVE_D ? vn : __builtin_fmaf(h6, __builtin_fmaf(2.0f, a + a, a) + a, v);
             sp0 = spe, cp0 = cpe, sy0 = sye, cy0 = cye;
             // the reference's checks in its order (speed, pitch, workspace,
             // segment) all end the edge as invalid at this step, so they are
