@@ -1180,21 +1180,6 @@ __device__ __forceinline__ uint32_t row_incl_scan(uint32_t v)
     return v;
 }
 
-// inclusive scan of v over each aligned group of 8 lanes: within quads
-// (quad_perm [0,0,1,2] then [0,0,0,1], lanes taking no term add 0), then
-// each quad's total (quad_perm [3,3,3,3]) added to the next quad of the group
-// (row_shr 4 into banks 1 and 3 only)
-__device__ __forceinline__ uint32_t group8_incl_scan(uint32_t v)
-{
-    const uint32_t q = lane_id() & 3u;
-    uint32_t t       = dpp_u<0x90>(v);
-    v += q >= 1u ? t : 0u;
-    t = dpp_u<0x40>(v);
-    v += q >= 2u ? t : 0u;
-    const uint32_t tq = dpp_u<0xFF>(v);
-    return v + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)tq, 0x114, 0xF, 0xA, false);
-}
-
 // inclusive scan of v over each half wave (lanes 0-31, 32-63): row scans,
 // then the first row's total added to the second row of each half
 __device__ __forceinline__ uint32_t half_incl_scan(uint32_t v)
