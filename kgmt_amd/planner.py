@@ -80,6 +80,10 @@ class KGMT(Planner):
         self._h = h
         self.result = KgmtResult()
         self.h_pathRows_ = 0
+        # start / goal staging: kgmt_plan copies them before it returns, so
+        # one pair of buffers (addresses taken once) serves every plan
+        self._sg = np.zeros((2, self.D), np.float32)
+        self._sg_ptr = (C.c_void_p(self._sg.ctypes.data), C.c_void_p(self._sg.ctypes.data + 4 * self.D))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -125,13 +129,21 @@ class KGMT(Planner):
                                               int(propagate_grid), int(chunk_grid)))
 
     # KGMT::plan, src/planners/KGMT.cu:41-105
+    def _stage(self, row, v):
+        a = np.asarray(v, dtype=np.float32).ravel()
+        if a.size >= self.D:
+            self._sg[row] = a[: self.D]
+        else:
+            self._sg[row] = 0.0
+            self._sg[row, : a.size] = a
+
     def plan(self, h_initial, h_goal, d_obstacles_ptr, h_obstaclesCount) -> KgmtResult:
-        start = _f32(h_initial, self.D)
-        goal = _f32(h_goal, self.D)
+        self._stage(0, h_initial)
+        self._stage(1, h_goal)
         n = int(h_obstaclesCount)
         p, on_dev, keep = _obstacle_arg(d_obstacles_ptr, n, wdim(self.cfg.model))
         res = KgmtResult()
-        _lib.check(_lib.lib().kgmt_plan(self._h, _ptr(start), _ptr(goal), p, n, on_dev, C.byref(res)))
+        _lib.check(_lib.lib().kgmt_plan(self._h, self._sg_ptr[0], self._sg_ptr[1], p, n, on_dev, C.byref(res)))
         del keep
         self.result = res
         self._publish()
