@@ -156,11 +156,14 @@ int dev_alloc(T*& p, size_t count, std::vector<void*>& owned)
 }
 
 constexpr size_t kHostPathFloats = size_t(1) << 20;  // host-mapped path rows of a single query (4 MiB)
-// launch units the host stays ahead of the GPU: one (every launch is queued
-// while the one before runs, and a finished plan leaves one unit behind);
-// two while events time the launches (a deeper queue keeps the event pairs
-// back to back, so they time kernels rather than host gaps)
-constexpr uint32_t kLag = 1, kLagProfiled = 2;
+// launch units the host stays ahead of the GPU: one for a plan's first
+// kLongAfter iterations (every launch is queued while the one before runs,
+// and a plan that finishes leaves one unit behind), two after them (a plan
+// that runs long is most often one that runs to the iteration limit: the
+// deeper queue absorbs the host's own delays); two while events time the
+// launches (a deeper queue keeps the event pairs back to back, so they time
+// kernels rather than host gaps)
+constexpr uint32_t kLag = 1, kLagLong = 2, kLongAfter = 16, kLagProfiled = 2;
 
 }  // namespace
 
@@ -572,7 +575,7 @@ struct Engine
                 it += perUnit;
                 // wait while the device is more than kLag units behind (after
                 // the last unit: until it has closed every iteration)
-                const uint32_t ahead = (prof ? kLagProfiled : kLag) * perUnit;
+                const uint32_t ahead = (prof ? kLagProfiled : it > kLongAfter ? kLagLong : kLag) * perUnit;
                 const uint32_t need  = it < maxIter ? (it > ahead ? it - ahead : 0u) : maxIter;
                 while(true)
                     {
