@@ -114,6 +114,7 @@ struct Pools
                          // dense per 32-child segment
     int32_t* parent;     // tree parents
     int32_t* grand;      // tree grandparents (-1 for the root and its children)
+    int4* anc;           // tree ancestors at distances 3, 4, 5, 6 (-1 past the root)
     float* cost;         // distance of node position to goal
     uint8_t* frontier;   // frontier flag per tree node
     int32_t* r1node;     // region of each tree node

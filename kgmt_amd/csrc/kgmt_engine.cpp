@@ -313,6 +313,7 @@ struct Engine
         if((rc = dev_alloc(dq, n, owned)) || (rc = dev_alloc(ds, n, owned)) ||
            (rc = dev_alloc(P.treeS, samples, owned)) || (rc = dev_alloc(P.childR, rows, owned)) ||
            (rc = dev_alloc(P.parent, nodes, owned)) || (rc = dev_alloc(P.grand, nodes, owned)) || (rc = dev_alloc(P.cost, nodes, owned)) ||
+           (rc = dev_alloc(P.anc, nodes, owned)) ||
            (rc = dev_alloc(P.frontier, nodes, owned)) || (rc = dev_alloc(P.r1node, nodes, owned)) ||
            (rc = dev_alloc(P.r1child, nodes, owned)) ||
            (rc = dev_alloc(P.fnext, nodes, owned)) || (rc = dev_alloc(P.react, nodes, owned)) ||
@@ -1482,6 +1483,7 @@ int kgmt_frontier_update(const kgmt_config* cfg, uint32_t itr, uint32_t tree_siz
     if((rc = a.get(dq, 1)) || (rc = a.get(ds, 1)) || (rc = a.get(info, 4)) ||
        (rc = a.get(P.treeS, (size_t)h.D * cap)) || (rc = a.get(P.childR, (size_t)row_words(h.D) * cap)) ||
        (rc = a.get(P.parent, nodes)) || (rc = a.get(P.grand, nodes)) || (rc = a.get(P.cost, nodes)) ||
+       (rc = a.get(P.anc, nodes)) ||
        (rc = a.get(P.frontier, nodes)) || (rc = a.get(P.r1node, nodes)) || (rc = a.get(P.r1child, nodes)) ||
        (rc = a.get(P.fnext, nodes)) || (rc = a.get(P.react, nodes)) || (rc = a.get(P.repIdx, nodes)) ||
        (rc = a.get(P.partA, h.nChunks + 1)) || (rc = a.get(P.look, 4 * ((size_t)h.nChunks + 1))) ||

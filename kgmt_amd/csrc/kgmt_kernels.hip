@@ -395,6 +395,7 @@ __device__ void root_query(const QueryDesc& qd, QueryState& st, const Pools& P)
         {
             P.parent[qd.nodeOff]   = -1;
             P.grand[qd.nodeOff]    = -1;
+            P.anc[qd.nodeOff]      = make_int4(-1, -1, -1, -1);
             P.cost[qd.nodeOff]     = 0.0f;
             P.frontier[qd.nodeOff] = 1;
             int ix, iy, iz;
@@ -837,6 +838,87 @@ __device__ __forceinline__ void single_worklists(const QueryState& st, const Wor
 
 __device__ void walk_path(const QueryDesc& qd, QueryState& st, const Pools& P, float* __restrict__ out, int32_t goal);
 
+// goal -> root rows of a solved single query by one wave (every lane): the
+// chain is followed six levels per round of loads (parent, grandparent and
+// the ancestors at 3..6), lane l takes the row of the l-th node, and the
+// rows go out together; cost and length are summed in walk order, so they
+// have the bits of walk_path's. Paths of more than 64 rows take walk_path.
+__device__ void walk_path_wave(const QueryDesc& qd, QueryState& st, const Pools& P, float* __restrict__ out,
+                               int32_t goal)
+{
+    const uint32_t lane    = lane_id();
+    const uint32_t cap     = qd.cap;
+    const int D            = qd.D;
+    const uint32_t maxRows = qd.cfg.max_iter + 2;
+    const uint32_t lim     = maxRows < 64u ? maxRows : 64u;
+    const int32_t* par     = P.parent + qd.nodeOff;
+    const int32_t* gpar    = P.grand + qd.nodeOff;
+    const int4* anc        = P.anc + qd.nodeOff;
+    int32_t mine = lane == 0 ? goal : -1;  // this lane's node
+    int32_t f    = goal;                    // the last node found
+    uint32_t cnt = 1;                       // nodes found
+    bool root    = false;                   // the chain ended at the root
+    while(!root && cnt < lim)
+        {
+            const int32_t p = par[f], g = gpar[f];
+            const int4 a    = anc[f];
+            const int32_t up[6] = {p, g, a.x, a.y, a.z, a.w};
+#pragma unroll
+            for(int j = 0; j < 6; ++j)
+                if(!root && cnt < lim)
+                    {
+                        if(up[j] < 0 || (uint32_t)up[j] >= cap)  // -1: past the root
+                            root = true;
+                        else
+                            {
+                                if(lane == cnt) mine = up[j];
+                                f = up[j];
+                                ++cnt;
+                            }
+                    }
+        }
+    // stopped at the plan's bound of rows: the last row's parent decides
+    // whether its duration counts
+    if(!root && cnt >= maxRows) root = par[f] == -1;
+    if(!root && cnt < maxRows)
+        {
+            // longer than one wave's rows
+            if(lane == 0) walk_path(qd, st, P, out, goal);
+            return;
+        }
+    const float* __restrict__ T = P.treeS + qd.sampleOff;
+    float r[KGMT_MAX_SAMPLE] = {};
+    float dn = 0.0f;
+    if(lane < cnt)
+        {
+#pragma unroll
+            for(int k = 0; k < KGMT_MAX_SAMPLE; ++k)
+                if(k < D) r[k] = T[(size_t)k * cap + mine];
+            dn = T[(size_t)(D - 1) * cap + mine];
+#pragma unroll
+            for(int k = 0; k < KGMT_MAX_SAMPLE; ++k)
+                if(k < D) out[lane * D + k] = r[k];
+        }
+    // segment length from the row before (lane l - 1) to this one
+    const float qx = __shfl_up(r[0], 1, 64), qy = __shfl_up(r[1], 1, 64), qz = __shfl_up(r[2], 1, 64);
+    const float e  = lane > 0 && lane < cnt ? pos_dist(qd, qx, qy, qz, r[0], r[1], r[2]) : 0.0f;
+    // a node's edge duration counts when it has a parent: every node but the
+    // root
+    float pc = 0.0f, pl = 0.0f;
+    for(uint32_t l = 0; l < cnt; ++l)
+        {
+            const float dl = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dn), l));
+            if(!(root && l == cnt - 1)) pc = pc + dl;
+            if(l > 0) pl = pl + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e), l));
+        }
+    if(lane == 0)
+        {
+            st.pathRows = cnt;
+            st.pathCost = pc;
+            st.pathLen  = pl;
+        }
+}
+
 // How a single-query plan ended.
 struct Outcome
 {
@@ -1200,7 +1282,7 @@ __device__ __forceinline__ void react_job(const QueryDesc& qd, const QueryState&
 // chunk counts of all chunks / of the chunks before c.
 __device__ __forceinline__ void append_job(const QueryDesc& qd, const QueryState& st, uint32_t c, const Pools& P,
                                            uint32_t pAll, uint32_t pBefore, uint32_t* sbuf, uint32_t (&node)[kPer],
-                                           uint32_t (&rep)[kPer], uint32_t& goal, uint32_t& nAcc)
+                                           uint32_t (&rep)[kPer], uint32_t& goal, uint32_t& nAcc, int32_t& apar)
 {
     const uint32_t cap = qd.cap, nC = st.nChildren, tree = st.treeSize;
     const size_t nodeO    = qd.nodeOff;
@@ -1253,6 +1335,7 @@ __device__ __forceinline__ void append_job(const QueryDesc& qd, const QueryState
     const uint32_t next = nAcc < cap - tree ? nAcc : cap - tree;
     uint32_t rank       = tot3[2] + ex3[0];
     goal                = kLbNone;
+    apar                = -1;
     uint32_t t          = tree + rank;
 #pragma unroll
     for(uint32_t k = 0; k < kPer; ++k)
@@ -1268,6 +1351,7 @@ __device__ __forceinline__ void append_job(const QueryDesc& qd, const QueryState
                                 if(d < D) T[(size_t)d * cap + t] = v[k][2 + d];
                             P.parent[nodeO + t]   = __float_as_int(v[k][0]);
                             P.grand[nodeO + t]    = __float_as_int(v[k][1]);
+                            apar                  = __float_as_int(v[k][0]);
                             const float dg        = goal_dist(qd, v[k][2], v[k][3], v[k][4]);
                             P.cost[nodeO + t]     = dg;
                             P.frontier[nodeO + t] = 1;
@@ -1312,6 +1396,7 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
             pBefore += i < c ? x : 0u;
         }
     uint32_t node[kPer], rep[kPer], goal = kLbNone, nAcc;
+    int32_t apar = -1;  // parent of the node this thread appends
     if(li < nOC)
         {
             // the job's nodes first, then the totals (their loads overlap)
@@ -1344,7 +1429,7 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
             react_job(qd, st, li, P, fw, rr, u, own, L.raw, L.sum, faccept(qd, st.itr, tree, next), node, rep);
         }
     else
-        append_job(qd, st, c, P, pAll, pBefore, L.sbuf, node, rep, goal, nAcc);
+        append_job(qd, st, c, P, pAll, pBefore, L.sbuf, node, rep, goal, nAcc, apar);
     uint32_t total, jobGoal;
     const uint32_t ex = block_excl_scan_min<kUpd>(rep[0], goal, L.sbuf, total, jobGoal);
     if(threadIdx.x == 0)
@@ -1355,6 +1440,18 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
         }
     // the aggregate goes out before the look-back's loads
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    // the appended node's ancestors at distances 3..6 are its parent's at
+    // 2..5, for the path walk (no job of this launch reads them): written
+    // while the look-back runs, by wave 0 after it
+    auto link = [&]() {
+        if(apar >= 0 && (uint32_t)apar < cap)
+            {
+                const int32_t g = P.grand[qd.nodeOff + apar];
+                const int4 a    = P.anc[qd.nodeOff + apar];
+                P.anc[qd.nodeOff + node[0]] = make_int4(g, a.x, a.y, a.z);
+            }
+    };
+    if(threadIdx.x >= 64) link();
     // the job's entries into LDS while the look-back runs (written out below
     // as whole rows; a job with more entries than the stage holds, and job 0,
     // whose region scores stay in L.raw, write them directly)
@@ -1381,6 +1478,7 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
                     L.before = pc, L.goalBefore = pg;
                 }
         }
+    if(threadIdx.x < 64) link();
     if(staged && threadIdx.x < 64)
         {
             uint32_t p = ex;
@@ -1423,6 +1521,11 @@ __device__ void update_job(const QueryDesc* __restrict__ qds, QueryState* __rest
         }
     const uint32_t next = nAcc < cap - tree ? nAcc : cap - tree;
     const bool last     = li == nJ - 1;
+    // the first goal of a single query's iteration: wave 0 walks its path
+    // into host memory before its first lane publishes the result
+    if(nq == 1 && jobGoal != kLbNone && goalBefore == kLbNone && W.result != nullptr && threadIdx.x < 64 &&
+       st.pathRows == 0u)
+        walk_path_wave(qd, st, P, W.hostPath, (int32_t)jobGoal);
     // The last job's look-back has seen a word of every other job, each
     // published after that job read the chunk counts: they start at zero
     // again for the next propagate.
