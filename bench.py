@@ -49,7 +49,7 @@ BOXES = {"dubins5d": 100, "di6d": 20, "quad12d": 56}
 DEFAULT_SEEDS = {"dubins5d": 200, "di6d": 50, "quad12d": 200}
 
 
-def survey_bytes_per_child(model: int) -> int:
+def algorithmic_bytes_per_child(model: int) -> int:
     """Algorithmic HBM bytes of one expanded child, SoA FP32 with
     pre-generated controls: parent index 4, parent state 4S, controls 4C and
     u16 steps, child state+controls+duration 4(S+C+1), child parent index 4,
@@ -239,7 +239,7 @@ def run_batch(n, seed=0, tune=True, launch=None, sequential=True, profile=True):
         if [(r.tree_size, r.iterations) for r in res_e] != [(r.tree_size, r.iterations) for r in res_b]:
             raise RuntimeError("the profiled batch differs from the timed one")
         pk = prof["kernels"]["propagate"]
-        bytes_alg = sum(survey_bytes_per_child(c.model) * r.expanded for c, r in zip(cfgs, res_e))
+        bytes_alg = sum(algorithmic_bytes_per_child(c.model) * r.expanded for c, r in zip(cfgs, res_e))
         children = sum(r.expanded for r in res_e)
         quad_children = sum(r.expanded for c, r in zip(cfgs, res_e) if c.model == KGMT_MODEL_QUADCOPTER)
         avg_ms = pk["ms"] / max(pk["launches"], 1)
@@ -440,7 +440,7 @@ def main(argv=None):
         pk = prof["kernels"]["propagate"]
         avg_ms = pk["ms"] / max(pk["launches"], 1)
         per_launch = prof["children"] / max(pk["launches"], 1)
-        b_child = survey_bytes_per_child(cfg.model)
+        b_child = algorithmic_bytes_per_child(cfg.model)
         achieved = b_child * prof["children"] / (pk["ms"] * 1e-3) / 1e9
         moved = moved_bytes(cfg.model, prof["children"], prof["flagged"])
         achieved_moved = moved / (pk["ms"] * 1e-3) / 1e9
